@@ -1,0 +1,208 @@
+// C-ABI entry points of librtamd (declared in include/rt_api.h).
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+
+#include <new>
+
+#include "rt_internal.h"
+#include "rt_math.h"
+
+rt_status rt_set_error(rt_ctx* ctx, rt_status s, const char* fmt, ...) {
+  if (ctx) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    ctx->err = buf;
+  }
+  return s;
+}
+
+#define RT_HIP(ctx, call)                                                                           \
+  do {                                                                                              \
+    hipError_t e_ = (call);                                                                         \
+    if (e_ != hipSuccess)                                                                           \
+      return rt_set_error((ctx), RT_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));     \
+  } while (0)
+
+namespace {
+
+size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
+
+bool mesh_ok(const rt_mesh_view* m) {
+  return m && m->vx_dev && m->vy_dev && m->vz_dev && m->idx_dev && m->n > 0 && m->nv > 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rt_version(void) { return "rtamd 0.1 (gfx950)"; }
+
+rt_status rt_ctx_create(int32_t device, rt_ctx** out) {
+  if (!out) return RT_ERR_INVALID;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) return RT_ERR_DEVICE;
+  rt_ctx* c = new (std::nothrow) rt_ctx();
+  if (!c) return RT_ERR_OOM;
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  *out = c;
+  return RT_OK;
+}
+
+void rt_ctx_destroy(rt_ctx* ctx) { delete ctx; }
+
+const char* rt_last_error(const rt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, rt_bvh** out) {
+  rt_bvh* b = new (std::nothrow) rt_bvh();
+  if (!b) return rt_set_error(ctx, RT_ERR_OOM, "host allocation failed");
+  b->ctx = ctx;
+  b->n = n;
+  b->bits = bits;
+  b->passes = bits == 30 ? 4 : 8;
+  b->ntiles = (n + kSortTile - 1) / kSortTile;
+  const size_t ks = bits == 30 ? 4 : 8;
+  const size_t ni = (size_t)(n > 1 ? n - 1 : 1);
+  const size_t nn = 2 * (size_t)n - 1;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    size_t o = off;
+    off += align_up(bytes);
+    return o;
+  };
+  size_t o_tri = take((size_t)n * 6 * 4);
+  size_t o_misc = take(kMiscWords * 4);
+  size_t o_hist = off;  // hist and status are contiguous (cleared together)
+  off += (size_t)b->passes * 256 * 4;
+  size_t o_status = off;
+  off = align_up(off + (size_t)b->passes * b->ntiles * 256 * 4);
+  size_t o_k0 = take((size_t)n * ks), o_k1 = take((size_t)n * ks);
+  size_t o_i0 = take((size_t)n * 4), o_i1 = take((size_t)n * 4);
+  size_t o_par = take(nn * 4), o_l = take(ni * 4), o_r = take(ni * 4), o_f = take(ni * 4);
+  size_t o_nb = take(nn * 6 * 4), o_tn = take(ni * 64), o_lt = take((size_t)n * 48);
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, off);
+  if (e != hipSuccess || !p) {
+    delete b;
+    (void)hipGetLastError();
+    return rt_set_error(ctx, RT_ERR_OOM, "hipMalloc(%zu) failed", off);
+  }
+  char* a = (char*)p;
+  b->arena = p;
+  b->arena_bytes = off;
+  b->tri_box = (float*)(a + o_tri);
+  b->misc = (uint32_t*)(a + o_misc);
+  b->hist = (uint32_t*)(a + o_hist);
+  b->status = (uint32_t*)(a + o_status);
+  b->keys[0] = a + o_k0;
+  b->keys[1] = a + o_k1;
+  b->ids[0] = (uint32_t*)(a + o_i0);
+  b->ids[1] = (uint32_t*)(a + o_i1);
+  b->parent = (int32_t*)(a + o_par);
+  b->left = (int32_t*)(a + o_l);
+  b->right = (int32_t*)(a + o_r);
+  b->flags = (uint32_t*)(a + o_f);
+  b->node_box = (float*)(a + o_nb);
+  b->tnode = (float4*)(a + o_tn);
+  b->ltri = (float4*)(a + o_lt);
+  *out = b;
+  return RT_OK;
+}
+
+rt_status rt_bvh_build(rt_ctx* ctx, const rt_mesh_view* mesh, const rt_build_opts* opts, void* stream,
+                       rt_bvh** out) {
+  if (!ctx || !out) return rt_set_error(ctx, RT_ERR_INVALID, "null context or output");
+  *out = nullptr;
+  if (!mesh_ok(mesh)) return rt_set_error(ctx, RT_ERR_INVALID, "mesh must have n > 0 triangles and positions");
+  int bits = opts ? opts->morton_bits : 30;
+  if (bits != 30 && bits != 63) return rt_set_error(ctx, RT_ERR_INVALID, "morton_bits must be 30 or 63");
+  if (mesh->n >= (1 << 30)) return rt_set_error(ctx, RT_ERR_INVALID, "n must be < 2^30");
+  RT_HIP(ctx, hipSetDevice(ctx->device));
+  rt_bvh* b = nullptr;
+  rt_status st = alloc_bvh(ctx, mesh->n, bits, &b);
+  if (st != RT_OK) return st;
+  hipError_t e = rt_launch_build(b, mesh, (hipStream_t)stream);
+  if (e != hipSuccess) {
+    rt_bvh_destroy(b);
+    return rt_set_error(ctx, RT_ERR_DEVICE, "build launch failed: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return RT_OK;
+}
+
+rt_status rt_bvh_rebuild(rt_bvh* bvh, const rt_mesh_view* mesh, void* stream) {
+  if (!bvh) return RT_ERR_INVALID;
+  if (!mesh_ok(mesh) || mesh->n != bvh->n)
+    return rt_set_error(bvh->ctx, RT_ERR_INVALID, "rebuild needs a mesh with the same triangle count");
+  RT_HIP(bvh->ctx, rt_launch_build(bvh, mesh, (hipStream_t)stream));
+  return RT_OK;
+}
+
+rt_status rt_bvh_refit(rt_bvh* bvh, const rt_mesh_view* mesh, void* stream) {
+  if (!bvh) return RT_ERR_INVALID;
+  if (!mesh_ok(mesh) || mesh->n != bvh->n)
+    return rt_set_error(bvh->ctx, RT_ERR_INVALID, "refit needs a mesh with the same triangle count");
+  RT_HIP(bvh->ctx, rt_launch_refit(bvh, mesh, (hipStream_t)stream));
+  return RT_OK;
+}
+
+rt_status rt_bvh_export(const rt_bvh* bvh, const rt_bvh_export_bufs* dst, void* stream) {
+  if (!bvh || !dst) return RT_ERR_INVALID;
+  RT_HIP(bvh->ctx, rt_launch_export(bvh, dst, (hipStream_t)stream));
+  return RT_OK;
+}
+
+rt_status rt_bvh_scene_bounds(const rt_bvh* bvh, void* stream, double mn[3], double mx[3]) {
+  if (!bvh || !mn || !mx) return RT_ERR_INVALID;
+  uint32_t h[6];
+  RT_HIP(bvh->ctx, hipMemcpyAsync(h, bvh->misc + kMiscBounds, sizeof(h), hipMemcpyDeviceToHost, (hipStream_t)stream));
+  RT_HIP(bvh->ctx, hipStreamSynchronize((hipStream_t)stream));
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = (double)rt::ord2f(h[k]);
+    mx[k] = (double)rt::ord2f(h[3 + k]);
+  }
+  return RT_OK;
+}
+
+int32_t rt_bvh_num_triangles(const rt_bvh* bvh) { return bvh ? bvh->n : 0; }
+
+void rt_bvh_destroy(rt_bvh* bvh) {
+  if (!bvh) return;
+  if (bvh->arena) (void)hipFree(bvh->arena);
+  delete bvh;
+}
+
+rt_status rt_trace(const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_rays* rays, int32_t mode, double tmin,
+                   const rt_hits* hits, void* stream) {
+  if (!bvh || !rays || !hits || !hits->tri_dev) return RT_ERR_INVALID;
+  if (rays->n > 0 && (!rays->orig_dev || !rays->dir_dev))
+    return rt_set_error(bvh->ctx, RT_ERR_INVALID, "null ray arrays");
+  if (mode != RT_TRACE_CLOSEST && mode != RT_TRACE_ANY)
+    return rt_set_error(bvh->ctx, RT_ERR_INVALID, "mode must be RT_TRACE_CLOSEST or RT_TRACE_ANY");
+  RT_HIP(bvh->ctx, rt_launch_trace(bvh, mesh, rays, mode, tmin, hits, (hipStream_t)stream));
+  return RT_OK;
+}
+
+rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_material* mats_dev,
+                    int32_t nmat, const rt_light* light, const rt_camera* cam, const rt_render_opts* opts,
+                    float* rgb_dev, uint64_t* counters_dev, void* stream) {
+  if (!ctx) return RT_ERR_INVALID;
+  if (!bvh || !mesh_ok(mesh) || !mats_dev || nmat <= 0 || !light || !cam || !opts || !rgb_dev)
+    return rt_set_error(ctx, RT_ERR_INVALID, "null argument to rt_render");
+  if (!mesh->nx_dev || !mesh->ny_dev || !mesh->nz_dev)
+    return rt_set_error(ctx, RT_ERR_INVALID, "rt_render needs shading normals");
+  if (cam->width <= 0 || cam->height <= 0 || opts->spp <= 0 || opts->depth < 0)
+    return rt_set_error(ctx, RT_ERR_INVALID, "bad image size, spp or depth");
+  if (mesh->n != bvh->n) return rt_set_error(ctx, RT_ERR_INVALID, "mesh and BVH triangle counts differ");
+  RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, counters_dev,
+                               (hipStream_t)stream));
+  return RT_OK;
+}
+
+}  // extern "C"

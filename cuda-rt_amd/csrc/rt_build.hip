@@ -1,0 +1,558 @@
+// LBVH build for gfx950: fused triangle AABB + scene bounds, SoA Morton
+// keys with an all-digit histogram, onesweep radix sort, Karras hierarchy,
+// bottom-up refit with per-node atomic flags, and the export into the
+// reference's node layout.
+//
+// Reference path: kernels/scene.cu:48-120 (AABBs, bounds),
+// kernels/bvh.cu:13-61 (Morton), :267-273 (Thrust sort_by_key),
+// :71-207 (Karras hierarchy), :209-258 (refit).
+#include <hip/hip_runtime.h>
+
+#include "rt_internal.h"
+#include "rt_math.h"
+
+namespace {
+
+using rt::f2ord;
+using rt::ord2f;
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  const uint32_t lane = __lane_id();
+  return lane == 0 ? 0ull : (~0ull >> (64 - lane));
+}
+
+// ------------------------------------------------------------ init
+// Zero the histogram, lookback status, tile counters and overflow counter;
+// set the scene-bound accumulators to the reference's initial values
+// (+1e9 for the minimum, -1e9 for the maximum, kernels/scene.cu:78,83).
+__global__ void k_init(uint32_t* __restrict__ misc, uint32_t* __restrict__ zero, size_t words) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t k = i; k < words; k += stride) zero[k] = 0u;
+  if (i < kMiscWords) {
+    uint32_t v = 0u;
+    if (i < 3) v = f2ord(1e9f);
+    else if (i < 6) v = f2ord(-1e9f);
+    misc[i] = v;
+  }
+}
+
+// ------------------------------------------------------------ AABB + bounds
+// One thread per triangle (kernels/scene.cu:88-120): box = min/max of the
+// three vertices with fminf/fmaxf, then a wave/workgroup min/max reduction
+// and one ordered-integer atomic per axis per workgroup
+// (kernels/scene.cu:72-85). NaN coordinates are ignored like fminf does.
+__global__ __launch_bounds__(256) void k_aabb_bounds(const float* __restrict__ vx, const float* __restrict__ vy,
+                                                     const float* __restrict__ vz, const int32_t* __restrict__ idx,
+                                                     int n, float* __restrict__ tri_box, uint32_t* __restrict__ misc) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  float acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  if (i < n) {
+    int a = idx[3 * i + 0], b = idx[3 * i + 1], c = idx[3 * i + 2];
+    float ax = vx[a], ay = vy[a], az = vz[a];
+    float bx = vx[b], by = vy[b], bz = vz[b];
+    float cx = vx[c], cy = vy[c], cz = vz[c];
+    float lo0 = fminf(cx, fminf(ax, bx)), lo1 = fminf(cy, fminf(ay, by)), lo2 = fminf(cz, fminf(az, bz));
+    float hi0 = fmaxf(cx, fmaxf(ax, bx)), hi1 = fmaxf(cy, fmaxf(ay, by)), hi2 = fmaxf(cz, fmaxf(az, bz));
+    float* o = tri_box + 6 * (size_t)i;
+    o[0] = lo0; o[1] = lo1; o[2] = lo2; o[3] = hi0; o[4] = hi1; o[5] = hi2;
+    acc[0] = lo0; acc[1] = lo1; acc[2] = lo2; acc[3] = hi0; acc[4] = hi1; acc[5] = hi2;
+    // fminf(inf, NaN) = inf keeps an all-NaN coordinate out of the bounds.
+    acc[0] = fminf(INFINITY, acc[0]); acc[1] = fminf(INFINITY, acc[1]); acc[2] = fminf(INFINITY, acc[2]);
+    acc[3] = fmaxf(-INFINITY, acc[3]); acc[4] = fmaxf(-INFINITY, acc[4]); acc[5] = fmaxf(-INFINITY, acc[5]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) acc[k] = fminf(acc[k], __shfl_xor(acc[k], off));
+#pragma unroll
+    for (int k = 3; k < 6; ++k) acc[k] = fmaxf(acc[k], __shfl_xor(acc[k], off));
+  }
+  __shared__ float s_acc[4][6];
+  int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 6; ++k) s_acc[w][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    int k = threadIdx.x;
+    float v = s_acc[0][k];
+    for (int j = 1; j < 4; ++j) v = (k < 3) ? fminf(v, s_acc[j][k]) : fmaxf(v, s_acc[j][k]);
+    if (k < 3) {
+      if (v < INFINITY) atomicMin(&misc[kMiscBounds + k], f2ord(v));
+    } else {
+      if (v > -INFINITY) atomicMax(&misc[kMiscBounds + k], f2ord(v));
+    }
+  }
+}
+
+// ------------------------------------------------------------ Morton
+__device__ __forceinline__ uint32_t spread10(uint32_t x) {
+  x &= 0x3ffu;
+  x = (x | (x << 16)) & 0x030000ffu;
+  x = (x | (x << 8)) & 0x0300f00fu;
+  x = (x | (x << 4)) & 0x030c30c3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+__device__ __forceinline__ uint64_t spread21(uint64_t x) {
+  x &= 0x1fffffull;
+  x = (x | (x << 32)) & 0x001f00000000ffffull;
+  x = (x | (x << 16)) & 0x001f0000ff0000ffull;
+  x = (x | (x << 8)) & 0x100f00f00f00f00full;
+  x = (x | (x << 4)) & 0x10c30c30c30c30c3ull;
+  x = (x | (x << 2)) & 0x1249249249249249ull;
+  return x;
+}
+
+template <typename K>
+struct KeyTraits;
+template <>
+struct KeyTraits<uint32_t> {
+  static constexpr int kPasses = 4;
+  static constexpr double kScale = 1024.0, kMax = 1023.0;
+  __device__ static uint32_t make(uint32_t qx, uint32_t qy, uint32_t qz) {
+    return spread10(qx) * 4u + spread10(qy) * 2u + spread10(qz);
+  }
+  __device__ static int delta(uint32_t a, uint32_t b, int i, int j) {
+    return a != b ? __clz(a ^ b) : 32 + __clz((uint32_t)(i ^ j));
+  }
+};
+template <>
+struct KeyTraits<uint64_t> {
+  static constexpr int kPasses = 8;
+  static constexpr double kScale = 2097152.0, kMax = 2097151.0;
+  __device__ static uint64_t make(uint32_t qx, uint32_t qy, uint32_t qz) {
+    return (spread21(qx) << 2) | (spread21(qy) << 1) | spread21(qz);
+  }
+  __device__ static int delta(uint64_t a, uint64_t b, int i, int j) {
+    return a != b ? __clzll(a ^ b) : 64 + __clz((uint32_t)(i ^ j));
+  }
+};
+
+// kernels/bvh.cu:22-32,52-58: centroid in FP64, (c - min)/(max - min),
+// scaled, clamped with fmax/fmin (NaN -> 0), rounded to float, truncated.
+// Each workgroup also builds the digit histograms of every sort pass in
+// LDS and adds them to the global histogram once.
+template <typename K>
+__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ tri_box, int n,
+                                                const uint32_t* __restrict__ misc, K* __restrict__ keys,
+                                                uint32_t* __restrict__ hist, int per_block) {
+  constexpr int P = KeyTraits<K>::kPasses;
+  __shared__ uint32_t s_hist[P * 256];
+  for (int k = threadIdx.x; k < P * 256; k += 256) s_hist[k] = 0u;
+  double mn[3], ext[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    mn[k] = (double)ord2f(misc[kMiscBounds + k]);
+    ext[k] = (double)ord2f(misc[kMiscBounds + 3 + k]) - mn[k];
+  }
+  __syncthreads();
+  int begin = blockIdx.x * per_block;
+  int end = min(n, begin + per_block);
+  for (int i = begin + threadIdx.x; i < end; i += 256) {
+    const float* bx = tri_box + 6 * (size_t)i;
+    uint32_t q[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      double c = ((double)bx[k] + (double)bx[3 + k]) * 0.5;
+      double u = (c - mn[k]) / ext[k];
+      float f = (float)fmin(fmax(u * KeyTraits<K>::kScale, 0.0), KeyTraits<K>::kMax);
+      q[k] = (uint32_t)f;
+    }
+    K key = KeyTraits<K>::make(q[0], q[1], q[2]);
+    keys[i] = key;
+#pragma unroll
+    for (int p = 0; p < P; ++p) atomicAdd(&s_hist[p * 256 + (int)((key >> (8 * p)) & 255u)], 1u);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < P * 256; k += 256) {
+    uint32_t v = s_hist[k];
+    if (v) atomicAdd(&hist[k], v);
+  }
+}
+
+// ------------------------------------------------------------ onesweep sort
+// One pass of a stable LSD radix sort over 8-bit digits (Adinets & Merrill,
+// "Onesweep", 2022): tiles are taken in launch order from an atomic
+// counter; each tile ranks its keys with wave64 ballot matching, publishes
+// its per-digit counts, and finds its global offsets by decoupled look-back
+// over the preceding tiles. Keys are staged in LDS in digit order so the
+// global scatter is mostly contiguous. Ties keep input order (stable), as
+// thrust::sort_by_key does (kernels/bvh.cu:272).
+constexpr uint32_t kFlagAgg = 1u << 30;
+constexpr uint32_t kFlagPrefix = 2u << 30;
+constexpr uint32_t kValMask = (1u << 30) - 1u;
+
+// Exclusive scan of one value per thread across a 256-thread workgroup.
+__device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_wsum) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) s_wsum[w] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (int j = 0; j < w; ++j) pre += s_wsum[j];
+  __syncthreads();
+  return pre + x - v;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ iin,
+                                                           K* __restrict__ kout, uint32_t* __restrict__ iout, int n,
+                                                           int shift, const uint32_t* __restrict__ hist,
+                                                           uint32_t* status, uint32_t* tile_ctr) {
+  constexpr int KPT = kSortKeysPerThread;
+  constexpr int NW = kSortThreads / kWave;
+  __shared__ uint32_t s_tile;
+  __shared__ uint32_t s_wcnt[NW][256];
+  __shared__ uint32_t s_lstart[256];
+  __shared__ uint32_t s_gbase[256];
+  __shared__ uint32_t s_wsum[NW];
+  __shared__ K s_keys[kSortTile];
+  __shared__ uint32_t s_ids[kSortTile];
+
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  if (tid == 0) s_tile = atomicAdd(tile_ctr, 1u);
+#pragma unroll
+  for (int j = 0; j < NW; ++j) s_wcnt[j][tid] = 0u;
+  __syncthreads();
+  const int tile = (int)s_tile;
+  const int base = tile * kSortTile;
+  const int wbase = base + w * kWave * KPT;
+
+  K key[KPT];
+  uint32_t id[KPT];
+  uint32_t rank[KPT];
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    int i = wbase + k * kWave + lane;
+    bool valid = i < n;
+    key[k] = valid ? kin[i] : (K)0;
+    id[k] = valid ? (iin ? iin[i] : (uint32_t)i) : 0u;
+  }
+  const uint64_t mlt = lanemask_lt();
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    int i = wbase + k * kWave + lane;
+    bool valid = i < n;
+    uint32_t d = (uint32_t)(key[k] >> shift) & 255u;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      bool bit = (d >> b) & 1u;
+      uint64_t bal = __ballot(bit);
+      m &= bit ? bal : ~bal;
+    }
+    uint32_t r = (uint32_t)__popcll(m & mlt);
+    uint32_t cnt = valid ? s_wcnt[w][d] : 0u;
+    rank[k] = cnt + r;
+    if (valid && r == 0) s_wcnt[w][d] = cnt + (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+
+  // Per digit (thread = digit): wave-exclusive prefixes and tile total.
+  uint32_t tot = 0;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) {
+    uint32_t c = s_wcnt[j][tid];
+    s_wcnt[j][tid] = tot;
+    tot += c;
+  }
+  uint32_t* my = status + (size_t)tile * 256 + tid;
+  __hip_atomic_store(my, (tile == 0 ? kFlagPrefix : kFlagAgg) | tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  uint32_t lstart = block_excl_scan256(tot, s_wsum);
+  uint32_t gex = block_excl_scan256(hist[tid], s_wsum);
+  uint32_t excl = 0;
+  if (tile > 0) {
+    int j = tile - 1;
+    while (true) {
+      uint32_t v = __hip_atomic_load(status + (size_t)j * 256 + tid, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if ((v & ~kValMask) == 0u) {
+        __builtin_amdgcn_s_sleep(1);
+        continue;
+      }
+      excl += v & kValMask;
+      if (v & kFlagPrefix) break;
+      --j;
+    }
+    __hip_atomic_store(my, kFlagPrefix | (excl + tot), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  s_lstart[tid] = lstart;
+  s_gbase[tid] = gex + excl;
+  __syncthreads();
+
+#pragma unroll
+  for (int k = 0; k < KPT; ++k) {
+    int i = wbase + k * kWave + lane;
+    if (i < n) {
+      uint32_t d = (uint32_t)(key[k] >> shift) & 255u;
+      uint32_t pos = s_lstart[d] + s_wcnt[w][d] + rank[k];
+      s_keys[pos] = key[k];
+      s_ids[pos] = id[k];
+    }
+  }
+  __syncthreads();
+  const int cnt = min(kSortTile, n - base);
+  for (int j = tid; j < cnt; j += kSortThreads) {
+    K kk = s_keys[j];
+    uint32_t d = (uint32_t)(kk >> shift) & 255u;
+    uint32_t g = s_gbase[d] + (uint32_t)j - s_lstart[d];
+    kout[g] = kk;
+    iout[g] = s_ids[j];
+  }
+}
+
+// ------------------------------------------------------------ hierarchy
+// kernels/bvh.cu:71-207. One thread per sorted position: leaf setup
+// (node = triangle id, box = the triangle's box, traversal record = its
+// vertices) and, for positions < n-1, the internal node's span (doubling
+// then binary search) and split (bisection), children and parent links.
+template <typename K>
+__device__ __forceinline__ int kdelta(const K* __restrict__ keys, int n, int i, K ki, int j) {
+  if (j < 0 || j >= n) return -1;
+  return KeyTraits<K>::delta(ki, keys[j], i, j);
+}
+template <typename K>
+__device__ __forceinline__ int kdelta2(const K* __restrict__ keys, int n, int i, int j) {
+  int l = min(i, j), r = max(i, j);
+  if (l < 0 || r >= n) return -1;
+  return KeyTraits<K>::delta(keys[l], keys[r], l, r);
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_hierarchy(const K* __restrict__ keys, const uint32_t* __restrict__ ids, int n,
+                                                   const float* __restrict__ tri_box, const float* __restrict__ vx,
+                                                   const float* __restrict__ vy, const float* __restrict__ vz,
+                                                   const int32_t* __restrict__ idx, int32_t* __restrict__ parent,
+                                                   int32_t* __restrict__ left, int32_t* __restrict__ right,
+                                                   uint32_t* __restrict__ flags, float* __restrict__ node_box,
+                                                   float4* __restrict__ ltri) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  {
+    int tri = (int)ids[i];
+    const float* src = tri_box + 6 * (size_t)tri;
+    float* dst = node_box + 6 * (size_t)(n - 1 + i);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst[k] = src[k];
+    int a = idx[3 * tri + 0], b = idx[3 * tri + 1], c = idx[3 * tri + 2];
+    float4* L = ltri + 3 * (size_t)i;
+    L[0] = make_float4(vx[a], vy[a], vz[a], vx[b]);
+    L[1] = make_float4(vy[b], vz[b], vx[c], vy[c]);
+    L[2] = make_float4(vz[c], __int_as_float(tri), 0.f, 0.f);
+  }
+  if (i == 0) parent[0] = -1;
+  if (i >= n - 1) return;
+  flags[i] = 0u;
+  const K ki = keys[i];
+  // FindSpan (kernels/bvh.cu:100-133)
+  int dr = kdelta<K>(keys, n, i, ki, i + 1), dl = kdelta<K>(keys, n, i, ki, i - 1);
+  int d = (dr - dl) > 0 ? 1 : ((dr - dl) < 0 ? -1 : 0);
+  int dmin = kdelta<K>(keys, n, i, ki, i - d);
+  int lmax = 2;
+  while (kdelta<K>(keys, n, i, ki, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0, t = lmax;
+  do {
+    t /= 2;
+    if (kdelta<K>(keys, n, i, ki, i + (l + t) * d) > dmin) l += t;
+  } while (t > 1);
+  int first = min(i, i + l * d), last = max(i, i + l * d);
+  // FindSplit (kernels/bvh.cu:134-162)
+  int common = kdelta2<K>(keys, n, first, last);
+  int lo = first, hi = last;
+  do {
+    int mid = (hi + lo) / 2;
+    if (kdelta2<K>(keys, n, lo, mid) > common)
+      lo = mid;
+    else
+      hi = mid;
+  } while (hi > lo + 1);
+  int s = lo;
+  int c1 = (s == first) ? (n - 1 + s) : s;
+  int c2 = (s + 1 == last) ? (n - 1 + s + 1) : (s + 1);
+  left[i] = c1;
+  right[i] = c2;
+  parent[c1] = i;
+  parent[c2] = i;
+}
+
+// ------------------------------------------------------------ refit
+// kernels/bvh.cu:209-258 with the missing release/acquire made explicit:
+// each leaf climbs; the first arrival at a node leaves, the second unions
+// the two child boxes (fminf/fmaxf), writes the node box and the node's
+// 64-byte traversal record, and continues until the root.
+__device__ __forceinline__ int child_ref(int c, int n) { return c >= n - 1 ? ~(c - (n - 1)) : c; }
+
+__global__ __launch_bounds__(256) void k_refit(int n, const int32_t* __restrict__ parent,
+                                               const int32_t* __restrict__ left, const int32_t* __restrict__ right,
+                                               uint32_t* flags, float* node_box, float4* __restrict__ tnode) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n || n < 2) return;
+  int idx = n - 1 + i;
+  float b[6];
+  {
+    const float* src = node_box + 6 * (size_t)idx;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) b[k] = src[k];
+  }
+  while (true) {
+    int p = parent[idx];
+    uint32_t old = __hip_atomic_fetch_add(&flags[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0u) break;
+    int l = left[p], r = right[p];
+    int sib = (l == idx) ? r : l;
+    float o[6];
+    const float* src = node_box + 6 * (size_t)sib;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o[k] = src[k];
+    const float* lb = (l == idx) ? b : o;
+    const float* rb = (l == idx) ? o : b;
+    float nb[6];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      nb[k] = fminf(lb[k], rb[k]);
+      nb[3 + k] = fmaxf(lb[3 + k], rb[3 + k]);
+    }
+    float* dst = node_box + 6 * (size_t)p;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst[k] = nb[k];
+    float4* T = tnode + 4 * (size_t)p;
+    T[0] = make_float4(lb[0], lb[1], lb[2], lb[3]);
+    T[1] = make_float4(lb[4], lb[5], rb[0], rb[1]);
+    T[2] = make_float4(rb[2], rb[3], rb[4], rb[5]);
+    T[3] = make_float4(__int_as_float(child_ref(l, n)), __int_as_float(child_ref(r, n)), 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) b[k] = nb[k];
+    idx = p;
+    if (p == 0) break;
+  }
+}
+
+// Triangle boxes for a refit: same arithmetic as k_aabb_bounds without the
+// scene bounds; the leaf copies and leaf records are refreshed in place.
+__global__ __launch_bounds__(256) void k_refit_leaves(const float* __restrict__ vx, const float* __restrict__ vy,
+                                                      const float* __restrict__ vz, const int32_t* __restrict__ idx,
+                                                      const uint32_t* __restrict__ ids, int n,
+                                                      float* __restrict__ tri_box, float* __restrict__ node_box,
+                                                      float4* __restrict__ ltri, uint32_t* __restrict__ flags) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  if (i < n - 1) flags[i] = 0u;
+  int tri = (int)ids[i];
+  int a = idx[3 * tri + 0], b = idx[3 * tri + 1], c = idx[3 * tri + 2];
+  float ax = vx[a], ay = vy[a], az = vz[a];
+  float bx = vx[b], by = vy[b], bz = vz[b];
+  float cx = vx[c], cy = vy[c], cz = vz[c];
+  float bb[6] = {fminf(cx, fminf(ax, bx)), fminf(cy, fminf(ay, by)), fminf(cz, fminf(az, bz)),
+                 fmaxf(cx, fmaxf(ax, bx)), fmaxf(cy, fmaxf(ay, by)), fmaxf(cz, fmaxf(az, bz))};
+  float* t = tri_box + 6 * (size_t)tri;
+  float* d = node_box + 6 * (size_t)(n - 1 + i);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    t[k] = bb[k];
+    d[k] = bb[k];
+  }
+  float4* L = ltri + 3 * (size_t)i;
+  L[0] = make_float4(ax, ay, az, bx);
+  L[1] = make_float4(by, bz, cx, cy);
+  L[2] = make_float4(cz, __int_as_float(tri), 0.f, 0.f);
+}
+
+// ------------------------------------------------------------ export
+template <typename K>
+__global__ __launch_bounds__(256) void k_export(int n, const K* __restrict__ keys, const uint32_t* __restrict__ ids,
+                                                const int32_t* __restrict__ parent, const int32_t* __restrict__ left,
+                                                const int32_t* __restrict__ right, const float* __restrict__ node_box,
+                                                const float* __restrict__ tri_box, K* codes_out, uint32_t* ids_out,
+                                                int32_t* parent_out, int32_t* left_out, int32_t* right_out,
+                                                float* lo_out, float* hi_out, float* tlo_out, float* thi_out) {
+  int i = blockIdx.x * 256 + threadIdx.x;
+  int total = 2 * n - 1;
+  if (i >= total) return;
+  if (i < n) {
+    if (codes_out) codes_out[i] = keys[i];
+    if (ids_out) ids_out[i] = ids[i];
+    if (tlo_out) {
+      for (int k = 0; k < 3; ++k) tlo_out[3 * i + k] = tri_box[6 * (size_t)i + k];
+    }
+    if (thi_out) {
+      for (int k = 0; k < 3; ++k) thi_out[3 * i + k] = tri_box[6 * (size_t)i + 3 + k];
+    }
+  }
+  if (parent_out) parent_out[i] = parent[i];
+  int l, r;
+  if (i < n - 1) {
+    l = left[i];
+    r = right[i];
+  } else {
+    l = r = (int)ids[i - (n - 1)];
+  }
+  if (left_out) left_out[i] = l;
+  if (right_out) right_out[i] = r;
+  if (lo_out)
+    for (int k = 0; k < 3; ++k) lo_out[3 * i + k] = node_box[6 * (size_t)i + k];
+  if (hi_out)
+    for (int k = 0; k < 3; ++k) hi_out[3 * i + k] = node_box[6 * (size_t)i + 3 + k];
+}
+
+inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+template <typename K>
+hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
+  const int n = b->n;
+  constexpr int P = KeyTraits<K>::kPasses;
+  size_t zero_words = (size_t)P * 256 + (size_t)P * b->ntiles * 256;  // hist + status are contiguous
+  k_init<<<min(cdiv((long)zero_words, 256), 2048), 256, 0, s>>>(b->misc, b->hist, zero_words);
+  k_aabb_bounds<<<cdiv(n, 256), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box, b->misc);
+  int per_block = 4096;
+  k_morton<K><<<cdiv(n, per_block), 256, 0, s>>>(b->tri_box, n, b->misc, (K*)b->keys[0], b->hist, per_block);
+  for (int p = 0; p < P; ++p) {
+    int src = p & 1, dst = src ^ 1;
+    k_onesweep<K><<<b->ntiles, kSortThreads, 0, s>>>(
+        (const K*)b->keys[src], p == 0 ? nullptr : b->ids[src], (K*)b->keys[dst], b->ids[dst], n, 8 * p,
+        b->hist + 256 * p, b->status + (size_t)p * b->ntiles * 256, b->misc + kMiscTileCtr + p);
+  }
+  // P is even: the sorted keys and ids are back in buffer 0.
+  k_hierarchy<K><<<cdiv(n, 256), 256, 0, s>>>((const K*)b->keys[0], b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev,
+                                              m->vz_dev, m->idx_dev, b->parent, b->left, b->right, b->flags,
+                                              b->node_box, b->ltri);
+  if (n > 1)
+    k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t rt_launch_build(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
+  return b->bits == 30 ? build_typed<uint32_t>(b, m, s) : build_typed<uint64_t>(b, m, s);
+}
+
+hipError_t rt_launch_refit(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
+  const int n = b->n;
+  k_refit_leaves<<<cdiv(n, 256), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->ids[0], n,
+                                                b->tri_box, b->node_box, b->ltri, b->flags);
+  if (n > 1)
+    k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
+  return hipGetLastError();
+}
+
+hipError_t rt_launch_export(const rt_bvh* b, const rt_bvh_export_bufs* d, hipStream_t s) {
+  const int n = b->n;
+  if (b->bits == 30)
+    k_export<uint32_t><<<cdiv(2L * n - 1, 256), 256, 0, s>>>(
+        n, (const uint32_t*)b->keys[0], b->ids[0], b->parent, b->left, b->right, b->node_box, b->tri_box,
+        (uint32_t*)d->codes_dev, d->ids_dev, d->parent_dev, d->left_dev, d->right_dev, d->lo_dev, d->hi_dev,
+        d->tri_lo_dev, d->tri_hi_dev);
+  else
+    k_export<uint64_t><<<cdiv(2L * n - 1, 256), 256, 0, s>>>(
+        n, (const uint64_t*)b->keys[0], b->ids[0], b->parent, b->left, b->right, b->node_box, b->tri_box,
+        (uint64_t*)d->codes_dev, d->ids_dev, d->parent_dev, d->left_dev, d->right_dev, d->lo_dev, d->hi_dev,
+        d->tri_lo_dev, d->tri_hi_dev);
+  return hipGetLastError();
+}

@@ -1,0 +1,83 @@
+// Internal types shared by the HIP translation units of librtamd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/rt_api.h"
+
+struct rt_ctx {
+  int32_t device = 0;
+  int32_t num_cus = 256;
+  std::string err;
+};
+
+// Device-side view of a built BVH, passed by value to the trace kernels.
+// Node numbering follows the reference (inc/bvh.cuh:25-28): internal
+// nodes [0, n-2], leaf of sorted position p at n-1+p.
+struct BvhView {
+  int32_t n;
+  const float* node_box;   // (2n-1)*6: lo xyz, hi xyz
+  const float4* tnode;     // (n-1)*4: child boxes and child refs (64 B per internal node)
+  const float4* ltri;      // n*3: vertices a, b, c of the leaf's triangle + triangle id (48 B)
+};
+
+// Traversal record of internal node i (64 B, four float4):
+//   [0] = left.lo.xyz,  left.hi.x
+//   [1] = left.hi.yz,   right.lo.xy
+//   [2] = right.lo.z,   right.hi.xyz
+//   [3] = left ref, right ref (int bits), 0, 0
+// A ref >= 0 is an internal node; a ref < 0 is ~(sorted leaf position).
+// Leaf record p (48 B): [0] = a.xyz, b.x; [1] = b.yz, c.xy; [2] = c.z, tri id bits, 0, 0.
+
+struct rt_bvh {
+  rt_ctx* ctx = nullptr;
+  int32_t n = 0;
+  int32_t bits = 30;
+  int32_t passes = 4;
+  int32_t ntiles = 0;
+  // one device arena; the pointers below point into it
+  void* arena = nullptr;
+  size_t arena_bytes = 0;
+  float* tri_box = nullptr;      // n*6
+  uint32_t* misc = nullptr;      // [0..5] ordered-int scene bounds, [8..8+passes) tile counters, [32..] overflow counter
+  uint32_t* hist = nullptr;      // passes*256
+  uint32_t* status = nullptr;    // passes*ntiles*256
+  void* keys[2] = {nullptr, nullptr};
+  uint32_t* ids[2] = {nullptr, nullptr};
+  int32_t* parent = nullptr;     // 2n-1
+  int32_t* left = nullptr;       // max(n-1,1)
+  int32_t* right = nullptr;      // max(n-1,1)
+  uint32_t* flags = nullptr;     // max(n-1,1)
+  float* node_box = nullptr;     // (2n-1)*6
+  float4* tnode = nullptr;       // (n-1)*4
+  float4* ltri = nullptr;        // n*3
+
+  BvhView view() const { return BvhView{n, node_box, tnode, ltri}; }
+};
+
+// misc[] slots
+constexpr int kMiscBounds = 0;      // 6 words
+constexpr int kMiscTileCtr = 8;     // up to 8 words
+constexpr int kMiscOverflow = 32;   // 1 word: traversal stack overflows
+constexpr int kMiscWords = 64;
+
+// Sort tiling.
+constexpr int kSortThreads = 256;
+constexpr int kSortKeysPerThread = 16;
+constexpr int kSortTile = kSortThreads * kSortKeysPerThread;
+
+rt_status rt_set_error(rt_ctx* ctx, rt_status s, const char* fmt, ...);
+
+// Launchers (rt_build.hip).
+hipError_t rt_launch_build(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
+hipError_t rt_launch_refit(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
+hipError_t rt_launch_export(const rt_bvh* b, const rt_bvh_export_bufs* dst, hipStream_t s);
+
+// Launchers (rt_trace.hip).
+hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays* rays, int32_t mode, double tmin,
+                           const rt_hits* hits, hipStream_t s);
+hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
+                            const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
+                            uint64_t* counters, hipStream_t s);

@@ -1,0 +1,168 @@
+/* MI355X (gfx950) ray-tracer core: C-ABI boundary.
+ *
+ * This is the drop-in boundary for the reference's build / refit / trace
+ * path (mbartling/cuda-rt). Each entry point names the reference interface
+ * it replaces. All pointers named *_dev are device pointers (HBM); the
+ * caller owns them. The library owns every BVH buffer until
+ * rt_bvh_destroy. Calls are ordered on the given HIP stream (NULL = the
+ * null stream) and do not synchronise unless stated. Errors are returned
+ * as rt_status codes; rt_last_error() gives the message. The library never
+ * exits the process.
+ */
+#ifndef RT_API_H
+#define RT_API_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  RT_OK = 0,
+  RT_ERR_INVALID = 1,     /* bad argument (null pointer, size, option) */
+  RT_ERR_OOM = 2,         /* device allocation failed */
+  RT_ERR_DEVICE = 3,      /* HIP runtime error */
+  RT_ERR_UNSUPPORTED = 4, /* option not implemented */
+  RT_ERR_OVERFLOW = 5     /* traversal stack overflow was recorded */
+} rt_status;
+
+typedef struct rt_ctx rt_ctx;
+typedef struct rt_bvh rt_bvh;
+
+/* Triangle mesh, fp32 SoA positions + int32 vertex indices.
+ * Replaces the Vec3d vertices / normals / TriangleIndices arrays that
+ * Scene_d uploads (src/scene.cpp:82-138, inc/scene.h:60-117). */
+typedef struct {
+  const float *vx_dev, *vy_dev, *vz_dev; /* nv positions */
+  const float *nx_dev, *ny_dev, *nz_dev; /* nv shading normals (render only; may be NULL for build/trace) */
+  const int32_t *idx_dev;                /* 3*n vertex indices (a, b, c) */
+  const int32_t *mat_id_dev;             /* n material ids, or NULL for material 0 */
+  int32_t nv;
+  int32_t n;
+} rt_mesh_view;
+
+enum { RT_BUILD_DEFAULT = 0 };
+
+typedef struct {
+  int32_t morton_bits; /* 30 (reference, kernels/bvh.cu:22-32) or 63 */
+  int32_t flags;       /* RT_BUILD_* */
+} rt_build_opts;
+
+/* Destination buffers for rt_bvh_export, all device pointers, in the
+ * reference's layout (inc/bvh.cuh:18-28, kernels/bvh.cu:164-258):
+ * nodes [0, n-2] are internal, [n-1, 2n-2] are leaves with
+ * left == right == triangle id; the root's parent is -1. */
+typedef struct {
+  void *codes_dev;      /* n sorted Morton keys: uint32 (30-bit) or uint64 (63-bit) */
+  uint32_t *ids_dev;    /* n sorted triangle ids (object_ids) */
+  int32_t *parent_dev;  /* 2n-1 */
+  int32_t *left_dev;    /* 2n-1 */
+  int32_t *right_dev;   /* 2n-1 */
+  float *lo_dev;        /* (2n-1)*3 node AABB minimum, xyz per node (sortedBBoxs) */
+  float *hi_dev;        /* (2n-1)*3 node AABB maximum */
+  float *tri_lo_dev;    /* n*3 per-triangle AABB minimum (BBoxs), may be NULL */
+  float *tri_hi_dev;    /* n*3, may be NULL */
+} rt_bvh_export_bufs;
+
+/* Rays for rt_trace: AoS xyz doubles (the reference's ray, inc/ray.h:20-50). */
+typedef struct {
+  const double *orig_dev; /* 3*n */
+  const double *dir_dev;  /* 3*n */
+  int64_t n;
+} rt_rays;
+
+/* Hit records (the reference's isect, inc/isect.h:4-15). tri = -1 on miss.
+ * alpha/beta are the Moller-Trumbore barycentrics of vertices b and c. */
+typedef struct {
+  int32_t *tri_dev;
+  double *t_dev;     /* may be NULL */
+  double *alpha_dev; /* may be NULL */
+  double *beta_dev;  /* may be NULL */
+} rt_hits;
+
+enum {
+  RT_TRACE_CLOSEST = 0, /* BVH_d::intersect, kernels/bvh.cu:382-414 */
+  RT_TRACE_ANY = 1      /* BVH_d::intersectAny, kernels/bvh.cu:416-449 */
+};
+
+/* Material (inc/material.h:9-34). Trans() is true when kt == 0, as in the reference. */
+typedef struct {
+  double ka[3], kd[3], ks[3], kt[3], ke[3];
+  double shininess;
+} rt_material;
+
+/* Positional light (inc/Light.h:10-44, kernels/Light.cu). */
+typedef struct {
+  double position[3];
+  double color[3];
+  double radius;       /* jitter: offset = rand3*radius - 0.5 (kernels/Light.cu:73-75) */
+  double c0, c1, c2;   /* falloff min(1, 1/(c0 + c1 d + c2 d^2)) */
+} rt_light;
+
+/* Camera getters used by the render kernel (inc/camera.h:33-46). */
+typedef struct {
+  int32_t width, height;
+  double fov_deg;      /* getFOV */
+  double fstop;        /* getAperature() = 1/fstop; INFINITY gives a pinhole */
+  double focal_point;  /* getFocalPoint */
+  double eye[3];       /* getEye (used by the specular term) */
+} rt_camera;
+
+enum {
+  RT_RENDER_RANDVEC_RTL = 1, /* draw randVec3d's components z, y, x instead of x, y, z */
+  RT_RENDER_CLAMP = 2        /* clamp each channel to [0,1] (kernels/scene.cu:137) */
+};
+
+typedef struct {
+  int32_t spp;    /* samples per pixel (reference: 20, kernels/raytracer.cu:138) */
+  int32_t depth;  /* diffuse bounces after the first hit; 0 = reference shading */
+  uint32_t seed;  /* XORed into the seed index; 0 = reference seeding */
+  int32_t flags;  /* RT_RENDER_* */
+} rt_render_opts;
+
+/* Library / context. */
+const char *rt_version(void);
+rt_status rt_ctx_create(int32_t device, rt_ctx **out);
+void rt_ctx_destroy(rt_ctx *ctx);
+const char *rt_last_error(const rt_ctx *ctx);
+
+/* Build: Scene_d::computeBoundingBoxes + findMinMax (kernels/scene.cu:48-85)
+ * followed by BVH_d::setUp -> computeMortonCodes -> sortMortonCodes ->
+ * buildTree (src/bvh.cpp:3-32, kernels/bvh.cu:260-292). Allocates the BVH. */
+rt_status rt_bvh_build(rt_ctx *ctx, const rt_mesh_view *mesh, const rt_build_opts *opts, void *stream,
+                       rt_bvh **out);
+/* Rebuild the whole hierarchy for a mesh with the same triangle count into
+ * an existing BVH without allocating (stream-ordered, graph-capturable). */
+rt_status rt_bvh_rebuild(rt_bvh *bvh, const rt_mesh_view *mesh, void *stream);
+/* Refit only: recompute triangle and node AABBs for moved vertices keeping
+ * the topology. The reference runs RefitBounds only inside buildTree
+ * (kernels/bvh.cu:288); this entry point re-zeroes the flags itself. */
+rt_status rt_bvh_refit(rt_bvh *bvh, const rt_mesh_view *mesh, void *stream);
+/* Copy the BVH into the reference layout (for parity checks). */
+rt_status rt_bvh_export(const rt_bvh *bvh, const rt_bvh_export_bufs *dst, void *stream);
+/* Scene bounds mMin / mMax as Scene_d::findMinMax returns them
+ * (kernels/scene.cu:72-85). Synchronises the stream. */
+rt_status rt_bvh_scene_bounds(const rt_bvh *bvh, void *stream, double mn[3], double mx[3]);
+int32_t rt_bvh_num_triangles(const rt_bvh *bvh);
+void rt_bvh_destroy(rt_bvh *bvh);
+
+/* Trace: BVH_d::intersect / intersectAny (kernels/bvh.cu:297-449).
+ * tmin = 0 keeps the reference rule (every hit with |t| >= 1e-8 on the whole
+ * line counts); tmin > 0 only accepts t >= tmin. */
+rt_status rt_trace(const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_rays *rays, int32_t mode, double tmin,
+                   const rt_hits *hits, void *stream);
+
+/* Render: runRayTracerKernelRec + traceRay + Material::shade +
+ * Light::{getDirection, distanceAttenuation, shadowAttenuation}
+ * (kernels/raytracer.cu:94-249, kernels/material.cu:6-40, kernels/Light.cu).
+ * rgb_dev receives width*height*3 floats, row-flipped like the reference
+ * image (kernels/raytracer.cu:106). counters_dev, if not NULL, receives
+ * 3 uint64 counts added atomically: primary, shadow and bounce rays. */
+rt_status rt_render(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_material *mats_dev,
+                    int32_t nmat, const rt_light *light, const rt_camera *cam, const rt_render_opts *opts,
+                    float *rgb_dev, uint64_t *counters_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

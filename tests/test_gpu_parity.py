@@ -1,0 +1,180 @@
+"""GPU parity: librtamd (HIP, gfx950) against the FP64 CPU oracle.
+
+Build outputs (Morton codes, sorted ids, topology, parent links, node and
+triangle AABBs, scene bounds) must match bit for bit; hit triangle ids
+must match exactly with t / barycentrics within 1e-5 relative; rendered
+images within 1e-4 per channel.
+"""
+import numpy as np
+import pytest
+
+import rtamd
+from oracle import oracle as O
+from tests import scenes
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return rtamd.Context(0)
+
+
+def _compare_build(got, ref, hm):
+    assert np.array_equal(got["codes"], ref["codes"]), "morton codes / sort order differ"
+    assert np.array_equal(got["ids"], ref["ids"]), "sorted ids differ"
+    assert np.array_equal(got["parent"][1:], ref["parent"][1:]), "parent links differ"
+    assert got["parent"][0] == -1
+    assert np.array_equal(got["left"], ref["left"]), "left children differ"
+    assert np.array_equal(got["right"], ref["right"]), "right children differ"
+    assert np.array_equal(got["lo"].view(np.uint32), ref["lo"].view(np.uint32)), "node AABB min differ"
+    assert np.array_equal(got["hi"].view(np.uint32), ref["hi"].view(np.uint32)), "node AABB max differ"
+    tlo, thi = O.tri_aabb(hm)
+    assert np.array_equal(got["tri_lo"].view(np.uint32), tlo.view(np.uint32))
+    assert np.array_equal(got["tri_hi"].view(np.uint32), thi.view(np.uint32))
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+@pytest.mark.parametrize("scene", ["c1", "soup", "displaced"])
+def test_build_parity(ctx, bits, scene):
+    hm = {"c1": lambda: scenes.c1_mesh(32), "soup": lambda: scenes.soup_mesh(150_000),
+          "displaced": lambda: scenes.displaced_mesh(60)}[scene]()
+    _say(f"build parity scene={scene} bits={bits} n={hm.n}")
+    dm = rtamd.DeviceMesh(hm)
+    b = rtamd.BVH(ctx, dm, bits)
+    got = b.export()
+    ref = O.build(hm, bits)
+    _compare_build(got, ref, hm)
+    mn, mx = b.scene_bounds()
+    assert np.array_equal(mn, ref["mn"]) and np.array_equal(mx, ref["mx"])
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 4095, 4096, 4097, 70_000])
+def test_build_sizes(ctx, n):
+    _say(f"build sizes n={n}")
+    hm = scenes.soup_mesh(n, seed=1234 + n)
+    dm = rtamd.DeviceMesh(hm)
+    for bits in (30, 63):
+        b = rtamd.BVH(ctx, dm, bits)
+        got = b.export()
+        ref = O.build(hm, bits)
+        _compare_build(got, ref, hm)
+
+
+def test_build_duplicate_keys(ctx):
+    """Every triangle in a tiny region: 30-bit codes collide, the index fallback orders the tree."""
+    _say("build duplicate keys")
+    hm = rtamd.gen_soup(20_000, 77, edge_median=1e-6, offset=(0.25, 0.25, 0.25), scale=1e-5)
+    dm = rtamd.DeviceMesh(hm)
+    for bits in (30, 63):
+        b = rtamd.BVH(ctx, dm, bits)
+        got = b.export()
+        ref = O.build(hm, bits)
+        assert len(np.unique(ref["codes"])) < hm.n
+        _compare_build(got, ref, hm)
+
+
+def test_build_flat_axis(ctx):
+    """All triangles in the plane z = -1: (c - min)/0 is NaN and clamps to 0."""
+    _say("build flat axis")
+    hm = scenes.soup_mesh(5000)
+    hm.vz[:] = -1.0
+    dm = rtamd.DeviceMesh(hm)
+    for bits in (30, 63):
+        b = rtamd.BVH(ctx, dm, bits)
+        _compare_build(b.export(), O.build(hm, bits), hm)
+
+
+def _trace_compare(ctx, hm, bits, orig, dirs):
+    dm = rtamd.DeviceMesh(hm)
+    b = rtamd.BVH(ctx, dm, bits)
+    ref_b = O.build(hm, bits)
+    o_t = torch.from_numpy(orig).cuda()
+    d_t = torch.from_numpy(dirs).cuda()
+    for any_hit in (False, True):
+        tri, t, a, be = b.trace(dm, o_t, d_t, rtamd.RT_TRACE_ANY if any_hit else rtamd.RT_TRACE_CLOSEST)
+        torch.cuda.synchronize()
+        tri, t, a, be = [x.cpu().numpy() for x in (tri, t, a, be)]
+        rtri, rt_, ra, rb, _, _ = O.trace(hm, ref_b, orig, dirs, any_hit=any_hit)
+        assert np.array_equal(tri, rtri), f"hit ids differ (any={any_hit}): {np.sum(tri != rtri)} rays"
+        h = rtri >= 0
+        np.testing.assert_allclose(t[h], rt_[h], rtol=1e-5)
+        np.testing.assert_allclose(a[h], ra[h], rtol=1e-5, atol=1e-12)
+        np.testing.assert_allclose(be[h], rb[h], rtol=1e-5, atol=1e-12)
+    return int(h.sum())
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_trace_parity_c1(ctx, bits):
+    _say(f"trace parity c1 bits={bits}")
+    hm = scenes.c1_mesh(32)
+    orig, dirs = scenes.random_rays(20_000, 5)
+    hits = _trace_compare(ctx, hm, bits, orig, dirs)
+    assert hits > 1000
+
+
+def test_trace_parity_soup(ctx):
+    _say("trace parity soup")
+    hm = scenes.soup_mesh(100_000)
+    orig, dirs = scenes.random_rays(20_000, 6, spread=0.4)
+    hits = _trace_compare(ctx, hm, 63, orig, dirs)
+    assert hits > 1000
+
+
+def test_trace_from_inside(ctx):
+    """Origins inside the soup: negative-t hits count under the reference rule."""
+    _say("trace from inside")
+    hm = scenes.soup_mesh(50_000)
+    rng = np.random.default_rng(9)
+    orig = np.stack([rng.uniform(-.5, .5, 5000), rng.uniform(-.5, .5, 5000), rng.uniform(-2, -1, 5000)], 1)
+    dirs = rng.normal(size=(5000, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    _trace_compare(ctx, hm, 63, orig, dirs)
+
+
+def _render_compare(ctx, hm, bits, mats, light, cam, opts, pixels=None):
+    dm = rtamd.DeviceMesh(hm)
+    b = rtamd.BVH(ctx, dm, bits)
+    md = rtamd.materials_tensor(mats)
+    cnt = torch.zeros(3, dtype=torch.int64, device="cuda")
+    rgb = rtamd.render(ctx, b, dm, md, len(mats), light, cam, opts, counters=cnt)
+    torch.cuda.synchronize()
+    got = rgb.cpu().numpy()
+    ref_b = O.build(hm, bits)
+    ref, counts = O.render(hm, ref_b, mats, light, cam, opts)
+    diff = np.abs(got - ref)
+    assert diff.max() <= 1e-4, f"max pixel diff {diff.max()} at {np.unravel_index(diff.argmax(), diff.shape)}"
+    c = cnt.cpu().numpy()
+    assert c[0] == counts[0] and c[1] == counts[1] and c[2] == counts[2], (c, counts)
+    return got, diff
+
+
+def test_render_parity_c1(ctx):
+    _say("render parity c1 128x128 x2spp")
+    hm = scenes.c1_mesh(32)
+    mats, light, cam, opts = scenes.c1_setup(128, 128, 2)
+    got, diff = _render_compare(ctx, hm, 30, mats, light, cam, opts)
+    assert got.max() > 0.1
+
+
+def test_render_parity_thinlens_soup(ctx):
+    _say("render parity thin-lens soup 96x64 x3spp")
+    hm = scenes.soup_mesh(60_000)
+    mats, light, cam, opts = scenes.c2_setup(96, 64, 3)
+    _render_compare(ctx, hm, 63, mats, light, cam, opts)
+
+
+def test_render_parity_bounces(ctx):
+    _say("render parity bounces depth 3")
+    hm = scenes.displaced_mesh(20)
+    mats, light, cam, opts = scenes.c1_setup(64, 64, 2)
+    opts.depth = 3
+    _render_compare(ctx, hm, 30, mats, light, cam, opts)
