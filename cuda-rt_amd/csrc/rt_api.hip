@@ -59,12 +59,13 @@ void rt_ctx_destroy(rt_ctx* ctx) { delete ctx; }
 
 const char* rt_last_error(const rt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, rt_bvh** out) {
+static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, rt_bvh** out) {
   rt_bvh* b = new (std::nothrow) rt_bvh();
   if (!b) return rt_set_error(ctx, RT_ERR_OOM, "host allocation failed");
   b->ctx = ctx;
   b->n = n;
   b->bits = bits;
+  b->build_flags = flags;
   b->passes = bits == 30 ? 4 : 8;
   b->ntiles = (n + kSortTile - 1) / kSortTile;
   const size_t ks = bits == 30 ? 4 : 8;
@@ -78,10 +79,13 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, rt_bvh** out) {
   };
   size_t o_tri = take((size_t)n * 6 * 4);
   size_t o_misc = take(kMiscWords * 4);
-  size_t o_hist = off;  // hist and status are contiguous (cleared together)
+  size_t o_hist = off;  // hist, status and arrivals are contiguous (cleared together)
   off += (size_t)b->passes * 256 * 4;
   size_t o_status = off;
-  off = align_up(off + (size_t)b->passes * b->ntiles * 256 * 4);
+  off += (size_t)b->passes * b->ntiles * 256 * 4;
+  size_t o_arr = off;
+  off = align_up(off + ni * 4);
+  size_t o_dep = take(ni * 64), o_def = take((size_t)n * 4);
   size_t o_k0 = take((size_t)n * ks), o_k1 = take((size_t)n * ks);
   size_t o_i0 = take((size_t)n * 4), o_i1 = take((size_t)n * 4);
   size_t o_par = take(nn * 4), o_l = take(ni * 4), o_r = take(ni * 4), o_f = take(ni * 4);
@@ -100,6 +104,9 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, rt_bvh** out) {
   b->misc = (uint32_t*)(a + o_misc);
   b->hist = (uint32_t*)(a + o_hist);
   b->status = (uint32_t*)(a + o_status);
+  b->arrivals = (uint32_t*)(a + o_arr);
+  b->deposits = (float4*)(a + o_dep);
+  b->deferred = (uint32_t*)(a + o_def);
   b->keys[0] = a + o_k0;
   b->keys[1] = a + o_k1;
   b->ids[0] = (uint32_t*)(a + o_i0);
@@ -111,6 +118,13 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, rt_bvh** out) {
   b->node_box = (float*)(a + o_nb);
   b->tnode = (float4*)(a + o_tn);
   b->ltri = (float4*)(a + o_lt);
+  // The diagnostic counters in misc[] accumulate over the BVH's lifetime.
+  e = hipMemset(b->misc, 0, kMiscWords * sizeof(uint32_t));
+  if (e != hipSuccess) {
+    (void)hipFree(p);
+    delete b;
+    return rt_set_error(ctx, RT_ERR_DEVICE, "hipMemset failed: %s", hipGetErrorString(e));
+  }
   *out = b;
   return RT_OK;
 }
@@ -125,7 +139,9 @@ rt_status rt_bvh_build(rt_ctx* ctx, const rt_mesh_view* mesh, const rt_build_opt
   if (mesh->n >= (1 << 30)) return rt_set_error(ctx, RT_ERR_INVALID, "n must be < 2^30");
   RT_HIP(ctx, hipSetDevice(ctx->device));
   rt_bvh* b = nullptr;
-  rt_status st = alloc_bvh(ctx, mesh->n, bits, &b);
+  int flags = opts ? opts->flags : 0;
+  if (flags & ~RT_BUILD_KARRAS_SEARCH) return rt_set_error(ctx, RT_ERR_INVALID, "unknown build flags");
+  rt_status st = alloc_bvh(ctx, mesh->n, bits, flags, &b);
   if (st != RT_OK) return st;
   hipError_t e = rt_launch_build(b, mesh, (hipStream_t)stream);
   if (e != hipSuccess) {
@@ -160,17 +176,32 @@ rt_status rt_bvh_export(const rt_bvh* bvh, const rt_bvh_export_bufs* dst, void* 
 
 rt_status rt_bvh_scene_bounds(const rt_bvh* bvh, void* stream, double mn[3], double mx[3]) {
   if (!bvh || !mn || !mx) return RT_ERR_INVALID;
-  uint32_t h[6];
+  uint32_t h[6 * kBoundsReplicas];
   RT_HIP(bvh->ctx, hipMemcpyAsync(h, bvh->misc + kMiscBounds, sizeof(h), hipMemcpyDeviceToHost, (hipStream_t)stream));
   RT_HIP(bvh->ctx, hipStreamSynchronize((hipStream_t)stream));
   for (int k = 0; k < 3; ++k) {
-    mn[k] = (double)rt::ord2f(h[k]);
-    mx[k] = (double)rt::ord2f(h[3 + k]);
+    uint32_t a = h[k], b = h[3 + k];
+    for (int r = 1; r < kBoundsReplicas; ++r) {
+      a = a < h[6 * r + k] ? a : h[6 * r + k];
+      b = b > h[6 * r + 3 + k] ? b : h[6 * r + 3 + k];
+    }
+    mn[k] = (double)rt::ord2f(a);
+    mx[k] = (double)rt::ord2f(b);
   }
   return RT_OK;
 }
 
 int32_t rt_bvh_num_triangles(const rt_bvh* bvh) { return bvh ? bvh->n : 0; }
+
+rt_status rt_bvh_diagnostics(const rt_bvh* bvh, void* stream, uint32_t out[2]) {
+  if (!bvh || !out) return RT_ERR_INVALID;
+  uint32_t h[kMiscWords];
+  RT_HIP(bvh->ctx, hipMemcpyAsync(h, bvh->misc, sizeof(h), hipMemcpyDeviceToHost, (hipStream_t)stream));
+  RT_HIP(bvh->ctx, hipStreamSynchronize((hipStream_t)stream));
+  out[0] = h[kMiscOverflow];
+  out[1] = h[kMiscBuildError];
+  return RT_OK;
+}
 
 void rt_bvh_destroy(rt_bvh* bvh) {
   if (!bvh) return;

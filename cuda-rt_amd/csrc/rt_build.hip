@@ -24,45 +24,51 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // ------------------------------------------------------------ init
-// Zero the histogram, lookback status, tile counters and overflow counter;
-// set the scene-bound accumulators to the reference's initial values
-// (+1e9 for the minimum, -1e9 for the maximum, kernels/scene.cu:78,83).
-__global__ void k_init(uint32_t* __restrict__ misc, uint32_t* __restrict__ zero, size_t words) {
+// Zero the histogram, look-back status and climb arrival counters (one
+// contiguous range) and the per-build counters; if init_bounds, set the
+// scene-bound accumulators to the reference's initial values (+1e9 for
+// the minimum, -1e9 for the maximum, kernels/scene.cu:78,83).
+__global__ void k_init(uint32_t* __restrict__ misc, uint32_t* __restrict__ zero, size_t words, int init_bounds) {
   size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t k = i; k < words; k += stride) zero[k] = 0u;
   if (i < kMiscWords) {
-    uint32_t v = 0u;
-    if (i < 3) v = f2ord(1e9f);
-    else if (i < 6) v = f2ord(-1e9f);
-    misc[i] = v;
+    if (i < kMiscBounds + 6 * kBoundsReplicas) {
+      if (init_bounds) misc[i] = (i % 6) < 3 ? f2ord(1e9f) : f2ord(-1e9f);
+    } else if (i != kMiscOverflow && i != kMiscBuildError) {
+      misc[i] = 0u;
+    }
   }
 }
 
 // ------------------------------------------------------------ AABB + bounds
-// One thread per triangle (kernels/scene.cu:88-120): box = min/max of the
-// three vertices with fminf/fmaxf, then a wave/workgroup min/max reduction
-// and one ordered-integer atomic per axis per workgroup
-// (kernels/scene.cu:72-85). NaN coordinates are ignored like fminf does.
+// Per-triangle box (kernels/scene.cu:88-120): min/max of the three vertices
+// with fminf/fmaxf. If with_bounds, each workgroup also reduces its boxes
+// (wave shuffles, then LDS) and adds one ordered-integer atomic per axis to
+// one of kBoundsReplicas accumulator sets (kernels/scene.cu:72-85). NaN
+// coordinates are ignored, as fminf/fmaxf ignore them.
 __global__ __launch_bounds__(256) void k_aabb_bounds(const float* __restrict__ vx, const float* __restrict__ vy,
                                                      const float* __restrict__ vz, const int32_t* __restrict__ idx,
-                                                     int n, float* __restrict__ tri_box, uint32_t* __restrict__ misc) {
-  int i = blockIdx.x * 256 + threadIdx.x;
+                                                     int n, float* __restrict__ tri_box, uint32_t* __restrict__ misc,
+                                                     int with_bounds) {
   float acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  if (i < n) {
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     int a = idx[3 * i + 0], b = idx[3 * i + 1], c = idx[3 * i + 2];
     float ax = vx[a], ay = vy[a], az = vz[a];
     float bx = vx[b], by = vy[b], bz = vz[b];
     float cx = vx[c], cy = vy[c], cz = vz[c];
-    float lo0 = fminf(cx, fminf(ax, bx)), lo1 = fminf(cy, fminf(ay, by)), lo2 = fminf(cz, fminf(az, bz));
-    float hi0 = fmaxf(cx, fmaxf(ax, bx)), hi1 = fmaxf(cy, fmaxf(ay, by)), hi2 = fmaxf(cz, fmaxf(az, bz));
+    float bb[6] = {fminf(cx, fminf(ax, bx)), fminf(cy, fminf(ay, by)), fminf(cz, fminf(az, bz)),
+                   fmaxf(cx, fmaxf(ax, bx)), fmaxf(cy, fmaxf(ay, by)), fmaxf(cz, fmaxf(az, bz))};
     float* o = tri_box + 6 * (size_t)i;
-    o[0] = lo0; o[1] = lo1; o[2] = lo2; o[3] = hi0; o[4] = hi1; o[5] = hi2;
-    acc[0] = lo0; acc[1] = lo1; acc[2] = lo2; acc[3] = hi0; acc[4] = hi1; acc[5] = hi2;
-    // fminf(inf, NaN) = inf keeps an all-NaN coordinate out of the bounds.
-    acc[0] = fminf(INFINITY, acc[0]); acc[1] = fminf(INFINITY, acc[1]); acc[2] = fminf(INFINITY, acc[2]);
-    acc[3] = fmaxf(-INFINITY, acc[3]); acc[4] = fmaxf(-INFINITY, acc[4]); acc[5] = fmaxf(-INFINITY, acc[5]);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) o[k] = bb[k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      acc[k] = fminf(acc[k], bb[k]);
+      acc[3 + k] = fmaxf(acc[3 + k], bb[3 + k]);
+    }
   }
+  if (!with_bounds) return;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
@@ -79,11 +85,26 @@ __global__ __launch_bounds__(256) void k_aabb_bounds(const float* __restrict__ v
     int k = threadIdx.x;
     float v = s_acc[0][k];
     for (int j = 1; j < 4; ++j) v = (k < 3) ? fminf(v, s_acc[j][k]) : fmaxf(v, s_acc[j][k]);
+    uint32_t* dst = misc + kMiscBounds + 6 * (blockIdx.x % kBoundsReplicas) + k;
     if (k < 3) {
-      if (v < INFINITY) atomicMin(&misc[kMiscBounds + k], f2ord(v));
+      if (v < INFINITY) atomicMin(dst, f2ord(v));
     } else {
-      if (v > -INFINITY) atomicMax(&misc[kMiscBounds + k], f2ord(v));
+      if (v > -INFINITY) atomicMax(dst, f2ord(v));
     }
+  }
+}
+
+// Scene bounds from the accumulator replicas.
+__device__ __forceinline__ void load_bounds(const uint32_t* __restrict__ misc, double* mn, double* mx) {
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    uint32_t a = misc[kMiscBounds + k], b = misc[kMiscBounds + 3 + k];
+    for (int r = 1; r < kBoundsReplicas; ++r) {
+      a = min(a, misc[kMiscBounds + 6 * r + k]);
+      b = max(b, misc[kMiscBounds + 6 * r + 3 + k]);
+    }
+    mn[k] = (double)ord2f(a);
+    mx[k] = (double)ord2f(b);
   }
 }
 
@@ -142,12 +163,10 @@ __global__ __launch_bounds__(256) void k_morton(const float* __restrict__ tri_bo
   constexpr int P = KeyTraits<K>::kPasses;
   __shared__ uint32_t s_hist[P * 256];
   for (int k = threadIdx.x; k < P * 256; k += 256) s_hist[k] = 0u;
-  double mn[3], ext[3];
+  double mn[3], mx[3], ext[3];
+  load_bounds(misc, mn, mx);
 #pragma unroll
-  for (int k = 0; k < 3; ++k) {
-    mn[k] = (double)ord2f(misc[kMiscBounds + k]);
-    ext[k] = (double)ord2f(misc[kMiscBounds + 3 + k]) - mn[k];
-  }
+  for (int k = 0; k < 3; ++k) ext[k] = mx[k] - mn[k];
   __syncthreads();
   int begin = blockIdx.x * per_block;
   int end = min(n, begin + per_block);
@@ -383,12 +402,11 @@ __global__ __launch_bounds__(256) void k_hierarchy(const K* __restrict__ keys, c
 }
 
 // ------------------------------------------------------------ refit
+__device__ __forceinline__ int child_ref(int c, int n) { return c >= n - 1 ? ~(c - (n - 1)) : c; }
 // kernels/bvh.cu:209-258 with the missing release/acquire made explicit:
 // each leaf climbs; the first arrival at a node leaves, the second unions
 // the two child boxes (fminf/fmaxf), writes the node box and the node's
 // 64-byte traversal record, and continues until the root.
-__device__ __forceinline__ int child_ref(int c, int n) { return c >= n - 1 ? ~(c - (n - 1)) : c; }
-
 __global__ __launch_bounds__(256) void k_refit(int n, const int32_t* __restrict__ parent,
                                                const int32_t* __restrict__ left, const int32_t* __restrict__ right,
                                                uint32_t* flags, float* node_box, float4* __restrict__ tnode) {
@@ -464,6 +482,210 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const float* __restrict__ 
   L[2] = make_float4(cz, __int_as_float(tri), 0.f, 0.f);
 }
 
+// ------------------------------------------------------------ fused climb
+// Hierarchy and refit in one bottom-up pass (the agglomerative LBVH
+// construction of Apetrei, "Fast and Simple Agglomerative LBVH
+// Construction", 2014), numbered exactly like the reference's Karras tree
+// (kernels/bvh.cu:164-207): node [l, r]'s parent splits at r (the node is
+// its left child) when delta(r, r+1) > delta(l-1, l), else at l-1 (right
+// child); delta is the reference's prefix length with the index fallback
+// (kernels/bvh.cu:72-88). A left child takes index r, a right child index l,
+// the root [0, n-1] index 0 -- the numbering Karras's span/split search
+// produces. Each split position is shared by exactly two arrivals; the
+// second one unions the boxes (fminf/fmaxf, kernels/bvh.cu:63-69) and goes
+// on. The per-split arrival counter plays the role of the reference's
+// per-node atomicCAS flag (kernels/bvh.cu:228).
+//
+// One workgroup handles a chunk of kClimbChunk sorted leaves and counts the
+// arrivals at splits inside its chunk with LDS atomics (workgroup scope, no
+// cache maintenance). A node whose parent split leaves the chunk is handed
+// to k_climb_top, a single workgroup that finishes the top of the tree with
+// global counters, so no agent-scope release/acquire is needed anywhere.
+// Each arriving child first writes a 32-byte deposit (Karras index, far end
+// of its range, box) into deposits[2*split + side].
+template <typename K>
+__device__ __forceinline__ int climb_delta(const K* __restrict__ keys, int n, int i, int j) {
+  if (i < 0 || j >= n) return -1;
+  return KeyTraits<K>::delta(keys[i], keys[j], i, j);
+}
+
+struct ClimbNode {
+  int l, r, idx;        // sorted-leaf range and Karras index (idx < 0: not yet written)
+  int lidx, ridx;       // children (internal nodes only)
+  float b[6], lb[6], rb[6];
+};
+
+__device__ __forceinline__ void write_node(const ClimbNode& c, int idx, int n, int32_t* __restrict__ parent,
+                                           int32_t* __restrict__ left, int32_t* __restrict__ right,
+                                           float* __restrict__ node_box, float4* __restrict__ tnode) {
+  float* dst = node_box + 6 * (size_t)idx;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) dst[k] = c.b[k];
+  float4* T = tnode + 4 * (size_t)idx;
+  T[0] = make_float4(c.lb[0], c.lb[1], c.lb[2], c.lb[3]);
+  T[1] = make_float4(c.lb[4], c.lb[5], c.rb[0], c.rb[1]);
+  T[2] = make_float4(c.rb[2], c.rb[3], c.rb[4], c.rb[5]);
+  T[3] = make_float4(__int_as_float(child_ref(c.lidx, n)), __int_as_float(child_ref(c.ridx, n)), 0.f, 0.f);
+  left[idx] = c.lidx;
+  right[idx] = c.ridx;
+  parent[c.lidx] = idx;
+  parent[c.ridx] = idx;
+}
+
+// Merge the arriving child c with the sibling's deposit; c becomes the parent.
+__device__ __forceinline__ void merge_deposit(ClimbNode& c, bool is_left, const float4* __restrict__ dep) {
+  float4 o0 = dep[0], o1 = dep[1];
+  int oidx = __float_as_int(o0.x), ofar = __float_as_int(o0.y);
+  float ob[6] = {o0.z, o0.w, o1.x, o1.y, o1.z, o1.w};
+#pragma unroll
+  for (int k = 0; k < 6; ++k) {
+    c.lb[k] = is_left ? c.b[k] : ob[k];
+    c.rb[k] = is_left ? ob[k] : c.b[k];
+  }
+  c.lidx = is_left ? c.idx : oidx;
+  c.ridx = is_left ? oidx : c.idx;
+  if (is_left)
+    c.r = ofar;
+  else
+    c.l = ofar;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    c.b[k] = fminf(c.lb[k], c.rb[k]);
+    c.b[3 + k] = fmaxf(c.lb[3 + k], c.rb[3 + k]);
+  }
+  c.idx = -1;
+}
+
+// Decide the parent split of c; write c's node record if it is a new
+// internal node; write c's deposit. Returns the split; *side = 0 if c is
+// the left child.
+template <typename K>
+__device__ __forceinline__ int climb_step(ClimbNode& c, const K* __restrict__ keys, int n, int* side,
+                                          int32_t* __restrict__ parent, int32_t* __restrict__ left,
+                                          int32_t* __restrict__ right, float* __restrict__ node_box,
+                                          float4* __restrict__ tnode, float4* __restrict__ deposits) {
+  int dl = climb_delta<K>(keys, n, c.l - 1, c.l);
+  int dr = climb_delta<K>(keys, n, c.r, c.r + 1);
+  bool is_left = dr > dl;
+  int split = is_left ? c.r : c.l - 1;
+  if (split < 0 || split >= n - 1) return -1;  // cannot happen for a consistent climb
+  if (c.idx < 0) {
+    c.idx = is_left ? c.r : c.l;
+    write_node(c, c.idx, n, parent, left, right, node_box, tnode);
+  }
+  float4* d = deposits + 2 * (2 * (size_t)split + (is_left ? 0 : 1));
+  d[0] = make_float4(__int_as_float(c.idx), __int_as_float(is_left ? c.l : c.r), c.b[0], c.b[1]);
+  d[1] = make_float4(c.b[2], c.b[3], c.b[4], c.b[5]);
+  *side = is_left ? 0 : 1;
+  return split;
+}
+
+template <typename K>
+__global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
+    const K* __restrict__ keys, const uint32_t* __restrict__ ids, int n, const float* __restrict__ tri_box,
+    const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
+    const int32_t* __restrict__ idx, int32_t* __restrict__ parent, int32_t* __restrict__ left,
+    int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode, float4* __restrict__ ltri,
+    float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, uint32_t* __restrict__ deferred,
+    uint32_t* __restrict__ misc) {
+  __shared__ uint32_t s_arr[kClimbChunk];
+  const int s0 = blockIdx.x * kClimbChunk;
+  const int e0 = min(n, s0 + kClimbChunk);
+  for (int k = threadIdx.x; k < kClimbChunk; k += kClimbThreads) s_arr[k] = 0u;
+  // Leaves: node n-1+i = the triangle's box, plus its vertex record.
+  for (int i = s0 + threadIdx.x; i < e0; i += kClimbThreads) {
+    int tri = (int)ids[i];
+    const float* src = tri_box + 6 * (size_t)tri;
+    float* dst = node_box + 6 * (size_t)(n - 1 + i);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst[k] = src[k];
+    int a = idx[3 * tri + 0], b = idx[3 * tri + 1], c = idx[3 * tri + 2];
+    float4* L = ltri + 3 * (size_t)i;
+    L[0] = make_float4(vx[a], vy[a], vz[a], vx[b]);
+    L[1] = make_float4(vy[b], vz[b], vx[c], vy[c]);
+    L[2] = make_float4(vz[c], __int_as_float(tri), 0.f, 0.f);
+  }
+  if (s0 == 0 && threadIdx.x == 0) parent[0] = -1;
+  __syncthreads();
+  if (n == 1) return;
+  for (int i = s0 + threadIdx.x; i < e0; i += kClimbThreads) {
+    ClimbNode c;
+    c.l = c.r = i;
+    c.idx = n - 1 + i;
+    c.lidx = c.ridx = 0;
+    const float* src = tri_box + 6 * (size_t)ids[i];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) c.b[k] = src[k];
+    for (int guard = 0;; ++guard) {
+      int side;
+      int split = climb_step<K>(c, keys, n, &side, parent, left, right, node_box, tnode, deposits);
+      if (split < 0 || guard >= n) {
+        atomicAdd(&misc[kMiscBuildError], 1u);
+        break;
+      }
+      if (split >= s0 && split + 1 < e0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        uint32_t old = atomicAdd(&s_arr[split - s0], 1u);
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+        if (old == 0u) {
+          // The sibling may only arrive in k_climb_top: leave the count there.
+          arrivals[split] = 1u;
+          break;
+        }
+        merge_deposit(c, side == 0, deposits + 2 * (2 * (size_t)split + (side ^ 1)));
+        if (c.l == 0 && c.r == n - 1) {
+          write_node(c, 0, n, parent, left, right, node_box, tnode);
+          break;
+        }
+      } else {
+        uint32_t slot = atomicAdd(&misc[kMiscDeferred], 1u);
+        deferred[slot] = 2u * (uint32_t)split + (uint32_t)side;
+        break;
+      }
+    }
+  }
+}
+
+template <typename K>
+__global__ __launch_bounds__(kClimbTopThreads) void k_climb_top(
+    const K* __restrict__ keys, int n, int32_t* __restrict__ parent, int32_t* __restrict__ left,
+    int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode,
+    float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, const uint32_t* __restrict__ deferred,
+    uint32_t* __restrict__ misc) {
+  const uint32_t count = misc[kMiscDeferred];
+  for (uint32_t e = threadIdx.x; e < count; e += kClimbTopThreads) {
+    uint32_t code = deferred[e];
+    int split = (int)(code >> 1), side = (int)(code & 1u);
+    ClimbNode c;
+    {
+      const float4* d = deposits + 2 * (size_t)code;
+      float4 q0 = d[0], q1 = d[1];
+      c.idx = __float_as_int(q0.x);
+      int far = __float_as_int(q0.y);
+      c.l = side == 0 ? far : split + 1;
+      c.r = side == 0 ? split : far;
+      c.b[0] = q0.z; c.b[1] = q0.w; c.b[2] = q1.x; c.b[3] = q1.y; c.b[4] = q1.z; c.b[5] = q1.w;
+      c.lidx = c.ridx = 0;
+    }
+    for (int guard = 0;; ++guard) {
+      if (split < 0 || guard >= n) {
+        atomicAdd((uint32_t*)&misc[kMiscBuildError], 1u);
+        break;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      uint32_t old = atomicAdd(&arrivals[split], 1u);
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+      if (old == 0u) break;
+      merge_deposit(c, side == 0, deposits + 2 * (2 * (size_t)split + (side ^ 1)));
+      if (c.l == 0 && c.r == n - 1) {
+        write_node(c, 0, n, parent, left, right, node_box, tnode);
+        break;
+      }
+      split = climb_step<K>(c, keys, n, &side, parent, left, right, node_box, tnode, deposits);
+    }
+  }
+}
+
 // ------------------------------------------------------------ export
 template <typename K>
 __global__ __launch_bounds__(256) void k_export(int n, const K* __restrict__ keys, const uint32_t* __restrict__ ids,
@@ -507,9 +729,11 @@ template <typename K>
 hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
   const int n = b->n;
   constexpr int P = KeyTraits<K>::kPasses;
-  size_t zero_words = (size_t)P * 256 + (size_t)P * b->ntiles * 256;  // hist + status are contiguous
-  k_init<<<min(cdiv((long)zero_words, 256), 2048), 256, 0, s>>>(b->misc, b->hist, zero_words);
-  k_aabb_bounds<<<cdiv(n, 256), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box, b->misc);
+  // hist, look-back status and arrival counters are one contiguous range
+  size_t zero_words = (size_t)P * 256 + (size_t)P * b->ntiles * 256 + (size_t)(n > 1 ? n - 1 : 1);
+  k_init<<<min(cdiv((long)zero_words, 256), 2048), 256, 0, s>>>(b->misc, b->hist, zero_words, 1);
+  k_aabb_bounds<<<min(cdiv(n, 256), 1024), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box,
+                                                        b->misc, 1);
   int per_block = 4096;
   k_morton<K><<<cdiv(n, per_block), 256, 0, s>>>(b->tri_box, n, b->misc, (K*)b->keys[0], b->hist, per_block);
   for (int p = 0; p < P; ++p) {
@@ -519,11 +743,47 @@ hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
         b->hist + 256 * p, b->status + (size_t)p * b->ntiles * 256, b->misc + kMiscTileCtr + p);
   }
   // P is even: the sorted keys and ids are back in buffer 0.
-  k_hierarchy<K><<<cdiv(n, 256), 256, 0, s>>>((const K*)b->keys[0], b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev,
-                                              m->vz_dev, m->idx_dev, b->parent, b->left, b->right, b->flags,
-                                              b->node_box, b->ltri);
+  const K* keys = (const K*)b->keys[0];
+  if (b->build_flags & RT_BUILD_KARRAS_SEARCH) {
+    k_hierarchy<K><<<cdiv(n, 256), 256, 0, s>>>(keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev,
+                                                m->idx_dev, b->parent, b->left, b->right, b->flags, b->node_box,
+                                                b->ltri);
+    if (n > 1)
+      k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
+  } else {
+    k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
+        keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->parent, b->left, b->right,
+        b->node_box, b->tnode, b->ltri, b->deposits, b->arrivals, b->deferred, b->misc);
+    if (n > 1)
+      k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
+                                                    b->deposits, b->arrivals, b->deferred, b->misc);
+  }
+  return hipGetLastError();
+}
+
+template <typename K>
+hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
+  const int n = b->n;
+  if (b->build_flags & RT_BUILD_KARRAS_SEARCH) {
+    k_refit_leaves<<<cdiv(n, 256), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->ids[0], n,
+                                                  b->tri_box, b->node_box, b->ltri, b->flags);
+    if (n > 1)
+      k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
+    return hipGetLastError();
+  }
+  // The keys are unchanged, so the climb re-derives the same topology and
+  // recomputes every box from the moved vertices.
+  size_t zero_words = (size_t)(n > 1 ? n - 1 : 1);
+  k_init<<<min(cdiv((long)zero_words, 256), 2048), 256, 0, s>>>(b->misc, b->arrivals, zero_words, 0);
+  k_aabb_bounds<<<min(cdiv(n, 256), 1024), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box,
+                                                        b->misc, 0);
+  const K* keys = (const K*)b->keys[0];
+  k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
+      keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->parent, b->left, b->right,
+      b->node_box, b->tnode, b->ltri, b->deposits, b->arrivals, b->deferred, b->misc);
   if (n > 1)
-    k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
+    k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
+                                                  b->deposits, b->arrivals, b->deferred, b->misc);
   return hipGetLastError();
 }
 
@@ -534,12 +794,7 @@ hipError_t rt_launch_build(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
 }
 
 hipError_t rt_launch_refit(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
-  const int n = b->n;
-  k_refit_leaves<<<cdiv(n, 256), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->ids[0], n,
-                                                b->tri_box, b->node_box, b->ltri, b->flags);
-  if (n > 1)
-    k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
-  return hipGetLastError();
+  return b->bits == 30 ? refit_typed<uint32_t>(b, m, s) : refit_typed<uint64_t>(b, m, s);
 }
 
 hipError_t rt_launch_export(const rt_bvh* b, const rt_bvh_export_bufs* d, hipStream_t s) {

@@ -41,9 +41,13 @@ struct rt_bvh {
   void* arena = nullptr;
   size_t arena_bytes = 0;
   float* tri_box = nullptr;      // n*6
-  uint32_t* misc = nullptr;      // [0..5] ordered-int scene bounds, [8..8+passes) tile counters, [32..] overflow counter
+  int32_t build_flags = 0;
+  uint32_t* misc = nullptr;      // see kMisc* below
   uint32_t* hist = nullptr;      // passes*256
-  uint32_t* status = nullptr;    // passes*ntiles*256
+  uint32_t* status = nullptr;    // passes*ntiles*256 (contiguous after hist)
+  uint32_t* arrivals = nullptr;  // n-1 arrival counters of the top-level climb (contiguous after status)
+  float4* deposits = nullptr;    // (n-1)*2 child deposits, 32 B each
+  uint32_t* deferred = nullptr;  // n climb hand-offs to the top-level pass
   void* keys[2] = {nullptr, nullptr};
   uint32_t* ids[2] = {nullptr, nullptr};
   int32_t* parent = nullptr;     // 2n-1
@@ -58,10 +62,19 @@ struct rt_bvh {
 };
 
 // misc[] slots
-constexpr int kMiscBounds = 0;      // 6 words
-constexpr int kMiscTileCtr = 8;     // up to 8 words
-constexpr int kMiscOverflow = 32;   // 1 word: traversal stack overflows
+constexpr int kBoundsReplicas = 8;  // scene-bound accumulators, spread to cut atomic contention
+constexpr int kMiscBounds = 0;      // kBoundsReplicas * 6 words (min xyz, max xyz)
+constexpr int kMiscTileCtr = 48;    // up to 8 words: onesweep tile counters
+constexpr int kMiscOverflow = 56;   // 1 word: traversal stack overflows
+constexpr int kMiscDeferred = 57;   // 1 word: number of climb hand-offs
+constexpr int kMiscBuildError = 58; // 1 word: climb consistency failures (must stay 0)
 constexpr int kMiscWords = 64;
+
+// Fused hierarchy + refit climb: one workgroup per chunk of sorted leaves.
+constexpr int kClimbThreads = 256;
+constexpr int kClimbLeavesPerThread = 4;
+constexpr int kClimbChunk = kClimbThreads * kClimbLeavesPerThread;
+constexpr int kClimbTopThreads = 1024;
 
 // Sort tiling.
 constexpr int kSortThreads = 256;

@@ -130,7 +130,9 @@ __device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, H
   int stack[kStack];
   int sp = 0;
   int node = 0;
-  while (true) {
+  // Each internal node is entered at most once per ray; the bound only
+  // guards against a malformed tree.
+  for (int guard = 0; guard < n; ++guard) {
     float4 q0, q1, q2, q3;
     load_node(B.tnode, node, q0, q1, q2, q3);
     double l0, l1, r0, r1;
@@ -168,11 +170,13 @@ __device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, H
     } else if (hr) {
       node = cr;
     } else {
-      if (sp == 0) break;
+      if (sp == 0) return true;
       node = stack[--sp];
     }
   }
-  return true;
+  atomicAdd(overflow, 1u);
+  best.pos = -1;
+  return false;
 }
 
 // Any hit (kernels/bvh.cu:416-449): the first triangle in left-first DFS
@@ -194,7 +198,7 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
   int stack[kStack];
   int sp = 0;
   int node = 0;
-  while (true) {
+  for (int guard = 0; guard < n; ++guard) {
     float4 q0, q1, q2, q3;
     load_node(B.tnode, node, q0, q1, q2, q3);
     double l0, l1, r0, r1;
@@ -247,6 +251,8 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
     }
     node = next;
   }
+  atomicAdd(overflow, 1u);
+  return false;
 }
 
 __device__ __forceinline__ int32_t leaf_tri(const BvhView& B, int pos) {

@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "librtamd.so")
 
 RT_OK, RT_ERR_INVALID, RT_ERR_OOM, RT_ERR_DEVICE, RT_ERR_UNSUPPORTED, RT_ERR_OVERFLOW = range(6)
 RT_TRACE_CLOSEST, RT_TRACE_ANY = 0, 1
+RT_BUILD_DEFAULT, RT_BUILD_KARRAS_SEARCH = 0, 1
 RT_RENDER_RANDVEC_RTL, RT_RENDER_CLAMP = 1, 2
 
 c_f = ctypes.POINTER(ctypes.c_float)
@@ -68,7 +69,7 @@ class RenderOpts(ctypes.Structure):
 
 EXPORTED = [
     "rt_version", "rt_ctx_create", "rt_ctx_destroy", "rt_last_error", "rt_bvh_build", "rt_bvh_rebuild",
-    "rt_bvh_refit", "rt_bvh_export", "rt_bvh_scene_bounds", "rt_bvh_num_triangles", "rt_bvh_destroy",
+    "rt_bvh_refit", "rt_bvh_export", "rt_bvh_scene_bounds", "rt_bvh_num_triangles", "rt_bvh_diagnostics", "rt_bvh_destroy",
     "rt_trace", "rt_render", "rt_gen_soup", "rt_gen_icosphere_counts", "rt_gen_icosphere", "rt_gen_jitter",
 ]
 
@@ -94,6 +95,7 @@ def lib():
     L.rt_bvh_export.argtypes = [vp, ctypes.POINTER(ExportBufs), vp]
     L.rt_bvh_scene_bounds.argtypes = [vp, vp, c_d, c_d]
     L.rt_bvh_num_triangles.argtypes = [vp]
+    L.rt_bvh_diagnostics.argtypes = [vp, vp, c_u32]
     L.rt_bvh_destroy.argtypes = [vp]
     L.rt_trace.argtypes = [vp, ctypes.POINTER(MeshView), ctypes.POINTER(Rays), ctypes.c_int32, ctypes.c_double,
                            ctypes.POINTER(Hits), vp]
@@ -284,12 +286,12 @@ class Context:
 
 
 class BVH:
-    def __init__(self, ctx, mesh, morton_bits=30, stream=None):
+    def __init__(self, ctx, mesh, morton_bits=30, stream=None, flags=RT_BUILD_DEFAULT):
         self.ctx, self.L = ctx, ctx.L
         self.h = vp()
         self.n = mesh.n
         self.bits = morton_bits
-        o = BuildOpts(morton_bits, 0)
+        o = BuildOpts(morton_bits, flags)
         mv = mesh.view()
         ctx.check(self.L.rt_bvh_build(ctx.h, ctypes.byref(mv), ctypes.byref(o), _stream_handle(stream),
                                       ctypes.byref(self.h)))
@@ -307,6 +309,12 @@ class BVH:
         mx = (ctypes.c_double * 3)()
         self.ctx.check(self.L.rt_bvh_scene_bounds(self.h, _stream_handle(stream), mn, mx))
         return np.array(mn[:]), np.array(mx[:])
+
+    def diagnostics(self, stream=None):
+        """(traversal stack overflows, build consistency failures); both 0 when healthy."""
+        out = (ctypes.c_uint32 * 2)()
+        self.ctx.check(self.L.rt_bvh_diagnostics(self.h, _stream_handle(stream), out))
+        return int(out[0]), int(out[1])
 
     def export(self, stream=None):
         """Copy the BVH into the reference layout; returns host numpy arrays."""

@@ -41,7 +41,10 @@ typedef struct {
   int32_t n;
 } rt_mesh_view;
 
-enum { RT_BUILD_DEFAULT = 0 };
+enum {
+  RT_BUILD_DEFAULT = 0,       /* fused bottom-up hierarchy + refit climb */
+  RT_BUILD_KARRAS_SEARCH = 1  /* per-node span/split search (kernels/bvh.cu:100-207), then the atomic refit */
+};
 
 typedef struct {
   int32_t morton_bits; /* 30 (reference, kernels/bvh.cu:22-32) or 63 */
@@ -144,6 +147,10 @@ rt_status rt_bvh_export(const rt_bvh *bvh, const rt_bvh_export_bufs *dst, void *
  * (kernels/scene.cu:72-85). Synchronises the stream. */
 rt_status rt_bvh_scene_bounds(const rt_bvh *bvh, void *stream, double mn[3], double mx[3]);
 int32_t rt_bvh_num_triangles(const rt_bvh *bvh);
+/* Diagnostic counters since the BVH was created: out[0] = traversal stack
+ * overflows (rays reported as misses), out[1] = build consistency
+ * failures. Both stay 0 in correct operation. Synchronises the stream. */
+rt_status rt_bvh_diagnostics(const rt_bvh *bvh, void *stream, uint32_t out[2]);
 void rt_bvh_destroy(rt_bvh *bvh);
 
 /* Trace: BVH_d::intersect / intersectAny (kernels/bvh.cu:297-449).

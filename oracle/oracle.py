@@ -260,3 +260,60 @@ def sincos(x):
     c = ctypes.c_double()
     lib().orc_sincos(x, ctypes.byref(s), ctypes.byref(c))
     return s.value, c.value
+
+
+def _cam(cam):
+    C = Camera(cam.width, cam.height, cam.fov_deg, cam.fstop, cam.focal_point)
+    C.eye[:] = cam.eye[:]
+    return C
+
+
+def primary_ray(cam, px, py, sample=0, seed=0):
+    o = np.empty(3)
+    d = np.empty(3)
+    O = RenderOpts(1, 0, seed, 0)
+    lib().orc_primary_ray(ctypes.byref(_cam(cam)), ctypes.byref(O), px, py, sample, _p(o, c_d), _p(d, c_d))
+    return o, d
+
+
+def intersect_triangle(hm, tri, orig, dirs):
+    om = OMesh(hm)
+    t = ctypes.c_double()
+    a = ctypes.c_double()
+    b = ctypes.c_double()
+    nrm = np.empty(3)
+    ok = lib().orc_intersect_triangle(ctypes.byref(om.s), tri, _p(np.ascontiguousarray(orig, np.float64), c_d),
+                                      _p(np.ascontiguousarray(dirs, np.float64), c_d), ctypes.byref(t),
+                                      ctypes.byref(a), ctypes.byref(b), _p(nrm, c_d))
+    return bool(ok), t.value, a.value, b.value, nrm
+
+
+def box_hit(lo, hi, orig, dirs):
+    return bool(lib().orc_box_hit(_p(np.ascontiguousarray(lo, np.float32), c_f),
+                                  _p(np.ascontiguousarray(hi, np.float32), c_f),
+                                  _p(np.ascontiguousarray(orig, np.float64), c_d),
+                                  _p(np.ascontiguousarray(dirs, np.float64), c_d)))
+
+
+def wang_hash(s):
+    return lib().orc_wang_hash(s)
+
+
+def xorshift(state):
+    st = ctypes.c_uint32(state)
+    v = lib().orc_xorshift(ctypes.byref(st))
+    return v, st.value
+
+
+def rand_double(state):
+    st = ctypes.c_uint32(state)
+    v = lib().orc_rand_double(ctypes.byref(st))
+    return v, st.value
+
+
+def seed_index(px, py, width):
+    return lib().orc_seed_index(px, py, width)
+
+
+def expand_bits10(v):
+    return lib().orc_expand_bits10(v)

@@ -420,28 +420,41 @@ static V3 trace_path(const RenderCtx &c, V3 p, V3 d, uint32_t &rng, int64_t *cnt
   return color;
 }
 
+// kernels/raytracer.cu:142-154: two draws, thin-lens origin, direction.
+static void lens_ray(const orc_camera *cam, double xx, double yy, uint32_t &rng, V3 *o_out, V3 *d_out) {
+  double aperture = 1 / cam->fstop;
+  double ang = 2 * M_PI * rand_double(rng);
+  double rad = aperture * sqrt(rand_double(rng)) / 2.0;
+  double s, co;
+  orc_math::sincos(ang, &s, &co);
+  V3 o = v3(rad * co, rad * s, 0);
+  V3 d = normalize(v3(xx, yy, -1.0));
+  d = sub(scale(d, cam->focal_point), o);
+  *o_out = o;
+  *d_out = normalize(d);
+}
+
+static void pixel_xy(const orc_camera *cam, int px, int py, double *xx, double *yy) {  // kernels/raytracer.cu:126-132
+  int W = cam->width, H = cam->height;
+  double invW = 1.0 / double(W), invH = 1.0 / double(H);
+  double aspect = double(W) / double(H);
+  double angle = tan(M_PI * 0.5 * cam->fov_deg / 180.0);
+  *xx = (2 * ((px + 0.5) * invW) - 1) * angle * aspect;
+  *yy = (1 - 2 * ((py + 0.5) * invH)) * angle;
+}
+
 static void render_pixel(const RenderCtx &c, int px, int py, float *rgb, int64_t *cnt) {
   const orc_camera *cam = c.cam;
   int W = cam->width, H = cam->height;
   int64_t oidx = (int64_t)(H - py - 1) * W + px;  // kernels/raytracer.cu:106
   uint32_t rng = wang(seed_index(px, py, W) ^ c.o->seed);
-  double invW = 1.0 / double(W), invH = 1.0 / double(H);
-  double aspect = double(W) / double(H);
-  double angle = tan(M_PI * 0.5 * cam->fov_deg / 180.0);
-  double aperture = 1 / cam->fstop;
-  double xx = (2 * ((px + 0.5) * invW) - 1) * angle * aspect;
-  double yy = (1 - 2 * ((py + 0.5) * invH)) * angle;
+  double xx, yy;
+  pixel_xy(cam, px, py, &xx, &yy);
   V3 color = v3(0, 0, 0);
   int N = c.o->spp;
   for (int it = 0; it < N; ++it) {
-    double ang = 2 * M_PI * rand_double(rng);
-    double rad = aperture * sqrt(rand_double(rng)) / 2.0;
-    double s, co;
-    orc_math::sincos(ang, &s, &co);
-    V3 o = v3(rad * co, rad * s, 0);
-    V3 d = normalize(v3(xx, yy, -1.0));
-    d = sub(scale(d, cam->focal_point), o);
-    d = normalize(d);
+    V3 o, d;
+    lens_ray(cam, xx, yy, rng, &o, &d);
     ++cnt[0];
     color = add(color, trace_path(c, o, d, rng, cnt));
   }
@@ -770,6 +783,18 @@ uint32_t orc_xorshift(uint32_t *state) { return xorshift(*state); }
 double orc_rand_double(uint32_t *state) { return rand_double(*state); }
 uint32_t orc_seed_index(int32_t px, int32_t py, int32_t width) { return seed_index(px, py, width); }
 void orc_sincos(double x, double *s, double *c) { orc_math::sincos(x, s, c); }
+
+void orc_primary_ray(const orc_camera *cam, const orc_render_opts *opts, int32_t px, int32_t py, int32_t sample,
+                     double *orig, double *dir) {
+  // Only valid for samples whose predecessors missed (a hit draws 3 more numbers).
+  uint32_t rng = wang(seed_index(px, py, cam->width) ^ opts->seed);
+  double xx, yy;
+  pixel_xy(cam, px, py, &xx, &yy);
+  V3 o, d;
+  for (int k = 0; k <= sample; ++k) lens_ray(cam, xx, yy, rng, &o, &d);
+  orig[0] = o.x; orig[1] = o.y; orig[2] = o.z;
+  dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
 
 void orc_render(const orc_mesh *m, const orc_bvh *b, const orc_material *mats, int32_t nmat, const orc_light *light,
                 const orc_camera *cam, const orc_render_opts *opts, float *rgb, const int64_t *pixels, int64_t npix,

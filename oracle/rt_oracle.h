@@ -122,6 +122,11 @@ double orc_rand_double(uint32_t *state);
 uint32_t orc_seed_index(int32_t px, int32_t py, int32_t width);
 void orc_sincos(double x, double *s, double *c);
 
+/* A15: primary ray of sample `sample` of launch pixel (px,py) with the
+ * reference's draw order (kernels/raytracer.cu:126-154). */
+void orc_primary_ray(const orc_camera *cam, const orc_render_opts *opts, int32_t px, int32_t py, int32_t sample,
+                     double *orig, double *dir);
+
 /* A15-A18: render (kernels/raytracer.cu:94-249, kernels/material.cu:6-40,
  * kernels/Light.cu:5-86). rgb is width*height*3 floats, row-flipped like
  * the reference image. If pixels != NULL only those launch pixels

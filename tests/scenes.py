@@ -40,3 +40,16 @@ def random_rays(n, seed, origin=(0.0, 0.0, 0.0), spread=0.6):
     d = np.stack([rng.uniform(-spread, spread, n), rng.uniform(-spread, spread, n), -np.ones(n)], 1)
     d /= np.linalg.norm(d, axis=1)[:, None]
     return o, d
+
+
+def duplicate_key_mesh(n, seed=77):
+    """Most triangles packed into one tiny cluster, a few spread over the unit
+    cube: the cluster's centroids share Morton cells, so keys repeat and the
+    tree falls back to index bits (kernels/bvh.cu:87)."""
+    hm = rtamd.gen_soup(n, seed, edge_median=1e-4)
+    k = n - 16
+    c = np.float32(0.3)
+    for a in (hm.vx, hm.vy, hm.vz):
+        a[: 3 * k] = c + (a[: 3 * k] - a[: 3 * k].mean()) * np.float32(1e-3)
+        a[3: 3 * 1000] = np.tile(a[0:3], 999)  # 1000 identical triangles: equal keys at any width
+    return hm

@@ -28,7 +28,9 @@ def ctx():
     return rtamd.Context(0)
 
 
-def _compare_build(got, ref, hm):
+def _compare_build(got, ref, hm, bvh=None):
+    if bvh is not None:
+        assert bvh.diagnostics() == (0, 0)
     assert np.array_equal(got["codes"], ref["codes"]), "morton codes / sort order differ"
     assert np.array_equal(got["ids"], ref["ids"]), "sorted ids differ"
     assert np.array_equal(got["parent"][1:], ref["parent"][1:]), "parent links differ"
@@ -52,12 +54,12 @@ def test_build_parity(ctx, bits, scene):
     b = rtamd.BVH(ctx, dm, bits)
     got = b.export()
     ref = O.build(hm, bits)
-    _compare_build(got, ref, hm)
+    _compare_build(got, ref, hm, b)
     mn, mx = b.scene_bounds()
     assert np.array_equal(mn, ref["mn"]) and np.array_equal(mx, ref["mx"])
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 17, 4095, 4096, 4097, 70_000])
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 1023, 1024, 1025, 4095, 4096, 4097, 70_000])
 def test_build_sizes(ctx, n):
     _say(f"build sizes n={n}")
     hm = scenes.soup_mesh(n, seed=1234 + n)
@@ -66,13 +68,45 @@ def test_build_sizes(ctx, n):
         b = rtamd.BVH(ctx, dm, bits)
         got = b.export()
         ref = O.build(hm, bits)
-        _compare_build(got, ref, hm)
+        _compare_build(got, ref, hm, b)
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_build_karras_search_path(ctx, bits):
+    """The per-node span/split search + atomic refit path gives the same BVH."""
+    _say(f"karras search path bits={bits}")
+    hm = scenes.soup_mesh(50_000, seed=99)
+    dm = rtamd.DeviceMesh(hm)
+    b = rtamd.BVH(ctx, dm, bits, flags=rtamd.RT_BUILD_KARRAS_SEARCH)
+    _compare_build(b.export(), O.build(hm, bits), hm)
+
+
+@pytest.mark.parametrize("flags", [rtamd.RT_BUILD_DEFAULT, rtamd.RT_BUILD_KARRAS_SEARCH])
+def test_refit_parity(ctx, flags):
+    """Refit after vertex jitter: same topology, boxes = oracle refit of the moved mesh."""
+    _say(f"refit parity flags={flags}")
+    hm = scenes.displaced_mesh(40)
+    dm = rtamd.DeviceMesh(hm)
+    b = rtamd.BVH(ctx, dm, 63, flags=flags)
+    ref = O.build(hm, 63)
+    for frame in range(3):
+        moved = rtamd.gen_jitter(hm, 1e-3, 1000 + frame)
+        dm.set_positions(*[torch.from_numpy(a).cuda() for a in (moved.vx, moved.vy, moved.vz)])
+        b.refit(dm)
+        got = b.export()
+        tlo, thi = O.tri_aabb(moved)
+        lo, hi = O.refit(ref["parent"], ref["left"], ref["right"], tlo, thi)
+        assert np.array_equal(got["left"], ref["left"]) and np.array_equal(got["right"], ref["right"])
+        assert np.array_equal(got["parent"][1:], ref["parent"][1:])
+        assert np.array_equal(got["lo"].view(np.uint32), lo.view(np.uint32))
+        assert np.array_equal(got["hi"].view(np.uint32), hi.view(np.uint32))
+        assert b.diagnostics() == (0, 0)
 
 
 def test_build_duplicate_keys(ctx):
     """Every triangle in a tiny region: 30-bit codes collide, the index fallback orders the tree."""
     _say("build duplicate keys")
-    hm = rtamd.gen_soup(20_000, 77, edge_median=1e-6, offset=(0.25, 0.25, 0.25), scale=1e-5)
+    hm = scenes.duplicate_key_mesh(20_000)
     dm = rtamd.DeviceMesh(hm)
     for bits in (30, 63):
         b = rtamd.BVH(ctx, dm, bits)
@@ -104,6 +138,7 @@ def _trace_compare(ctx, hm, bits, orig, dirs):
         torch.cuda.synchronize()
         tri, t, a, be = [x.cpu().numpy() for x in (tri, t, a, be)]
         rtri, rt_, ra, rb, _, _ = O.trace(hm, ref_b, orig, dirs, any_hit=any_hit)
+        assert b.diagnostics() == (0, 0)
         assert np.array_equal(tri, rtri), f"hit ids differ (any={any_hit}): {np.sum(tri != rtri)} rays"
         h = rtri >= 0
         np.testing.assert_allclose(t[h], rt_[h], rtol=1e-5)
