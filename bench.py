@@ -1,24 +1,36 @@
 """Benchmark: LBVH build + thin-lens render of a 1M-triangle random soup on one MI355X.
 
-Workload (one step): 1,000,000-triangle random soup (centroids uniform in the
-unit cube, lognormal edges, random orientation, fp32), 63-bit Morton LBVH
-build (AABBs -> Morton -> onesweep sort -> Karras -> atomic refit), then a
-1920x1080 thin-lens depth-of-field render (aperture 0.05, focus 1.5) at 16
-spp with one shadow ray per hit. Inputs are resident in HBM before timing.
+Workload (one step, "soup1M"): 1,000,000-triangle random soup (centroids
+uniform in the unit cube, lognormal edges with median 0.002 and sigma 0.5,
+uniformly random orientation, fp32), 63-bit Morton LBVH build (triangle
+AABBs + scene bounds -> Morton keys -> onesweep radix sort -> fused
+hierarchy/refit climb), then a 1920x1080 thin-lens depth-of-field render
+(aperture 0.05, focus distance 1.5, fov 40) at 16 spp with one shadow ray
+per hit. Inputs are resident in HBM before the timed region.
 
 Prints one JSON line. value = primary rays per second of the whole step
-(build + render) in Mrays/s.
+(build + render), in Mrays/s.
 """
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "cuda-rt_amd"))
+
+HBM_PEAK_GBS = 8000.0
+# FP64 vector issue rate of MI355X: 256 CU x 64 lanes x 2.4 GHz = 39.3 T lane-op/s
+# (the published 78.6 TFLOP/s counts an FMA as two flops).
+FP64_LANE_OPS_PEAK_T = 39.3
+# Algorithmic work per unit (fp32 canonical counts; FP64 evaluation counts twice).
+OPS_PER_NODE_VISIT = 32     # two slab tests per internal node entered
+OPS_PER_TRI_TEST = 42       # Moller-Trumbore
+FP64_FACTOR = 2
+# Algorithmic HBM bytes per triangle of a 63-bit build.
+BUILD_BYTES_PER_TRI_63 = 396
 
 
 def parse():
@@ -30,9 +42,51 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=16)
+    ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront",
+                    help="render kernel structure (identical images)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-pixel-stride", type=int, default=5,
+                    help="CPU baseline renders every k-th pixel (launch order)")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="only run device copies of this many MiB (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    import torch
+    a = torch.empty(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_mib": mib, "bytes_read": a.numel() * 4, "bytes_written": a.numel() * 4,
+                      "copies": 3}))
+
+
+def cpu_baseline(hm, mats, light, cam, opts, stride, tris):
+    """Time the FP64 CPU oracle (a restatement of the reference algorithm) on a
+    bounded sample: the full 63-bit build, then every stride-th launch pixel."""
+    import numpy as np
+    from oracle import oracle as O
+    threads = min(os.cpu_count() or 1, 16)
+    t0 = time.perf_counter()
+    b = O.build(hm, 63, nthreads=threads)
+    t_build = time.perf_counter() - t0
+    pixels = np.arange(0, cam.width * cam.height, stride, dtype=np.int64)
+    t0 = time.perf_counter()
+    _, counts = O.render(hm, b, mats, light, cam, opts, pixels=pixels, nthreads=threads)
+    t_render = time.perf_counter() - t0
+    return {
+        "value": round(counts[0] / t_render / 1e6, 4),
+        "unit": "Mrays/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"oracle FP64 build of all {tris} tris ({t_build:.2f} s, "
+                  f"{tris / t_build / 1e6:.3f} Mtris/s) + render of every {stride}th pixel "
+                  f"({pixels.size} px x {opts.spp} spp = {int(counts[0])} primary, {int(counts[1])} shadow rays, "
+                  f"{t_render:.2f} s)",
+        "build_mtris_s": round(tris / t_build / 1e6, 4),
+    }
 
 
 def main():
@@ -40,6 +94,9 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     import numpy as np
     import torch
 
@@ -51,11 +108,13 @@ def main():
     hm = scenes.soup_mesh(args.tris)
     dm = rtamd.DeviceMesh(hm)
     mats, light, cam, opts = scenes.c2_setup(args.width, args.height, args.spp)
+    if args.mode == "wavefront":
+        opts.flags |= rtamd.RT_RENDER_WAVEFRONT
     md = rtamd.materials_tensor(mats)
     stream = torch.cuda.current_stream()
     bvh = rtamd.BVH(ctx, dm, 63)
     rgb = torch.empty(args.width * args.height * 3, dtype=torch.float32, device=dev)
-    cnt = torch.zeros(3, dtype=torch.int64, device=dev)
+    cnt = torch.zeros(5, dtype=torch.int64, device=dev)
 
     def step(events=None):
         if events:
@@ -78,12 +137,18 @@ def main():
         step(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    build_ms = [e[0].elapsed_time(e[1]) for e in evs]
-    render_ms = [e[1].elapsed_time(e[2]) for e in evs]
-    counts = cnt.cpu().numpy() // args.steps
-    primary = int(counts[0])
+    diag = bvh.diagnostics()
+    build_ms = float(np.median([e[0].elapsed_time(e[1]) for e in evs]))
+    render_ms = float(np.median([e[1].elapsed_time(e[2]) for e in evs]))
+    c = cnt.cpu().numpy() // args.steps
+    primary, shadow, bounce, visits, tests = (int(x) for x in c)
     ms_per_step = wall * 1e3 / args.steps
-    value = primary / (ms_per_step * 1e-3) / 1e6
+    value = (primary + bounce) / (ms_per_step * 1e-3) / 1e6
+
+    lane_ops = (visits * OPS_PER_NODE_VISIT + tests * OPS_PER_TRI_TEST) * FP64_FACTOR
+    achieved = lane_ops / (render_ms * 1e-3) / 1e12
+    build_bytes = BUILD_BYTES_PER_TRI_63 * args.tris
+    build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
     out = {
         "metric": "Mrays/s (primary+bounce) & LBVH build Mtris/s @1M tris",
         "value": round(value, 2),
@@ -99,14 +164,42 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"soup{args.tris}_63bit_{args.width}x{args.height}_{args.spp}spp_thinlens",
                    "tris": args.tris, "width": args.width, "height": args.height, "spp": args.spp,
-                   "morton_bits": 63},
-        "build_ms": round(float(np.median(build_ms)), 4),
-        "build_mtris_s": round(args.tris / (np.median(build_ms) * 1e-3) / 1e6, 1),
-        "render_ms": round(float(np.median(render_ms)), 3),
-        "render_mrays_s": round(primary / (np.median(render_ms) * 1e-3) / 1e6, 2),
-        "shadow_rays": int(counts[1]),
+                   "morton_bits": 63, "aperture": 0.05, "focus": 1.5},
+        "render_mode": args.mode,
+        "roofline": {
+            "kernel": "render (all traversal kernels of one frame)",
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": FP64_LANE_OPS_PEAK_T,
+            "unit": "T FP64 lane-op/s",
+            "frac": round(achieved / FP64_LANE_OPS_PEAK_T, 4),
+            "traffic": None,
+            "work": {"node_visits": visits, "tri_tests": tests, "lane_ops": lane_ops,
+                     "ops_per_visit": OPS_PER_NODE_VISIT * FP64_FACTOR, "ops_per_test": OPS_PER_TRI_TEST * FP64_FACTOR},
+        },
+        "roofline_build": {
+            "kernels": "k_init..k_climb_top (one rebuild)",
+            "bound": "hbm",
+            "achieved": round(build_gbs, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(build_gbs / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "bytes_per_tri": BUILD_BYTES_PER_TRI_63,
+        },
+        "build_ms": round(build_ms, 4),
+        "build_mtris_s": round(args.tris / (build_ms * 1e-3) / 1e6, 1),
+        "render_ms": round(render_ms, 3),
+        "render_mrays_s": round((primary + bounce) / (render_ms * 1e-3) / 1e6, 2),
         "primary_rays": primary,
+        "shadow_rays": shadow,
+        "total_rays_incl_shadow_mrays_s": round((primary + bounce + shadow) / (render_ms * 1e-3) / 1e6, 2),
+        "diagnostics": {"stack_overflows": diag[0], "build_errors": diag[1]},
+        "cpu_baseline": None,
     }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(hm, mats, light, cam, opts, args.cpu_pixel_stride, args.tris)
+        out["gpu_vs_cpu"] = round(out["render_mrays_s"] / max(out["cpu_baseline"]["value"], 1e-9), 1)
     print(json.dumps(out), flush=True)
 
 

@@ -55,7 +55,11 @@ rt_status rt_ctx_create(int32_t device, rt_ctx** out) {
   return RT_OK;
 }
 
-void rt_ctx_destroy(rt_ctx* ctx) { delete ctx; }
+void rt_ctx_destroy(rt_ctx* ctx) {
+  if (!ctx) return;
+  if (ctx->workspace) (void)hipFree(ctx->workspace);
+  delete ctx;
+}
 
 const char* rt_last_error(const rt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
@@ -231,7 +235,28 @@ rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, co
   if (cam->width <= 0 || cam->height <= 0 || opts->spp <= 0 || opts->depth < 0)
     return rt_set_error(ctx, RT_ERR_INVALID, "bad image size, spp or depth");
   if (mesh->n != bvh->n) return rt_set_error(ctx, RT_ERR_INVALID, "mesh and BVH triangle counts differ");
-  RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, counters_dev,
+  if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT))
+    return rt_set_error(ctx, RT_ERR_INVALID, "unknown render flags");
+  void* ws = nullptr;
+  if (opts->flags & RT_RENDER_WAVEFRONT) {
+    size_t need = rt_wavefront_bytes(cam->width, cam->height);
+    if (ctx->workspace_bytes < need) {
+      if (ctx->workspace) {
+        RT_HIP(ctx, hipStreamSynchronize((hipStream_t)stream));
+        RT_HIP(ctx, hipFree(ctx->workspace));
+        ctx->workspace = nullptr;
+        ctx->workspace_bytes = 0;
+      }
+      if (hipMalloc(&ctx->workspace, need) != hipSuccess) {
+        ctx->workspace = nullptr;
+        (void)hipGetLastError();
+        return rt_set_error(ctx, RT_ERR_OOM, "wavefront workspace of %zu bytes", need);
+      }
+      ctx->workspace_bytes = need;
+    }
+    ws = ctx->workspace;
+  }
+  RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, counters_dev, ws,
                                (hipStream_t)stream));
   return RT_OK;
 }

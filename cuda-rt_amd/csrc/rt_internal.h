@@ -11,6 +11,8 @@ struct rt_ctx {
   int32_t device = 0;
   int32_t num_cus = 256;
   std::string err;
+  void* workspace = nullptr;     // wavefront render state, grown on demand
+  size_t workspace_bytes = 0;
 };
 
 // Device-side view of a built BVH, passed by value to the trace kernels.
@@ -91,6 +93,9 @@ hipError_t rt_launch_export(const rt_bvh* b, const rt_bvh_export_bufs* dst, hipS
 // Launchers (rt_trace.hip).
 hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays* rays, int32_t mode, double tmin,
                            const rt_hits* hits, hipStream_t s);
+// workspace == nullptr selects the megakernel, else the wavefront passes
+// (rt_wavefront_bytes(width, height) bytes of device memory).
 hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
-                            uint64_t* counters, hipStream_t s);
+                            uint64_t* counters, void* workspace, hipStream_t s);
+size_t rt_wavefront_bytes(int32_t width, int32_t height);

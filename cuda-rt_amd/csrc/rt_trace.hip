@@ -26,7 +26,6 @@ using rt::d3;
 using rt::kRayEps;
 using rt::mk;
 
-constexpr int kStack = 64;
 
 struct RayD {
   d3 o, d;
@@ -87,6 +86,12 @@ struct HitRec {
   int32_t pos;  // sorted leaf position, -1 = none
 };
 
+// Work counters: internal nodes entered (two child-box tests each) and
+// triangle tests.
+struct TravStats {
+  uint32_t visits = 0, tests = 0;
+};
+
 __device__ __forceinline__ double prune_margin(double t) { return 1e-4 * fabs(t) + 1e-7; }
 
 __device__ __forceinline__ void load_node(const float4* __restrict__ tn, int node, float4& a, float4& b, float4& c,
@@ -99,8 +104,9 @@ __device__ __forceinline__ void load_node(const float4* __restrict__ tn, int nod
 }
 
 __device__ __forceinline__ void consider_leaf(const BvhView& B, const RayD& r, int pos, double tmin_phys,
-                                              HitRec& best) {
+                                              HitRec& best, TravStats& st) {
   double t, al, be;
+  ++st.tests;
   if (!tri_test(r, B.ltri + 3 * (size_t)pos, t, al, be)) return;
   if (tmin_phys > 0 && !(t >= tmin_phys)) return;
   if (best.pos < 0 || t < best.t || (t == best.t && pos < best.pos)) {
@@ -111,11 +117,46 @@ __device__ __forceinline__ void consider_leaf(const BvhView& B, const RayD& r, i
   }
 }
 
+// Traversal stack: a per-lane ring of kRing entries in LDS (one column per
+// thread, conflict-free) backed by a private spill area that is only touched
+// when a path is more than kRing levels deep. Depth beyond kRing + kSpill is
+// reported as an overflow and the ray as a miss.
+constexpr int kThreads = 256;
+constexpr int kRing = 32;
+constexpr int kSpill = 64;
+
+struct RingStack {
+  int* col;  // this lane's column of the workgroup's LDS ring
+  int sp = 0, lo = 0;
+};
+
+__device__ __forceinline__ bool stack_push(RingStack& S, int* spill, int v) {
+  if (S.sp - S.lo == kRing) {
+    if (S.lo == kSpill) return false;
+    spill[S.lo] = S.col[(S.lo & (kRing - 1)) * kThreads];
+    ++S.lo;
+  }
+  S.col[(S.sp & (kRing - 1)) * kThreads] = v;
+  ++S.sp;
+  return true;
+}
+
+__device__ __forceinline__ int stack_pop(RingStack& S, int* spill) {
+  --S.sp;
+  int v = S.col[(S.sp & (kRing - 1)) * kThreads];
+  if (S.sp == S.lo && S.lo > 0) {
+    --S.lo;
+    S.col[(S.lo & (kRing - 1)) * kThreads] = spill[S.lo];
+  }
+  return v;
+}
+
 // Closest hit (kernels/bvh.cu:382-414): minimum (t, leaf position) over all
 // triangles whose ancestors' boxes the line enters. tmin_phys > 0 restricts
 // hits to t >= tmin_phys (bounce rays). Returns false on stack overflow
 // (counted), with best.pos = -1.
-__device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, HitRec& best, uint32_t* overflow) {
+__device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, HitRec& best, uint32_t* overflow,
+                            TravStats& st, int* lds_col) {
   best.pos = -1;
   best.t = 0;
   const int n = B.n;
@@ -124,17 +165,19 @@ __device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, H
   const float* rb = B.node_box;
   if (!slab(r, rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], t0, t1)) return true;
   if (n == 1) {
-    consider_leaf(B, r, 0, tmin_phys, best);
+    consider_leaf(B, r, 0, tmin_phys, best, st);
     return true;
   }
-  int stack[kStack];
-  int sp = 0;
+  RingStack S;
+  S.col = lds_col;
+  int spill[kSpill];
   int node = 0;
   // Each internal node is entered at most once per ray; the bound only
   // guards against a malformed tree.
   for (int guard = 0; guard < n; ++guard) {
     float4 q0, q1, q2, q3;
     load_node(B.tnode, node, q0, q1, q2, q3);
+    ++st.visits;
     double l0, l1, r0, r1;
     bool hl = slab(r, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, l0, l1);
     bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
@@ -149,29 +192,28 @@ __device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, H
       hr = hr && !(r0 > lim);
     }
     if (hl && cl < 0) {
-      consider_leaf(B, r, ~cl, tmin_phys, best);
+      consider_leaf(B, r, ~cl, tmin_phys, best, st);
       hl = false;
     }
     if (hr && cr < 0) {
-      consider_leaf(B, r, ~cr, tmin_phys, best);
+      consider_leaf(B, r, ~cr, tmin_phys, best, st);
       hr = false;
     }
     if (hl && hr) {
       int nearc = (l0 <= r0) ? cl : cr, farc = (l0 <= r0) ? cr : cl;
-      if (sp == kStack) {
+      if (!stack_push(S, spill, farc)) {
         atomicAdd(overflow, 1u);
         best.pos = -1;
         return false;
       }
-      stack[sp++] = farc;
       node = nearc;
     } else if (hl) {
       node = cl;
     } else if (hr) {
       node = cr;
     } else {
-      if (sp == 0) return true;
-      node = stack[--sp];
+      if (S.sp == 0) return true;
+      node = stack_pop(S, spill);
     }
   }
   atomicAdd(overflow, 1u);
@@ -181,7 +223,8 @@ __device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, H
 
 // Any hit (kernels/bvh.cu:416-449): the first triangle in left-first DFS
 // order whose test passes, whatever its t.
-__device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* overflow) {
+__device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* overflow, TravStats& st,
+                        int* lds_col) {
   hit.pos = -1;
   const int n = B.n;
   if (n <= 0) return false;
@@ -189,18 +232,21 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
   const float* rb = B.node_box;
   if (!slab(r, rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], t0, t1)) return false;
   if (n == 1) {
+    ++st.tests;
     if (tri_test(r, B.ltri, t, al, be)) {
       hit.t = t; hit.al = al; hit.be = be; hit.pos = 0;
       return true;
     }
     return false;
   }
-  int stack[kStack];
-  int sp = 0;
+  RingStack S;
+  S.col = lds_col;
+  int spill[kSpill];
   int node = 0;
   for (int guard = 0; guard < n; ++guard) {
     float4 q0, q1, q2, q3;
     load_node(B.tnode, node, q0, q1, q2, q3);
+    ++st.visits;
     double l0, l1, r0, r1;
     bool hl = slab(r, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, l0, l1);
     bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
@@ -209,17 +255,15 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
     bool have_next = false;
     if (hl) {
       if (cl < 0) {
+        ++st.tests;
         if (tri_test(r, B.ltri + 3 * (size_t)(~cl), t, al, be)) {
           hit.t = t; hit.al = al; hit.be = be; hit.pos = ~cl;
           return true;
         }
       } else {
-        if (hr) {
-          if (sp == kStack) {
-            atomicAdd(overflow, 1u);
-            return false;
-          }
-          stack[sp++] = cr;
+        if (hr && !stack_push(S, spill, cr)) {
+          atomicAdd(overflow, 1u);
+          return false;
         }
         next = cl;
         have_next = true;
@@ -227,6 +271,7 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
     }
     if (!have_next && hr) {
       if (cr < 0) {
+        ++st.tests;
         if (tri_test(r, B.ltri + 3 * (size_t)(~cr), t, al, be)) {
           hit.t = t; hit.al = al; hit.be = be; hit.pos = ~cr;
           return true;
@@ -237,9 +282,10 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
       }
     }
     while (!have_next) {
-      if (sp == 0) return false;
-      int ref = stack[--sp];
+      if (S.sp == 0) return false;
+      int ref = stack_pop(S, spill);
       if (ref < 0) {
+        ++st.tests;
         if (tri_test(r, B.ltri + 3 * (size_t)(~ref), t, al, be)) {
           hit.t = t; hit.al = al; hit.be = be; hit.pos = ~ref;
           return true;
@@ -260,19 +306,21 @@ __device__ __forceinline__ int32_t leaf_tri(const BvhView& B, int pos) {
 }
 
 // ------------------------------------------------------------ trace kernel
-__global__ __launch_bounds__(256) void k_trace(BvhView B, const double* __restrict__ orig,
-                                               const double* __restrict__ dir, long nrays, int mode, double tmin,
-                                               int32_t* __restrict__ tri, double* __restrict__ tout,
-                                               double* __restrict__ aout, double* __restrict__ bout,
-                                               uint32_t* overflow) {
+__global__ __launch_bounds__(kThreads) void k_trace(BvhView B, const double* __restrict__ orig,
+                                                    const double* __restrict__ dir, long nrays, int mode, double tmin,
+                                                    int32_t* __restrict__ tri, double* __restrict__ tout,
+                                                    double* __restrict__ aout, double* __restrict__ bout,
+                                                    uint32_t* overflow) {
+  __shared__ int s_stack[kRing * kThreads];
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= nrays) return;
   RayD r = make_ray(mk(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]), mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
   HitRec h;
+  TravStats st;
   if (mode == RT_TRACE_ANY)
-    any_hit(B, r, h, overflow);
+    any_hit(B, r, h, overflow, st, s_stack + threadIdx.x);
   else
-    closest_hit(B, r, tmin, h, overflow);
+    closest_hit(B, r, tmin, h, overflow, st, s_stack + threadIdx.x);
   bool ok = h.pos >= 0;
   tri[i] = ok ? leaf_tri(B, h.pos) : -1;
   if (tout) tout[i] = ok ? h.t : 0.0;
@@ -280,7 +328,7 @@ __global__ __launch_bounds__(256) void k_trace(BvhView B, const double* __restri
   if (bout) bout[i] = ok ? h.be : 0.0;
 }
 
-// ------------------------------------------------------------ render
+// ------------------------------------------------------------ shading
 struct RenderParams {
   int W, H, spp, depth;
   uint32_t seed;
@@ -334,149 +382,363 @@ __device__ __forceinline__ d3 cosine_dir(d3 n, double u1, double u2) {
 
 constexpr double kBounceTmin = 1e-6;
 
-__global__ __launch_bounds__(256) void k_render(BvhView B, MeshShade M, const rt_material* __restrict__ mats,
-                                                RenderParams P, float* __restrict__ rgb,
-                                                unsigned long long* counters, uint32_t* overflow) {
+// Thin-lens primary ray of the next sample (kernels/raytracer.cu:142-154).
+__device__ __forceinline__ void lens_ray(const RenderParams& P, double xx, double yy, uint32_t& rng, d3& o, d3& d) {
+  double ang = 2 * M_PI * rt::rand_unit(rng);
+  double rad = P.aperture * sqrt(rt::rand_unit(rng)) / 2.0;
+  double sa, ca;
+  rt::sincos_cw(ang, &sa, &ca);
+  o = mk(rad * ca, rad * sa, 0);
+  d = rt::normalize(mk(xx, yy, -1.0));
+  d = d * P.focal - o;
+  d = rt::normalize(d);
+}
+
+// Everything Material::shade computes before the shadow query
+// (kernels/material.cu:6-40, Light::getDirection / distanceAttenuation,
+// kernels/Light.cu:56-81): the shadow ray (q, L), the emissive base
+// I1 = ke + ka % 0, the falloff fo and the unshadowed (diffuse + specular) term.
+struct ShadePrep {
+  d3 q, L, I1, ds, N;
+  double fo;
+};
+
+__device__ __forceinline__ ShadePrep shade_prep(const RenderParams& P, const rt_material& m, d3 N, d3 o, d3 d,
+                                                double t, uint32_t& rng) {
+  ShadePrep S;
+  const d3 lpos = arr3(P.lpos), eye = arr3(P.eye);
+  d3 q = o + d * t;
+  S.q = q;
+  S.N = N;
+  S.I1 = arr3(m.ke) + rt::hadamard(arr3(m.ka), mk(0, 0, 0));
+  d3 V = eye - q;
+  V = mk(-V.x, -V.y, -V.z);
+  V = rt::normalize(V);
+  double r1, r2, r3;
+  if (!(P.flags & RT_RENDER_RANDVEC_RTL)) {
+    r1 = rt::rand_unit(rng); r2 = rt::rand_unit(rng); r3 = rt::rand_unit(rng);
+  } else {
+    r3 = rt::rand_unit(rng); r2 = rt::rand_unit(rng); r1 = rt::rand_unit(rng);
+  }
+  d3 off = mk(r1, r2, r3) * P.lrad;
+  off = mk(off.x - 0.5, off.y - 0.5, off.z - 0.5);
+  d3 L = rt::normalize((lpos + off) - q);
+  S.L = L;
+  double dl = rt::length_f(lpos - q);
+  double fo = P.c0 + (P.c1 + P.c2 * dl) * dl;
+  fo = (fabs(fo) < kRayEps) ? 1.0 : 1.0 / fo;
+  S.fo = fmin(1.0, fo);
+  double nl = rt::dot(N, L);
+  if (nl < 0) nl = 0;
+  d3 diffuse = arr3(m.kd) * nl;
+  d3 spec = mk(0, 0, 0);
+  if (!(m.ks[0] == 0 && m.ks[1] == 0 && m.ks[2] == 0)) {
+    d3 R = N * (-2.0 * rt::dot(L, N)) + L;
+    R = rt::normalize(R);
+    double rv = rt::dot(R, V);
+    double sf = (double)powf((float)fmax(0.0, rv), (float)m.shininess);
+    spec = arr3(m.ks) * sf;
+  }
+  S.ds = diffuse + spec;
+  return S;
+}
+
+// Light::shadowAttenuation from the any-hit result (kernels/Light.cu:5-53).
+__device__ __forceinline__ d3 shadow_factor(const RenderParams& P, const BvhView& B, const rt_material* mats,
+                                            const MeshShade& M, d3 q, bool hit, const HitRec& s) {
+  if (!hit) return mk(1, 1, 1);
+  if (!(s.t < rt::length_f(arr3(P.lpos) - q))) return mk(1, 1, 1);
+  if (s.t < kRayEps) return mk(1, 1, 1);
+  const rt_material& sm = mat_of(mats, P.nmat, M, leaf_tri(B, s.pos));
+  bool trans = (sm.kt[0] == 0) && (sm.kt[1] == 0) && (sm.kt[2] == 0);
+  return trans ? rt::hadamard(arr3(sm.kd), mk(1, 1, 1) - arr3(sm.kt)) : mk(0, 0, 0);
+}
+
+// I = I1 + ((sh * fo) % (diffuse + specular)) % light colour.
+__device__ __forceinline__ d3 shade_finish(const RenderParams& P, d3 I1, double fo, d3 ds, d3 sh) {
+  return I1 + rt::hadamard(rt::hadamard(sh * fo, ds), arr3(P.lcol));
+}
+
+__device__ __forceinline__ void pixel_of_tile(const RenderParams& P, int tile, int lane, int& px, int& py) {
   const int tiles_x = (P.W + 15) / 16;
-  const int tile = blockIdx.x;
-  const int px = (tile % tiles_x) * 16 + (threadIdx.x & 15);
-  const int py = (tile / tiles_x) * 16 + (threadIdx.x >> 4);
+  px = (tile % tiles_x) * 16 + (lane & 15);
+  py = (tile / tiles_x) * 16 + (lane >> 4);
+}
+
+__device__ __forceinline__ void pixel_xy(const RenderParams& P, int px, int py, double& xx, double& yy) {
+  xx = (2 * ((px + 0.5) * P.invW) - 1) * P.angle * P.aspect;
+  yy = (1 - 2 * ((py + 0.5) * P.invH)) * P.angle;
+}
+
+__device__ __forceinline__ void write_pixel(const RenderParams& P, float* rgb, int px, int py, d3 color) {
+  color = mk(color.x / double(P.spp), color.y / double(P.spp), color.z / double(P.spp));
+  if (P.flags & RT_RENDER_CLAMP) {
+    color.x = color.x < 0 ? 0.0 : (color.x > 1 ? 1.0 : color.x);
+    color.y = color.y < 0 ? 0.0 : (color.y > 1 ? 1.0 : color.y);
+    color.z = color.z < 0 ? 0.0 : (color.z > 1 ? 1.0 : color.z);
+  }
+  size_t oi = 3 * ((size_t)(P.H - py - 1) * P.W + px);
+  rgb[oi + 0] = (float)color.x;
+  rgb[oi + 1] = (float)color.y;
+  rgb[oi + 2] = (float)color.z;
+}
+
+__device__ __forceinline__ void flush_counters(unsigned long long* counters, unsigned long long prim,
+                                               unsigned long long shadow, unsigned long long bounce,
+                                               const TravStats& st) {
+  if (!counters) return;
+  unsigned long long vis = st.visits, tst = st.tests;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    prim += __shfl_xor(prim, off);
+    shadow += __shfl_xor(shadow, off);
+    bounce += __shfl_xor(bounce, off);
+    vis += __shfl_xor(vis, off);
+    tst += __shfl_xor(tst, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (prim) atomicAdd(&counters[0], prim);
+    if (shadow) atomicAdd(&counters[1], shadow);
+    if (bounce) atomicAdd(&counters[2], bounce);
+    if (vis) atomicAdd(&counters[3], vis);
+    if (tst) atomicAdd(&counters[4], tst);
+  }
+}
+
+// ------------------------------------------------------------ megakernel
+// One thread per pixel, all samples, bounces and shadow rays in one launch
+// (kernels/raytracer.cu:94-166 + traceRay :168-249).
+__global__ __launch_bounds__(kThreads) void k_render(BvhView B, MeshShade M, const rt_material* __restrict__ mats,
+                                                     RenderParams P, float* __restrict__ rgb,
+                                                     unsigned long long* counters, uint32_t* overflow) {
+  __shared__ int s_stack[kRing * kThreads];
+  int* col = s_stack + threadIdx.x;
+  int px, py;
+  pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
   const bool active = px < P.W && py < P.H;
   unsigned long long n_shadow = 0, n_bounce = 0;
+  TravStats st;
   if (active) {
     uint32_t rng = rt::wang_hash(rt::seed_index(px, py, P.W) ^ P.seed);
-    double xx = (2 * ((px + 0.5) * P.invW) - 1) * P.angle * P.aspect;
-    double yy = (1 - 2 * ((py + 0.5) * P.invH)) * P.angle;
-    const bool rtl = (P.flags & RT_RENDER_RANDVEC_RTL) != 0;
-    const d3 lpos = arr3(P.lpos), lcol = arr3(P.lcol), eye = arr3(P.eye);
+    double xx, yy;
+    pixel_xy(P, px, py, xx, yy);
     d3 color = mk(0, 0, 0);
     for (int it = 0; it < P.spp; ++it) {
-      double ang = 2 * M_PI * rt::rand_unit(rng);
-      double rad = P.aperture * sqrt(rt::rand_unit(rng)) / 2.0;
-      double sa, ca;
-      rt::sincos_cw(ang, &sa, &ca);
-      d3 o = mk(rad * ca, rad * sa, 0);
-      d3 d = rt::normalize(mk(xx, yy, -1.0));
-      d = d * P.focal - o;
-      d = rt::normalize(d);
+      d3 o, d;
+      lens_ray(P, xx, yy, rng, o, d);
       d3 acc = mk(0, 0, 0), thr = mk(1, 1, 1);
       for (int bounce = 0;; ++bounce) {
         RayD ray = make_ray(o, d);
         HitRec h;
-        closest_hit(B, ray, bounce == 0 ? 0.0 : kBounceTmin, h, overflow);
+        closest_hit(B, ray, bounce == 0 ? 0.0 : kBounceTmin, h, overflow, st, col);
         if (h.pos < 0) {
           acc = acc + rt::hadamard(thr, mk(0.9, 0.9, 0.9));
           break;
         }
         const int tri = leaf_tri(B, h.pos);
         const rt_material& m = mat_of(mats, P.nmat, M, tri);
-        d3 N = shading_normal(M, tri, h.al, h.be);
-        // Material::shade (kernels/material.cu:6-40)
-        d3 q = o + d * h.t;
-        d3 I = arr3(m.ke) + rt::hadamard(arr3(m.ka), mk(0, 0, 0));
-        d3 V = eye - q;
-        V = mk(-V.x, -V.y, -V.z);
-        V = rt::normalize(V);
-        // Light::getDirection (kernels/Light.cu:71-81)
-        double r1, r2, r3;
-        if (!rtl) {
-          r1 = rt::rand_unit(rng); r2 = rt::rand_unit(rng); r3 = rt::rand_unit(rng);
-        } else {
-          r3 = rt::rand_unit(rng); r2 = rt::rand_unit(rng); r1 = rt::rand_unit(rng);
-        }
-        d3 off = mk(r1, r2, r3) * P.lrad;
-        off = mk(off.x - 0.5, off.y - 0.5, off.z - 0.5);
-        d3 L = rt::normalize((lpos + off) - q);
-        // Light::shadowAttenuation (kernels/Light.cu:5-53)
-        d3 sh = mk(1, 1, 1);
-        {
-          RayD sray = make_ray(q, L);
-          HitRec s;
-          ++n_shadow;
-          if (any_hit(B, sray, s, overflow)) {
-            if (s.t < rt::length_f(lpos - q)) {
-              if (!(s.t < kRayEps)) {
-                const rt_material& sm = mat_of(mats, P.nmat, M, leaf_tri(B, s.pos));
-                bool trans = (sm.kt[0] == 0) && (sm.kt[1] == 0) && (sm.kt[2] == 0);
-                sh = trans ? rt::hadamard(arr3(sm.kd), mk(1, 1, 1) - arr3(sm.kt)) : mk(0, 0, 0);
-              }
-            }
-          }
-        }
-        // Light::distanceAttenuation (kernels/Light.cu:56-68)
-        double dl = rt::length_f(lpos - q);
-        double fo = P.c0 + (P.c1 + P.c2 * dl) * dl;
-        fo = (fabs(fo) < kRayEps) ? 1.0 : 1.0 / fo;
-        fo = fmin(1.0, fo);
-        d3 atten = sh * fo;
-        double nl = rt::dot(N, L);
-        if (nl < 0) nl = 0;
-        d3 diffuse = arr3(m.kd) * nl;
-        d3 spec = mk(0, 0, 0);
-        if (!(m.ks[0] == 0 && m.ks[1] == 0 && m.ks[2] == 0)) {
-          d3 R = N * (-2.0 * rt::dot(L, N)) + L;
-          R = rt::normalize(R);
-          double rv = rt::dot(R, V);
-          double sf = (double)powf((float)fmax(0.0, rv), (float)m.shininess);
-          spec = arr3(m.ks) * sf;
-        }
-        I = I + rt::hadamard(rt::hadamard(atten, diffuse + spec), lcol);
+        ShadePrep S = shade_prep(P, m, shading_normal(M, tri, h.al, h.be), o, d, h.t, rng);
+        HitRec s;
+        ++n_shadow;
+        bool sh_hit = any_hit(B, make_ray(S.q, S.L), s, overflow, st, col);
+        d3 I = shade_finish(P, S.I1, S.fo, S.ds, shadow_factor(P, B, mats, M, S.q, sh_hit, s));
         acc = acc + rt::hadamard(thr, I);
         if (bounce >= P.depth) break;
         double u1 = rt::rand_unit(rng);
         double u2 = rt::rand_unit(rng);
+        d3 N = S.N;
         if (rt::dot(N, d) > 0) N = mk(-N.x, -N.y, -N.z);
         thr = rt::hadamard(thr, arr3(m.kd));
-        o = q;
+        o = S.q;
         d = cosine_dir(N, u1, u2);
         ++n_bounce;
       }
       color = color + acc;
     }
-    color = mk(color.x / double(P.spp), color.y / double(P.spp), color.z / double(P.spp));
-    if (P.flags & RT_RENDER_CLAMP) {
-      color.x = color.x < 0 ? 0.0 : (color.x > 1 ? 1.0 : color.x);
-      color.y = color.y < 0 ? 0.0 : (color.y > 1 ? 1.0 : color.y);
-      color.z = color.z < 0 ? 0.0 : (color.z > 1 ? 1.0 : color.z);
-    }
-    size_t oi = 3 * ((size_t)(P.H - py - 1) * P.W + px);
-    rgb[oi + 0] = (float)color.x;
-    rgb[oi + 1] = (float)color.y;
-    rgb[oi + 2] = (float)color.z;
+    write_pixel(P, rgb, px, py, color);
   }
-  if (counters) {
-    unsigned long long prim = active ? (unsigned long long)P.spp : 0ull;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      prim += __shfl_xor(prim, off);
-      n_shadow += __shfl_xor(n_shadow, off);
-      n_bounce += __shfl_xor(n_bounce, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-      atomicAdd(&counters[0], prim);
-      atomicAdd(&counters[1], n_shadow);
-      atomicAdd(&counters[2], n_bounce);
+  flush_counters(counters, active ? (unsigned long long)P.spp : 0ull, n_shadow, n_bounce, st);
+}
+
+// ------------------------------------------------------------ wavefront
+// The same per-pixel computation split into passes over ray queues: per
+// sample, a camera pass, then for each bounce a closest-hit pass (which
+// also draws the light sample and, if the path continues, the bounce
+// direction) and an any-hit pass over the compacted shadow queue. Per-pixel
+// RNG state and partial sums live in HBM between passes; each pixel's
+// draws and additions happen in the megakernel's order, so both give the
+// same image bit for bit.
+struct WfState {
+  uint32_t* rng;        // W*H
+  double* color;        // W*H*3: sum over finished samples
+  double* acc;          // W*H*3: current sample's path sum
+  double* thr;          // W*H*3: current path throughput
+  double* ray;          // W*H*6: current ray origin, direction
+  uint32_t* queue[2];   // W*H pixel ids of live rays (ping-pong)
+  uint32_t* shq_pix;    // W*H shadow-queue pixel ids
+  double* shq;          // W*H*16: q, L, I1, ds, thr, fo
+  uint32_t* counts;     // [0] live rays in queue[0], [1] in queue[1], [2] shadow entries
+};
+
+__device__ __forceinline__ d3 ld3(const double* p) { return mk(p[0], p[1], p[2]); }
+__device__ __forceinline__ void st3(double* p, d3 v) {
+  p[0] = v.x;
+  p[1] = v.y;
+  p[2] = v.z;
+}
+
+// Append to a queue with one atomic per wave.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+  uint64_t m = __ballot(want);
+  if (m == 0) return 0;
+  const uint32_t lane = __lane_id();
+  uint32_t leader = (uint32_t)__builtin_ctzll(m);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, (int)leader);
+  uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
+  return base + (uint32_t)__popcll(below);
+}
+
+__global__ __launch_bounds__(kThreads) void k_wf_init(RenderParams P, WfState W) {
+  int px, py;
+  pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
+  if (px >= P.W || py >= P.H) return;
+  size_t p = (size_t)py * P.W + px;
+  W.rng[p] = rt::wang_hash(rt::seed_index(px, py, P.W) ^ P.seed);
+  st3(W.color + 3 * p, mk(0, 0, 0));
+}
+
+// Start sample: primary ray, throughput 1, empty path sum, every pixel live.
+__global__ __launch_bounds__(kThreads) void k_wf_camera(RenderParams P, WfState W) {
+  int px, py;
+  pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
+  const bool active = px < P.W && py < P.H;
+  if (active) {
+    size_t p = (size_t)py * P.W + px;
+    uint32_t rng = W.rng[p];
+    double xx, yy;
+    pixel_xy(P, px, py, xx, yy);
+    d3 o, d;
+    lens_ray(P, xx, yy, rng, o, d);
+    W.rng[p] = rng;
+    st3(W.ray + 6 * p, o);
+    st3(W.ray + 6 * p + 3, d);
+    st3(W.thr + 3 * p, mk(1, 1, 1));
+    st3(W.acc + 3 * p, mk(0, 0, 0));
+  }
+  uint32_t slot = wave_append(&W.counts[0], active);
+  if (active) W.queue[0][slot] = (uint32_t)(py * P.W + px);
+}
+
+__global__ __launch_bounds__(kThreads) void k_wf_closest(BvhView B, MeshShade M, const rt_material* __restrict__ mats,
+                                                         RenderParams P, WfState W, int bounce, int qin,
+                                                         unsigned long long* counters, uint32_t* overflow) {
+  __shared__ int s_stack[kRing * kThreads];
+  const uint32_t n_in = W.counts[qin];
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const bool live = i < n_in;
+  TravStats st;
+  bool want_shadow = false, want_next = false;
+  uint32_t p = 0;
+  ShadePrep S;
+  d3 thr = mk(0, 0, 0), nray_o = mk(0, 0, 0), nray_d = mk(0, 0, 0), nthr = mk(0, 0, 0);
+  if (live) {
+    p = W.queue[qin][i];
+    d3 o = ld3(W.ray + 6 * (size_t)p), d = ld3(W.ray + 6 * (size_t)p + 3);
+    thr = ld3(W.thr + 3 * (size_t)p);
+    HitRec h;
+    closest_hit(B, make_ray(o, d), bounce == 0 ? 0.0 : kBounceTmin, h, overflow, st, s_stack + threadIdx.x);
+    if (h.pos < 0) {
+      double* a = W.acc + 3 * (size_t)p;
+      d3 acc = ld3(a) + rt::hadamard(thr, mk(0.9, 0.9, 0.9));
+      st3(a, acc);
+    } else {
+      uint32_t rng = W.rng[p];
+      const int tri = leaf_tri(B, h.pos);
+      const rt_material& m = mat_of(mats, P.nmat, M, tri);
+      S = shade_prep(P, m, shading_normal(M, tri, h.al, h.be), o, d, h.t, rng);
+      want_shadow = true;
+      if (bounce < P.depth) {
+        double u1 = rt::rand_unit(rng);
+        double u2 = rt::rand_unit(rng);
+        d3 N = S.N;
+        if (rt::dot(N, d) > 0) N = mk(-N.x, -N.y, -N.z);
+        nthr = rt::hadamard(thr, arr3(m.kd));
+        nray_o = S.q;
+        nray_d = cosine_dir(N, u1, u2);
+        want_next = true;
+      }
+      W.rng[p] = rng;
     }
   }
+  uint32_t ss = wave_append(&W.counts[2], want_shadow);
+  if (want_shadow) {
+    W.shq_pix[ss] = p;
+    double* e = W.shq + 16 * (size_t)ss;
+    st3(e + 0, S.q);
+    st3(e + 3, S.L);
+    st3(e + 6, S.I1);
+    st3(e + 9, S.ds);
+    st3(e + 12, thr);
+    e[15] = S.fo;
+  }
+  uint32_t ns = wave_append(&W.counts[qin ^ 1], want_next);
+  if (want_next) {
+    W.queue[qin ^ 1][ns] = p;
+    st3(W.ray + 6 * (size_t)p, nray_o);
+    st3(W.ray + 6 * (size_t)p + 3, nray_d);
+    st3(W.thr + 3 * (size_t)p, nthr);
+  }
+  flush_counters(counters, bounce == 0 && live ? 1ull : 0ull, 0ull, bounce > 0 && live ? 1ull : 0ull, st);
+}
+
+__global__ __launch_bounds__(kThreads) void k_wf_shadow(BvhView B, MeshShade M, const rt_material* __restrict__ mats,
+                                                        RenderParams P, WfState W, unsigned long long* counters,
+                                                        uint32_t* overflow) {
+  __shared__ int s_stack[kRing * kThreads];
+  const uint32_t n_in = W.counts[2];
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const bool live = i < n_in;
+  TravStats st;
+  if (live) {
+    const double* e = W.shq + 16 * (size_t)i;
+    d3 q = ld3(e + 0), L = ld3(e + 3);
+    HitRec s;
+    bool hit = any_hit(B, make_ray(q, L), s, overflow, st, s_stack + threadIdx.x);
+    d3 I = shade_finish(P, ld3(e + 6), e[15], ld3(e + 9), shadow_factor(P, B, mats, M, q, hit, s));
+    uint32_t p = W.shq_pix[i];
+    double* a = W.acc + 3 * (size_t)p;
+    st3(a, ld3(a) + rt::hadamard(ld3(e + 12), I));
+  }
+  flush_counters(counters, 0ull, live ? 1ull : 0ull, 0ull, st);
+}
+
+// End of a sample: color += path sum; clear the queue counters.
+__global__ __launch_bounds__(kThreads) void k_wf_flush(RenderParams P, WfState W) {
+  int px, py;
+  pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
+  if (blockIdx.x == 0 && threadIdx.x < 3) W.counts[threadIdx.x] = 0u;
+  if (px >= P.W || py >= P.H) return;
+  size_t p = (size_t)py * P.W + px;
+  st3(W.color + 3 * p, ld3(W.color + 3 * p) + ld3(W.acc + 3 * p));
+}
+
+__global__ __launch_bounds__(kThreads) void k_wf_reset(uint32_t* counts, int which) {
+  if (threadIdx.x == 0) counts[which] = 0u;
+}
+
+__global__ __launch_bounds__(kThreads) void k_wf_resolve(RenderParams P, WfState W, float* __restrict__ rgb) {
+  int px, py;
+  pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
+  if (px >= P.W || py >= P.H) return;
+  size_t p = (size_t)py * P.W + px;
+  write_pixel(P, rgb, px, py, ld3(W.color + 3 * p));
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-}  // namespace
-
-hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays* rays, int32_t mode, double tmin,
-                           const rt_hits* hits, hipStream_t s) {
-  (void)m;
-  if (rays->n <= 0) return hipSuccess;
-  k_trace<<<cdiv(rays->n, 256), 256, 0, s>>>(b->view(), rays->orig_dev, rays->dir_dev, rays->n, mode, tmin,
-                                            hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev,
-                                            b->misc + kMiscOverflow);
-  return hipGetLastError();
-}
-
-hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
-                            const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
-                            uint64_t* counters, hipStream_t s) {
+RenderParams make_params(const rt_camera* cam, const rt_light* light, const rt_render_opts* opts, int32_t nmat) {
   RenderParams P;
   P.W = cam->width;
   P.H = cam->height;
@@ -501,9 +763,77 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   P.c0 = light->c0;
   P.c1 = light->c1;
   P.c2 = light->c2;
+  return P;
+}
+
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+
+hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays* rays, int32_t mode, double tmin,
+                           const rt_hits* hits, hipStream_t s) {
+  (void)m;
+  if (rays->n <= 0) return hipSuccess;
+  k_trace<<<cdiv(rays->n, kThreads), kThreads, 0, s>>>(b->view(), rays->orig_dev, rays->dir_dev, rays->n, mode, tmin,
+                                                      hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev,
+                                                      b->misc + kMiscOverflow);
+  return hipGetLastError();
+}
+
+size_t rt_wavefront_bytes(int32_t width, int32_t height) {
+  size_t np = (size_t)width * height;
+  return align256(np * 4) + 2 * align256(np * 24) + align256(np * 24) + align256(np * 48) + 2 * align256(np * 4) +
+         align256(np * 4) + align256(np * 128) + 256;
+}
+
+hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
+                            const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
+                            uint64_t* counters, void* workspace, hipStream_t s) {
+  RenderParams P = make_params(cam, light, opts, nmat);
   MeshShade M{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev};
-  int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
-  k_render<<<tiles, 256, 0, s>>>(b->view(), M, mats, P, rgb, (unsigned long long*)counters,
-                                 b->misc + kMiscOverflow);
+  const int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
+  unsigned long long* cnt = (unsigned long long*)counters;
+  uint32_t* ovf = b->misc + kMiscOverflow;
+  if (!workspace) {
+    k_render<<<tiles, kThreads, 0, s>>>(b->view(), M, mats, P, rgb, cnt, ovf);
+    return hipGetLastError();
+  }
+  const size_t np = (size_t)P.W * P.H;
+  char* w = (char*)workspace;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    char* r = w + off;
+    off += align256(bytes);
+    return r;
+  };
+  WfState W;
+  W.rng = (uint32_t*)take(np * 4);
+  W.color = (double*)take(np * 24);
+  W.acc = (double*)take(np * 24);
+  W.thr = (double*)take(np * 24);
+  W.ray = (double*)take(np * 48);
+  W.queue[0] = (uint32_t*)take(np * 4);
+  W.queue[1] = (uint32_t*)take(np * 4);
+  W.shq_pix = (uint32_t*)take(np * 4);
+  W.shq = (double*)take(np * 128);
+  W.counts = (uint32_t*)take(256);
+  const int qblocks = cdiv((long)np, kThreads);
+  k_wf_reset<<<1, kThreads, 0, s>>>(W.counts, 0);
+  k_wf_reset<<<1, kThreads, 0, s>>>(W.counts, 1);
+  k_wf_reset<<<1, kThreads, 0, s>>>(W.counts, 2);
+  k_wf_init<<<tiles, kThreads, 0, s>>>(P, W);
+  for (int it = 0; it < P.spp; ++it) {
+    k_wf_camera<<<tiles, kThreads, 0, s>>>(P, W);
+    int qin = 0;
+    for (int bounce = 0; bounce <= P.depth; ++bounce) {
+      k_wf_closest<<<qblocks, kThreads, 0, s>>>(b->view(), M, mats, P, W, bounce, qin, cnt, ovf);
+      k_wf_shadow<<<qblocks, kThreads, 0, s>>>(b->view(), M, mats, P, W, cnt, ovf);
+      k_wf_reset<<<1, kThreads, 0, s>>>(W.counts, 2);
+      k_wf_reset<<<1, kThreads, 0, s>>>(W.counts, qin);
+      qin ^= 1;
+    }
+    k_wf_flush<<<tiles, kThreads, 0, s>>>(P, W);
+  }
+  k_wf_resolve<<<tiles, kThreads, 0, s>>>(P, W, rgb);
   return hipGetLastError();
 }

@@ -113,7 +113,8 @@ typedef struct {
 
 enum {
   RT_RENDER_RANDVEC_RTL = 1, /* draw randVec3d's components z, y, x instead of x, y, z */
-  RT_RENDER_CLAMP = 2        /* clamp each channel to [0,1] (kernels/scene.cu:137) */
+  RT_RENDER_CLAMP = 2,       /* clamp each channel to [0,1] (kernels/scene.cu:137) */
+  RT_RENDER_WAVEFRONT = 4    /* ray-queue passes instead of the megakernel (same image) */
 };
 
 typedef struct {
@@ -164,7 +165,10 @@ rt_status rt_trace(const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_rays *r
  * (kernels/raytracer.cu:94-249, kernels/material.cu:6-40, kernels/Light.cu).
  * rgb_dev receives width*height*3 floats, row-flipped like the reference
  * image (kernels/raytracer.cu:106). counters_dev, if not NULL, receives
- * 3 uint64 counts added atomically: primary, shadow and bounce rays. */
+ * 5 uint64 counts added atomically: primary, shadow and bounce rays,
+ * internal nodes entered (two child-box tests each) and triangle tests.
+ * RT_RENDER_WAVEFRONT keeps per-pixel state in a context-owned workspace
+ * that the first such call for a given image size allocates. */
 rt_status rt_render(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_material *mats_dev,
                     int32_t nmat, const rt_light *light, const rt_camera *cam, const rt_render_opts *opts,
                     float *rgb_dev, uint64_t *counters_dev, void *stream);

@@ -81,6 +81,12 @@ def lib():
         L.orc_sincos.argtypes = [ctypes.c_double, c_d, c_d]
         L.orc_box_hit.restype = ctypes.c_int32
         L.orc_intersect_triangle.restype = ctypes.c_int32
+        # entry points with 64-bit sizes need explicit argtypes
+        L.orc_trace.argtypes = [vp, vp, ctypes.c_int64, c_d, c_d, ctypes.c_int32, c_i32, c_d, c_d, c_d, c_d, c_i64,
+                                ctypes.c_int32]
+        L.orc_trace_brute.argtypes = [vp, c_i32, ctypes.c_int64, c_d, c_d, c_i32, c_d]
+        L.orc_render.argtypes = [vp, vp, vp, ctypes.c_int32, vp, vp, vp, c_f, c_i64, ctypes.c_int64, c_i64,
+                                 ctypes.c_int32]
         _lib = L
     return _lib
 

@@ -179,17 +179,25 @@ def _render_compare(ctx, hm, bits, mats, light, cam, opts, pixels=None):
     dm = rtamd.DeviceMesh(hm)
     b = rtamd.BVH(ctx, dm, bits)
     md = rtamd.materials_tensor(mats)
-    cnt = torch.zeros(3, dtype=torch.int64, device="cuda")
-    rgb = rtamd.render(ctx, b, dm, md, len(mats), light, cam, opts, counters=cnt)
-    torch.cuda.synchronize()
-    got = rgb.cpu().numpy()
     ref_b = O.build(hm, bits)
     ref, counts = O.render(hm, ref_b, mats, light, cam, opts)
-    diff = np.abs(got - ref)
-    assert diff.max() <= 1e-4, f"max pixel diff {diff.max()} at {np.unravel_index(diff.argmax(), diff.shape)}"
-    c = cnt.cpu().numpy()
-    assert c[0] == counts[0] and c[1] == counts[1] and c[2] == counts[2], (c, counts)
-    return got, diff
+    images = []
+    for wavefront in (False, True):
+        o = rtamd.render_opts(opts.spp, opts.depth, opts.seed,
+                              opts.flags | (rtamd.RT_RENDER_WAVEFRONT if wavefront else 0))
+        cnt = torch.zeros(5, dtype=torch.int64, device="cuda")
+        rgb = rtamd.render(ctx, b, dm, md, len(mats), light, cam, o, counters=cnt)
+        torch.cuda.synchronize()
+        got = rgb.cpu().numpy()
+        diff = np.abs(got - ref)
+        assert diff.max() <= 1e-4, \
+            f"wavefront={wavefront} max pixel diff {diff.max()} at {np.unravel_index(diff.argmax(), diff.shape)}"
+        c = cnt.cpu().numpy()
+        assert c[0] == counts[0] and c[1] == counts[1] and c[2] == counts[2], (wavefront, c, counts)
+        images.append(got)
+    assert np.array_equal(images[0], images[1]), "megakernel and wavefront images differ"
+    assert b.diagnostics() == (0, 0)
+    return images[0], np.abs(images[0] - ref)
 
 
 def test_render_parity_c1(ctx):
