@@ -89,7 +89,7 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   off += (size_t)b->passes * b->ntiles * 256 * 4;
   size_t o_arr = off;
   off = align_up(off + ni * 4);
-  size_t o_dep = take(ni * 64), o_def = take((size_t)n * 4);
+  size_t o_dep = take(ni * 64), o_def = take((size_t)n * 4), o_fetch = take(1024 * 4);
   size_t o_k0 = take((size_t)n * ks), o_k1 = take((size_t)n * ks);
   size_t o_i0 = take((size_t)n * 4), o_i1 = take((size_t)n * 4);
   size_t o_par = take(nn * 4), o_l = take(ni * 4), o_r = take(ni * 4), o_f = take(ni * 4);
@@ -111,6 +111,7 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   b->arrivals = (uint32_t*)(a + o_arr);
   b->deposits = (float4*)(a + o_dep);
   b->deferred = (uint32_t*)(a + o_def);
+  b->fetch = (uint32_t*)(a + o_fetch);
   b->keys[0] = a + o_k0;
   b->keys[1] = a + o_k1;
   b->ids[0] = (uint32_t*)(a + o_i0);

@@ -50,6 +50,7 @@ struct rt_bvh {
   uint32_t* arrivals = nullptr;  // n-1 arrival counters of the top-level climb (contiguous after status)
   float4* deposits = nullptr;    // (n-1)*2 child deposits, 32 B each
   uint32_t* deferred = nullptr;  // n climb hand-offs to the top-level pass
+  uint32_t* fetch = nullptr;     // 1024 words: ray fetch counters of rt_trace
   void* keys[2] = {nullptr, nullptr};
   uint32_t* ids[2] = {nullptr, nullptr};
   int32_t* parent = nullptr;     // 2n-1
@@ -70,6 +71,7 @@ constexpr int kMiscTileCtr = 48;    // up to 8 words: onesweep tile counters
 constexpr int kMiscOverflow = 56;   // 1 word: traversal stack overflows
 constexpr int kMiscDeferred = 57;   // 1 word: number of climb hand-offs
 constexpr int kMiscBuildError = 58; // 1 word: climb consistency failures (must stay 0)
+constexpr int kMiscFetch = 59;      // 1 word: ray fetch counter of rt_trace
 constexpr int kMiscWords = 64;
 
 // Fused hierarchy + refit climb: one workgroup per chunk of sorted leaves.

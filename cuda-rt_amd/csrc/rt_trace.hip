@@ -305,6 +305,259 @@ __device__ __forceinline__ int32_t leaf_tri(const BvhView& B, int pos) {
   return __float_as_int(B.ltri[3 * (size_t)pos + 2].y);
 }
 
+__device__ __forceinline__ int32_t leaf_tri_of(const BvhView& B, int pos) {
+  return __float_as_int(B.ltri[3 * (size_t)pos + 2].y);
+}
+
+__device__ __forceinline__ void flush_stats(unsigned long long* counters, const TravStats& st) {
+  if (!counters) return;
+  unsigned long long vis = st.visits, tst = st.tests;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    vis += __shfl_xor(vis, off);
+    tst += __shfl_xor(tst, off);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (vis) atomicAdd(&counters[3], vis);
+    if (tst) atomicAdd(&counters[4], tst);
+  }
+}
+
+// ------------------------------------------------------------ persistent traversal
+// Ray-queue traversal with dynamic ray fetch: each lane walks one ray a node
+// per iteration and, as soon as its ray finishes, takes the next ray index
+// from a chunk its wave reserved with one atomic (kFetchChunk rays), so the
+// wave keeps its lanes busy until the queue drains instead of idling behind
+// its longest ray. Closest-hit and any-hit use the same decisions as
+// closest_hit()/any_hit() above, one node (or, for any-hit, one popped leaf)
+// per step.
+constexpr uint32_t kFetchChunk = 256;
+constexpr uint32_t kSegments = 8;       // one per XCD
+constexpr uint32_t kFetchStride = 32;   // counters 128 bytes apart
+constexpr uint32_t kFetchWords = kSegments * kFetchStride;
+
+struct RayIn {
+  const double* orig;  // origin of ray i at orig + i*stride
+  const double* dir;   // direction of ray i at dir + i*stride
+  int stride;          // in doubles
+  const uint32_t* count_dev;  // number of rays (device), or nullptr
+  uint32_t count;             // used when count_dev is nullptr
+};
+
+struct HitOut {
+  int32_t* pos;  // sorted leaf position or -1 (nullable)
+  int32_t* tri;  // triangle id or -1 (nullable)
+  double *t, *al, *be;  // nullable
+  double* tab;   // nullable: t, alpha, beta interleaved (3 per ray)
+};
+
+__device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uint32_t i, int pos, double t, double al,
+                                          double be) {
+  if (O.pos) O.pos[i] = pos;
+  if (O.tri) O.tri[i] = pos >= 0 ? leaf_tri_of(B, pos) : -1;
+  if (O.t) O.t[i] = pos >= 0 ? t : 0.0;
+  if (O.al) O.al[i] = pos >= 0 ? al : 0.0;
+  if (O.be) O.be[i] = pos >= 0 ? be : 0.0;
+  if (O.tab) {
+    O.tab[3 * (size_t)i] = pos >= 0 ? t : 0.0;
+    O.tab[3 * (size_t)i + 1] = pos >= 0 ? al : 0.0;
+    O.tab[3 * (size_t)i + 2] = pos >= 0 ? be : 0.0;
+  }
+}
+
+template <bool ANY>
+__global__ __launch_bounds__(kThreads) void k_trace_persistent(BvhView B, RayIn R, double tmin_phys, HitOut O,
+                                                               uint32_t* fetch_ctr, uint32_t* overflow,
+                                                               unsigned long long* counters) {
+  __shared__ int s_stack[kRing * kThreads];
+  const uint32_t nrays = R.count_dev ? *R.count_dev : R.count;
+  const int n = B.n;
+  const uint32_t lane = __lane_id();
+  const uint64_t mlt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  TravStats st;
+  RingStack S;
+  S.col = s_stack + threadIdx.x;
+  int spill[kSpill];
+  RayD r;
+  bool active = false;
+  uint32_t rid = 0;
+  int node = 0;
+  double bt = 0, bal = 0, bbe = 0;
+  int bpos = -1;
+  uint32_t chunk_next = 0, chunk_end = 0;
+  bool drained = (n <= 0);
+  // ray range split into kSegments contiguous segments of whole chunks
+  const uint32_t seg_len = ((nrays + kSegments - 1) / kSegments + kFetchChunk - 1) / kFetchChunk * kFetchChunk;
+  uint32_t seg = blockIdx.x % kSegments, tries = 0;
+  if (n <= 0) {
+    // Empty BVH: every ray misses.
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < nrays; i += gridDim.x * kThreads)
+      write_hit(B, O, i, -1, 0, 0, 0);
+    return;
+  }
+  while (true) {
+    // ---- refill idle lanes from the wave's chunk
+    uint64_t need = __ballot(!active);
+    if (need != 0 && !drained) {
+      uint32_t k = (uint32_t)__popcll(need);
+      uint32_t rank = (uint32_t)__popcll(need & mlt);
+      uint32_t avail = chunk_end - chunk_next;
+      uint32_t myid = 0xffffffffu;
+      if (!active && rank < avail) myid = chunk_next + rank;
+      uint32_t used = min(k, avail);
+      chunk_next += used;
+      if (k > avail) {
+        // Next chunk: the workgroup's home segment first (one per XCD, so
+        // an XCD's L2 serves one screen region), then the others in turn.
+        uint32_t base = 0xffffffffu, end = 0;
+        while (tries < kSegments) {
+          uint32_t seg_lo = seg * seg_len, seg_hi = min(nrays, seg_lo + seg_len);
+          uint32_t b = 0;
+          if (lane == 0) b = atomicAdd(fetch_ctr + seg * kFetchStride, kFetchChunk);
+          b = __shfl(b, 0) + seg_lo;
+          if (b < seg_hi) {
+            base = b;
+            end = min(b + kFetchChunk, seg_hi);
+            break;
+          }
+          seg = (seg + 1) % kSegments;
+          ++tries;
+        }
+        if (base == 0xffffffffu) {
+          drained = true;
+        } else {
+          chunk_next = base;
+          chunk_end = end;
+          uint32_t avail2 = chunk_end - chunk_next;
+          uint32_t r2 = rank - avail;
+          if (!active && rank >= avail && r2 < avail2) myid = chunk_next + r2;
+          chunk_next += min(k - avail, avail2);
+        }
+      }
+      if (myid != 0xffffffffu) {
+        rid = myid;
+        const double* po = R.orig + (size_t)rid * R.stride;
+        const double* pd = R.dir + (size_t)rid * R.stride;
+        r = make_ray(mk(po[0], po[1], po[2]), mk(pd[0], pd[1], pd[2]));
+        bpos = -1;
+        bt = 0;
+        double t0, t1;
+        const float* rb = B.node_box;
+        if (!slab(r, rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], t0, t1)) {
+          write_hit(B, O, rid, -1, 0, 0, 0);
+        } else if (n == 1) {
+          double t, al, be;
+          ++st.tests;
+          bool ok = tri_test(r, B.ltri, t, al, be);
+          if (!ANY && tmin_phys > 0 && !(t >= tmin_phys)) ok = false;
+          write_hit(B, O, rid, ok ? 0 : -1, t, al, be);
+        } else {
+          active = true;
+          node = 0;
+          S.sp = 0;
+          S.lo = 0;
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (drained) break;
+      continue;
+    }
+    if (!active) continue;
+    // ---- one traversal step
+    bool done = false;
+    if (!ANY) {
+      float4 q0, q1, q2, q3;
+      load_node(B.tnode, node, q0, q1, q2, q3);
+      ++st.visits;
+      double l0, l1, r0, r1;
+      bool hl = slab(r, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, l0, l1);
+      bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
+      int cl = __float_as_int(q3.x), cr = __float_as_int(q3.y);
+      if (tmin_phys > 0) {
+        hl = hl && !(l1 < tmin_phys - prune_margin(tmin_phys));
+        hr = hr && !(r1 < tmin_phys - prune_margin(tmin_phys));
+      }
+      if (bpos >= 0) {
+        double lim = bt + prune_margin(bt);
+        hl = hl && !(l0 > lim);
+        hr = hr && !(r0 > lim);
+      }
+      HitRec best{bt, bal, bbe, bpos};
+      if (hl && cl < 0) {
+        consider_leaf(B, r, ~cl, tmin_phys, best, st);
+        hl = false;
+      }
+      if (hr && cr < 0) {
+        consider_leaf(B, r, ~cr, tmin_phys, best, st);
+        hr = false;
+      }
+      bt = best.t;
+      bal = best.al;
+      bbe = best.be;
+      bpos = best.pos;
+      if (hl && hr) {
+        int nearc = (l0 <= r0) ? cl : cr, farc = (l0 <= r0) ? cr : cl;
+        if (!stack_push(S, spill, farc)) {
+          atomicAdd(overflow, 1u);
+          bpos = -1;
+          done = true;
+        }
+        node = nearc;
+      } else if (hl) {
+        node = cl;
+      } else if (hr) {
+        node = cr;
+      } else if (S.sp == 0) {
+        done = true;
+      } else {
+        node = stack_pop(S, spill);
+      }
+    } else {
+      // node >= 0: internal node to enter; node < 0: leaf whose box passed.
+      if (node < 0) {
+        double t, al, be;
+        ++st.tests;
+        if (tri_test(r, B.ltri + 3 * (size_t)(~node), t, al, be)) {
+          bt = t; bal = al; bbe = be; bpos = ~node;
+          done = true;
+        } else if (S.sp == 0) {
+          done = true;
+        } else {
+          node = stack_pop(S, spill);
+        }
+      } else {
+        float4 q0, q1, q2, q3;
+        load_node(B.tnode, node, q0, q1, q2, q3);
+        ++st.visits;
+        double l0, l1, r0, r1;
+        bool hl = slab(r, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, l0, l1);
+        bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
+        int cl = __float_as_int(q3.x), cr = __float_as_int(q3.y);
+        if (hl) {
+          if (hr && !stack_push(S, spill, cr)) {
+            atomicAdd(overflow, 1u);
+            bpos = -1;
+            done = true;
+          }
+          node = cl;
+        } else if (hr) {
+          node = cr;
+        } else if (S.sp == 0) {
+          done = true;
+        } else {
+          node = stack_pop(S, spill);
+        }
+      }
+    }
+    if (done) {
+      write_hit(B, O, rid, bpos, bt, bal, bbe);
+      active = false;
+    }
+  }
+  flush_stats(counters, st);
+}
+
 // ------------------------------------------------------------ trace kernel
 __global__ __launch_bounds__(kThreads) void k_trace(BvhView B, const double* __restrict__ orig,
                                                     const double* __restrict__ dir, long nrays, int mode, double tmin,
@@ -561,23 +814,29 @@ __global__ __launch_bounds__(kThreads) void k_render(BvhView B, MeshShade M, con
 }
 
 // ------------------------------------------------------------ wavefront
-// The same per-pixel computation split into passes over ray queues: per
-// sample, a camera pass, then for each bounce a closest-hit pass (which
-// also draws the light sample and, if the path continues, the bounce
-// direction) and an any-hit pass over the compacted shadow queue. Per-pixel
-// RNG state and partial sums live in HBM between passes; each pixel's
-// draws and additions happen in the megakernel's order, so both give the
-// same image bit for bit.
+// The same per-pixel computation split into passes: per sample a camera
+// pass, then per bounce a persistent closest-hit traversal, a shading pass
+// (which draws the light sample and, if the path continues, the bounce
+// direction, and appends the shadow ray and the next ray to queues), a
+// persistent any-hit traversal of the shadow queue and a pass that adds the
+// shaded result. Per-pixel RNG state and partial sums stay in HBM between
+// passes; each pixel's draws and additions happen in the megakernel's order,
+// so both give the same image bit for bit.
 struct WfState {
-  uint32_t* rng;        // W*H
-  double* color;        // W*H*3: sum over finished samples
-  double* acc;          // W*H*3: current sample's path sum
-  double* thr;          // W*H*3: current path throughput
-  double* ray;          // W*H*6: current ray origin, direction
-  uint32_t* queue[2];   // W*H pixel ids of live rays (ping-pong)
-  uint32_t* shq_pix;    // W*H shadow-queue pixel ids
-  double* shq;          // W*H*16: q, L, I1, ds, thr, fo
-  uint32_t* counts;     // [0] live rays in queue[0], [1] in queue[1], [2] shadow entries
+  uint32_t* rng;        // np
+  double* color;        // np*3: sum over finished samples
+  double* acc;          // np*3: current sample's path sum
+  double* thr;          // np*3: current path throughput
+  double* ray[2];       // cap*6: ray origin, direction per queue slot
+  uint32_t* pix[2];     // cap: pixel of each queue slot (~0 = unused slot)
+  int32_t* hpos;        // cap: closest hit leaf position or -1
+  double* hit;          // cap*3: t, alpha, beta
+  double* shq;          // cap*16: q, L, I1, ds, thr, fo
+  uint32_t* spix;       // cap: pixel of each shadow entry
+  int32_t* spos;        // cap: any-hit leaf position or -1
+  double* st;           // cap: any-hit t
+  uint32_t* counts;     // [0],[1] ray queue sizes, [2] shadow entries
+  uint32_t* fetch[2];   // kFetchWords each: closest-hit and any-hit fetch counters
 };
 
 __device__ __forceinline__ d3 ld3(const double* p) { return mk(p[0], p[1], p[2]); }
@@ -587,94 +846,107 @@ __device__ __forceinline__ void st3(double* p, d3 v) {
   p[2] = v.z;
 }
 
-// Append to a queue with one atomic per wave.
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
-  uint64_t m = __ballot(want);
-  if (m == 0) return 0;
+// Append to a queue with one atomic per workgroup; every thread must call it.
+__device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool want, uint32_t* s_tmp) {
   const uint32_t lane = __lane_id();
-  uint32_t leader = (uint32_t)__builtin_ctzll(m);
-  uint32_t base = 0;
-  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, (int)leader);
+  const int w = threadIdx.x >> 6;
+  uint64_t m = __ballot(want);
+  if (lane == 0) s_tmp[w] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int j = 0; j < kThreads / 64; ++j) tot += s_tmp[j];
+    s_tmp[kThreads / 64] = tot ? atomicAdd(counter, tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t base = s_tmp[kThreads / 64];
+  for (int j = 0; j < w; ++j) base += s_tmp[j];
   uint64_t below = lane == 0 ? 0ull : (m & (~0ull >> (64 - lane)));
-  return base + (uint32_t)__popcll(below);
+  uint32_t r = base + (uint32_t)__popcll(below);
+  __syncthreads();
+  return r;
 }
 
 __global__ __launch_bounds__(kThreads) void k_wf_init(RenderParams P, WfState W) {
   int px, py;
   pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
+  if (blockIdx.x == 0 && threadIdx.x < 8) W.counts[threadIdx.x] = 0u;
   if (px >= P.W || py >= P.H) return;
   size_t p = (size_t)py * P.W + px;
   W.rng[p] = rt::wang_hash(rt::seed_index(px, py, P.W) ^ P.seed);
   st3(W.color + 3 * p, mk(0, 0, 0));
 }
 
-// Start sample: primary ray, throughput 1, empty path sum, every pixel live.
-__global__ __launch_bounds__(kThreads) void k_wf_camera(RenderParams P, WfState W) {
+// Start a sample: slot = tile*256 + lane holds the pixel's primary ray.
+// Slots outside the image get a NaN ray, which misses the root box.
+__global__ __launch_bounds__(kThreads) void k_wf_camera(RenderParams P, WfState W, uint32_t nslots) {
   int px, py;
   pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
-  const bool active = px < P.W && py < P.H;
-  if (active) {
-    size_t p = (size_t)py * P.W + px;
-    uint32_t rng = W.rng[p];
-    double xx, yy;
-    pixel_xy(P, px, py, xx, yy);
-    d3 o, d;
-    lens_ray(P, xx, yy, rng, o, d);
-    W.rng[p] = rng;
-    st3(W.ray + 6 * p, o);
-    st3(W.ray + 6 * p + 3, d);
-    st3(W.thr + 3 * p, mk(1, 1, 1));
-    st3(W.acc + 3 * p, mk(0, 0, 0));
+  const size_t slot = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  if (blockIdx.x == 0 && threadIdx.x == 0) W.counts[0] = nslots;
+  double* rr = W.ray[0] + 6 * slot;
+  if (px >= P.W || py >= P.H) {
+    const double nan = __builtin_nan("");
+    for (int k = 0; k < 6; ++k) rr[k] = nan;
+    W.pix[0][slot] = 0xffffffffu;
+    return;
   }
-  uint32_t slot = wave_append(&W.counts[0], active);
-  if (active) W.queue[0][slot] = (uint32_t)(py * P.W + px);
+  size_t p = (size_t)py * P.W + px;
+  uint32_t rng = W.rng[p];
+  double xx, yy;
+  pixel_xy(P, px, py, xx, yy);
+  d3 o, d;
+  lens_ray(P, xx, yy, rng, o, d);
+  W.rng[p] = rng;
+  st3(rr, o);
+  st3(rr + 3, d);
+  W.pix[0][slot] = (uint32_t)p;
+  st3(W.thr + 3 * p, mk(1, 1, 1));
+  st3(W.acc + 3 * p, mk(0, 0, 0));
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_closest(BvhView B, MeshShade M, const rt_material* __restrict__ mats,
-                                                         RenderParams P, WfState W, int bounce, int qin,
-                                                         unsigned long long* counters, uint32_t* overflow) {
-  __shared__ int s_stack[kRing * kThreads];
-  const uint32_t n_in = W.counts[qin];
+__global__ __launch_bounds__(kThreads) void k_wf_shade(BvhView B, MeshShade M, const rt_material* __restrict__ mats,
+                                                       RenderParams P, WfState W, int bounce, int cur,
+                                                       unsigned long long* counters) {
+  __shared__ uint32_t s_tmp[8];
+  const uint32_t n_in = W.counts[cur];
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  const bool live = i < n_in;
-  TravStats st;
+  uint32_t p = i < n_in ? W.pix[cur][i] : 0xffffffffu;
+  const bool live = p != 0xffffffffu;
   bool want_shadow = false, want_next = false;
-  uint32_t p = 0;
   ShadePrep S;
-  d3 thr = mk(0, 0, 0), nray_o = mk(0, 0, 0), nray_d = mk(0, 0, 0), nthr = mk(0, 0, 0);
+  d3 thr = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
   if (live) {
-    p = W.queue[qin][i];
-    d3 o = ld3(W.ray + 6 * (size_t)p), d = ld3(W.ray + 6 * (size_t)p + 3);
     thr = ld3(W.thr + 3 * (size_t)p);
-    HitRec h;
-    closest_hit(B, make_ray(o, d), bounce == 0 ? 0.0 : kBounceTmin, h, overflow, st, s_stack + threadIdx.x);
-    if (h.pos < 0) {
+    const int pos = W.hpos[i];
+    if (pos < 0) {
       double* a = W.acc + 3 * (size_t)p;
-      d3 acc = ld3(a) + rt::hadamard(thr, mk(0.9, 0.9, 0.9));
-      st3(a, acc);
+      st3(a, ld3(a) + rt::hadamard(thr, mk(0.9, 0.9, 0.9)));
     } else {
+      const double* rr = W.ray[cur] + 6 * (size_t)i;
+      d3 o = ld3(rr), d = ld3(rr + 3);
+      const double* hh = W.hit + 3 * (size_t)i;
       uint32_t rng = W.rng[p];
-      const int tri = leaf_tri(B, h.pos);
+      const int tri = leaf_tri_of(B, pos);
       const rt_material& m = mat_of(mats, P.nmat, M, tri);
-      S = shade_prep(P, m, shading_normal(M, tri, h.al, h.be), o, d, h.t, rng);
+      S = shade_prep(P, m, shading_normal(M, tri, hh[1], hh[2]), o, d, hh[0], rng);
       want_shadow = true;
       if (bounce < P.depth) {
         double u1 = rt::rand_unit(rng);
         double u2 = rt::rand_unit(rng);
         d3 N = S.N;
         if (rt::dot(N, d) > 0) N = mk(-N.x, -N.y, -N.z);
-        nthr = rt::hadamard(thr, arr3(m.kd));
-        nray_o = S.q;
-        nray_d = cosine_dir(N, u1, u2);
+        st3(W.thr + 3 * (size_t)p, rt::hadamard(thr, arr3(m.kd)));
+        no = S.q;
+        nd = cosine_dir(N, u1, u2);
         want_next = true;
       }
       W.rng[p] = rng;
     }
   }
-  uint32_t ss = wave_append(&W.counts[2], want_shadow);
+  uint32_t ss = block_append(&W.counts[2], want_shadow, s_tmp);
   if (want_shadow) {
-    W.shq_pix[ss] = p;
+    W.spix[ss] = p;
     double* e = W.shq + 16 * (size_t)ss;
     st3(e + 0, S.q);
     st3(e + 3, S.L);
@@ -683,49 +955,57 @@ __global__ __launch_bounds__(kThreads) void k_wf_closest(BvhView B, MeshShade M,
     st3(e + 12, thr);
     e[15] = S.fo;
   }
-  uint32_t ns = wave_append(&W.counts[qin ^ 1], want_next);
+  uint32_t ns = block_append(&W.counts[cur ^ 1], want_next, s_tmp);
   if (want_next) {
-    W.queue[qin ^ 1][ns] = p;
-    st3(W.ray + 6 * (size_t)p, nray_o);
-    st3(W.ray + 6 * (size_t)p + 3, nray_d);
-    st3(W.thr + 3 * (size_t)p, nthr);
+    W.pix[cur ^ 1][ns] = p;
+    double* rr = W.ray[cur ^ 1] + 6 * (size_t)ns;
+    st3(rr, no);
+    st3(rr + 3, nd);
   }
-  flush_counters(counters, bounce == 0 && live ? 1ull : 0ull, 0ull, bounce > 0 && live ? 1ull : 0ull, st);
+  if (counters) {
+    // rays traced by the closest-hit pass of this bounce
+    uint64_t m = __ballot(live);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&counters[bounce == 0 ? 0 : 2], (unsigned long long)__popcll(m));
+  }
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_shadow(BvhView B, MeshShade M, const rt_material* __restrict__ mats,
-                                                        RenderParams P, WfState W, unsigned long long* counters,
-                                                        uint32_t* overflow) {
-  __shared__ int s_stack[kRing * kThreads];
+__global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(BvhView B, MeshShade M,
+                                                               const rt_material* __restrict__ mats, RenderParams P,
+                                                               WfState W, unsigned long long* counters) {
   const uint32_t n_in = W.counts[2];
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   const bool live = i < n_in;
-  TravStats st;
-  if (live) {
-    const double* e = W.shq + 16 * (size_t)i;
-    d3 q = ld3(e + 0), L = ld3(e + 3);
-    HitRec s;
-    bool hit = any_hit(B, make_ray(q, L), s, overflow, st, s_stack + threadIdx.x);
-    d3 I = shade_finish(P, ld3(e + 6), e[15], ld3(e + 9), shadow_factor(P, B, mats, M, q, hit, s));
-    uint32_t p = W.shq_pix[i];
-    double* a = W.acc + 3 * (size_t)p;
-    st3(a, ld3(a) + rt::hadamard(ld3(e + 12), I));
+  if (counters) {
+    uint64_t m = __ballot(live);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&counters[1], (unsigned long long)__popcll(m));
   }
-  flush_counters(counters, 0ull, live ? 1ull : 0ull, 0ull, st);
+  if (!live) return;
+  const double* e = W.shq + 16 * (size_t)i;
+  d3 q = ld3(e + 0);
+  HitRec s;
+  s.pos = W.spos[i];
+  s.t = W.st[i];
+  d3 I = shade_finish(P, ld3(e + 6), e[15], ld3(e + 9), shadow_factor(P, B, mats, M, q, s.pos >= 0, s));
+  uint32_t p = W.spix[i];
+  double* a = W.acc + 3 * (size_t)p;
+  st3(a, ld3(a) + rt::hadamard(ld3(e + 12), I));
 }
 
-// End of a sample: color += path sum; clear the queue counters.
+// Reset the listed counters and, if fetch != nullptr, kWords fetch
+// counters (graph-safe replacement for a memset).
+__global__ void k_wf_reset(uint32_t* counts, uint32_t mask, uint32_t* fetch, uint32_t words) {
+  if (counts && threadIdx.x < 32 && (mask >> threadIdx.x) & 1u) counts[threadIdx.x] = 0u;
+  if (fetch)
+    for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) fetch[k] = 0u;
+}
+
+// End of a sample: color += path sum.
 __global__ __launch_bounds__(kThreads) void k_wf_flush(RenderParams P, WfState W) {
   int px, py;
   pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
-  if (blockIdx.x == 0 && threadIdx.x < 3) W.counts[threadIdx.x] = 0u;
   if (px >= P.W || py >= P.H) return;
   size_t p = (size_t)py * P.W + px;
   st3(W.color + 3 * p, ld3(W.color + 3 * p) + ld3(W.acc + 3 * p));
-}
-
-__global__ __launch_bounds__(kThreads) void k_wf_reset(uint32_t* counts, int which) {
-  if (threadIdx.x == 0) counts[which] = 0u;
 }
 
 __global__ __launch_bounds__(kThreads) void k_wf_resolve(RenderParams P, WfState W, float* __restrict__ rgb) {
@@ -768,22 +1048,37 @@ RenderParams make_params(const rt_camera* cam, const rt_light* light, const rt_r
 
 size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 
+int persistent_blocks(const rt_bvh* b, uint32_t max_rays) {
+  int cap = (b->ctx ? b->ctx->num_cus : 256) * 5;
+  int need = (int)((max_rays + kThreads - 1) / kThreads);
+  return need < cap ? (need > 0 ? need : 1) : cap;
+}
+
 }  // namespace
 
 hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays* rays, int32_t mode, double tmin,
                            const rt_hits* hits, hipStream_t s) {
   (void)m;
   if (rays->n <= 0) return hipSuccess;
-  k_trace<<<cdiv(rays->n, kThreads), kThreads, 0, s>>>(b->view(), rays->orig_dev, rays->dir_dev, rays->n, mode, tmin,
-                                                      hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev,
-                                                      b->misc + kMiscOverflow);
+  uint32_t* fetch = b->fetch;
+  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, fetch, kFetchWords);
+  RayIn R{rays->orig_dev, rays->dir_dev, 3, nullptr, (uint32_t)rays->n};
+  HitOut O{nullptr, hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev, nullptr};
+  int grid = persistent_blocks(b, (uint32_t)rays->n);
+  if (mode == RT_TRACE_ANY)
+    k_trace_persistent<true><<<grid, kThreads, 0, s>>>(b->view(), R, 0.0, O, fetch, b->misc + kMiscOverflow, nullptr);
+  else
+    k_trace_persistent<false><<<grid, kThreads, 0, s>>>(b->view(), R, tmin, O, fetch, b->misc + kMiscOverflow,
+                                                       nullptr);
   return hipGetLastError();
 }
 
 size_t rt_wavefront_bytes(int32_t width, int32_t height) {
   size_t np = (size_t)width * height;
-  return align256(np * 4) + 2 * align256(np * 24) + align256(np * 24) + align256(np * 48) + 2 * align256(np * 4) +
-         align256(np * 4) + align256(np * 128) + 256;
+  size_t cap = (size_t)((width + 15) / 16) * ((height + 15) / 16) * kThreads;
+  return align256(np * 4) + 3 * align256(np * 24) + 2 * align256(cap * 48) + 2 * align256(cap * 4) +
+         align256(cap * 4) + align256(cap * 24) + align256(cap * 128) + 3 * align256(cap * 8) + 256 +
+         2 * align256(kFetchWords * 4);
 }
 
 hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
@@ -799,6 +1094,7 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
     return hipGetLastError();
   }
   const size_t np = (size_t)P.W * P.H;
+  const size_t cap = (size_t)tiles * kThreads;
   char* w = (char*)workspace;
   size_t off = 0;
   auto take = [&](size_t bytes) {
@@ -811,26 +1107,39 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   W.color = (double*)take(np * 24);
   W.acc = (double*)take(np * 24);
   W.thr = (double*)take(np * 24);
-  W.ray = (double*)take(np * 48);
-  W.queue[0] = (uint32_t*)take(np * 4);
-  W.queue[1] = (uint32_t*)take(np * 4);
-  W.shq_pix = (uint32_t*)take(np * 4);
-  W.shq = (double*)take(np * 128);
+  W.ray[0] = (double*)take(cap * 48);
+  W.ray[1] = (double*)take(cap * 48);
+  W.pix[0] = (uint32_t*)take(cap * 4);
+  W.pix[1] = (uint32_t*)take(cap * 4);
+  W.hpos = (int32_t*)take(cap * 4);
+  W.hit = (double*)take(cap * 24);
+  W.shq = (double*)take(cap * 128);
+  W.spix = (uint32_t*)take(cap * 8);
+  W.spos = (int32_t*)take(cap * 8);
+  W.st = (double*)take(cap * 8);
   W.counts = (uint32_t*)take(256);
-  const int qblocks = cdiv((long)np, kThreads);
-  k_wf_reset<<<1, kThreads, 0, s>>>(W.counts, 0);
-  k_wf_reset<<<1, kThreads, 0, s>>>(W.counts, 1);
-  k_wf_reset<<<1, kThreads, 0, s>>>(W.counts, 2);
+  W.fetch[0] = (uint32_t*)take(kFetchWords * 4);
+  W.fetch[1] = (uint32_t*)take(kFetchWords * 4);
+  const int qblocks = tiles;
+  const int grid = persistent_blocks(b, (uint32_t)cap);
+  const BvhView V = b->view();
   k_wf_init<<<tiles, kThreads, 0, s>>>(P, W);
+  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W.fetch[0], 2 * kFetchWords);
   for (int it = 0; it < P.spp; ++it) {
-    k_wf_camera<<<tiles, kThreads, 0, s>>>(P, W);
-    int qin = 0;
+    k_wf_camera<<<tiles, kThreads, 0, s>>>(P, W, (uint32_t)cap);
+    int cur = 0;
     for (int bounce = 0; bounce <= P.depth; ++bounce) {
-      k_wf_closest<<<qblocks, kThreads, 0, s>>>(b->view(), M, mats, P, W, bounce, qin, cnt, ovf);
-      k_wf_shadow<<<qblocks, kThreads, 0, s>>>(b->view(), M, mats, P, W, cnt, ovf);
-      k_wf_reset<<<1, kThreads, 0, s>>>(W.counts, 2);
-      k_wf_reset<<<1, kThreads, 0, s>>>(W.counts, qin);
-      qin ^= 1;
+      RayIn R{W.ray[cur], W.ray[cur] + 3, 6, &W.counts[cur], 0};
+      HitOut O{W.hpos, nullptr, nullptr, nullptr, nullptr, W.hit};
+      k_trace_persistent<false><<<grid, kThreads, 0, s>>>(V, R, bounce == 0 ? 0.0 : kBounceTmin, O, W.fetch[0],
+                                                          ovf, cnt);
+      k_wf_shade<<<qblocks, kThreads, 0, s>>>(V, M, mats, P, W, bounce, cur, cnt);
+      RayIn SR{W.shq, W.shq + 3, 16, &W.counts[2], 0};
+      HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr};
+      k_trace_persistent<true><<<grid, kThreads, 0, s>>>(V, SR, 0.0, SO, W.fetch[1], ovf, cnt);
+      k_wf_shadow_finish<<<qblocks, kThreads, 0, s>>>(V, M, mats, P, W, cnt);
+      k_wf_reset<<<1, 256, 0, s>>>(W.counts, (1u << 2) | (1u << cur), W.fetch[0], 2 * kFetchWords);
+      cur ^= 1;
     }
     k_wf_flush<<<tiles, kThreads, 0, s>>>(P, W);
   }
