@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=16)
+    ap.add_argument("--workload", choices=["soup1M", "batch64"], default="soup1M",
+                    help="soup1M: the headline single-scene step; batch64: 64 scenes, batched vs per-scene loop")
+    ap.add_argument("--scenes", type=int, default=64, help="batch64: number of scenes")
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront",
                     help="render kernel structure (identical images)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -89,6 +92,80 @@ def cpu_baseline(hm, mats, light, cam, opts, stride, tris):
     }
 
 
+def bench_batch(args):
+    """64 random soups (log-uniform 1K..2M triangles), cameras of 256^2, 512^2
+    or 1024^2 at 1..8 spp. Step = rebuild every BVH + render every scene.
+    Compares the per-scene launch loop with the batched form replayed from a
+    hipGraph, and checks that the images are identical."""
+    import numpy as np
+    import torch
+
+    import rtamd
+    from tests import scenes
+    ctx = rtamd.Context(0)
+    specs = scenes.batch_scenes(args.scenes, 0x5EED0004)
+    sc = []
+    for sp in specs:
+        dm = rtamd.DeviceMesh(sp["hm"])
+        sp["opts"].flags |= rtamd.RT_RENDER_WAVEFRONT
+        sc.append(dict(mesh=dm, bvh=rtamd.BVH(ctx, dm, 63), mats=rtamd.materials_tensor(sp["mats"]),
+                       nmat=len(sp["mats"]), light=sp["light"], cam=sp["cam"], opts=sp["opts"],
+                       rgb=torch.zeros(sp["cam"].width * sp["cam"].height * 3, dtype=torch.float32, device="cuda")))
+    tris = sum(s["mesh"].n for s in sc)
+    pixels = sum(s["cam"].width * s["cam"].height for s in sc)
+    primary = sum(s["cam"].width * s["cam"].height * s["opts"].spp for s in sc)
+    loop_rgb = [torch.zeros_like(s["rgb"]) for s in sc]
+
+    def loop_step():
+        for s, out in zip(sc, loop_rgb):
+            s["bvh"].rebuild(s["mesh"])
+            rtamd.render(ctx, s["bvh"], s["mesh"], s["mats"], s["nmat"], s["light"], s["cam"], s["opts"], rgb=out)
+
+    def timed(fn, steps, warmup):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
+
+    loop_ms = timed(loop_step, args.steps, args.warmup)
+    batch = rtamd.Batch(ctx, sc)
+    best, tuned_ms = batch.autotune()
+    batch.rebuild()
+    batch.render()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            batch.rebuild()
+            batch.render()
+    torch.cuda.current_stream().wait_stream(side)
+    batch_ms = timed(g.replay, args.steps, args.warmup)
+    same = all(torch.equal(a, s["rgb"]) for a, s in zip(loop_rgb, sc))
+    out = {
+        "metric": "batched hipGraph step vs per-scene launch loop (64 scenes, build + render)",
+        "value": round(loop_ms / batch_ms, 3),
+        "unit": "x speed-up",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(batch_ms, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"batch{len(sc)}_soups_1K-2M_tris_256-1024px_1-8spp", "scenes": len(sc),
+                   "tris": tris, "pixels": pixels, "primary_rays": primary},
+        "loop_ms": round(loop_ms, 3), "batch_graph_ms": round(batch_ms, 3),
+        "batch_mrays_s": round(primary / (batch_ms * 1e-3) / 1e6, 2),
+        "loop_mrays_s": round(primary / (loop_ms * 1e-3) / 1e6, 2),
+        "identical_images": bool(same),
+        "tuning": best.as_dict(), "autotune_best_render_ms": round(tuned_ms, 3),
+        "stack_overflows": batch.diagnostics(),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -96,6 +173,9 @@ def main():
         sys.exit(2)
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.workload == "batch64":
+        bench_batch(args)
         return
     import numpy as np
     import torch

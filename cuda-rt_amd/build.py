@@ -12,7 +12,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librtamd.so")
 
-HIP_SOURCES = ["rt_build.hip", "rt_trace.hip", "rt_api.hip"]
+HIP_SOURCES = ["rt_build.hip", "rt_trace.hip", "rt_api.hip", "rt_batch.hip"]
 CXX_SOURCES = ["rt_scenes.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,206 @@
+// Batched rendering: many scenes through one set of wavefront launches
+// (declared in include/rt_api.h).
+#include <hip/hip_runtime.h>
+
+#include <string.h>
+
+#include <new>
+#include <vector>
+
+#include "rt_internal.h"
+
+#define RT_HIP(ctx, call)                                                                       \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess)                                                                       \
+      return rt_set_error((ctx), RT_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+constexpr int kBuildStreams = 8;
+}
+
+struct rt_batch {
+  rt_ctx* ctx = nullptr;
+  std::vector<rt_batch_item> items;
+  void* ws = nullptr;
+  size_t npix = 0, ntiles = 0, nscenes = 0;
+  int max_spp = 0, max_depth = 0;
+  rt_tuning tuning;
+  hipStream_t streams[kBuildStreams] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t joins[kBuildStreams] = {};
+};
+
+extern "C" {
+
+void rt_batch_destroy(rt_batch* b) {
+  if (!b) return;
+  for (int k = 0; k < kBuildStreams; ++k) {
+    if (b->joins[k]) (void)hipEventDestroy(b->joins[k]);
+    if (b->streams[k]) (void)hipStreamDestroy(b->streams[k]);
+  }
+  if (b->fork) (void)hipEventDestroy(b->fork);
+  if (b->ws) (void)hipFree(b->ws);
+  delete b;
+}
+
+rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count, rt_batch** out) {
+  if (!ctx || !out) return RT_ERR_INVALID;
+  *out = nullptr;
+  if (!items || count <= 0 || count > 65535) return rt_set_error(ctx, RT_ERR_INVALID, "need 1..65535 batch items");
+  for (int i = 0; i < count; ++i) {
+    const rt_batch_item& it = items[i];
+    if (!it.bvh || !it.mats_dev || it.nmat <= 0 || !it.rgb_dev || !it.mesh.nx_dev || !it.mesh.ny_dev ||
+        !it.mesh.nz_dev || !it.mesh.idx_dev || it.mesh.n != rt_bvh_num_triangles(it.bvh) || it.cam.width <= 0 ||
+        it.cam.height <= 0 || it.opts.spp <= 0 || it.opts.depth < 0)
+      return rt_set_error(ctx, RT_ERR_INVALID, "invalid batch item %d", i);
+  }
+  rt_batch* b = new (std::nothrow) rt_batch();
+  if (!b) return rt_set_error(ctx, RT_ERR_OOM, "host allocation failed");
+  b->ctx = ctx;
+  b->items.assign(items, items + count);
+  b->nscenes = (size_t)count;
+  b->tuning = rt_default_tuning();
+  std::vector<size_t> pix_off(count), tile_off(count);
+  for (int i = 0; i < count; ++i) {
+    const rt_camera& c = items[i].cam;
+    pix_off[i] = b->npix;
+    tile_off[i] = b->ntiles;
+    b->npix += (size_t)c.width * c.height;
+    b->ntiles += (size_t)((c.width + 15) / 16) * ((c.height + 15) / 16);
+    b->max_spp = items[i].opts.spp > b->max_spp ? items[i].opts.spp : b->max_spp;
+    b->max_depth = items[i].opts.depth > b->max_depth ? items[i].opts.depth : b->max_depth;
+  }
+  if (b->npix >= (1ull << 32) || b->ntiles * 256 >= (1ull << 32)) {
+    rt_batch_destroy(b);
+    return rt_set_error(ctx, RT_ERR_INVALID, "batch has too many pixels");
+  }
+  size_t bytes = rt_wavefront_bytes_batch(b->npix, b->ntiles, b->nscenes);
+  if (hipMalloc(&b->ws, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    b->ws = nullptr;
+    rt_batch_destroy(b);
+    return rt_set_error(ctx, RT_ERR_OOM, "batch workspace of %zu bytes", bytes);
+  }
+  // Scene table, BVH views and tile -> scene map: the first three regions.
+  const size_t sd = rt_scene_desc_bytes(), vb = rt_bvh_view_bytes();
+  std::vector<char> table(sd * count), views(vb * count);
+  std::vector<uint16_t> tile_scene(b->ntiles);
+  for (int i = 0; i < count; ++i) {
+    const rt_batch_item& it = items[i];
+    rt_fill_scene(table.data() + sd * i, it.bvh, &it.mesh, it.mats_dev, it.nmat, &it.light, &it.cam, &it.opts,
+                  it.rgb_dev, (uint32_t)pix_off[i], (uint32_t)tile_off[i]);
+    BvhView v = it.bvh->view();
+    ::memcpy(views.data() + vb * i, &v, sizeof(v));
+    size_t nt = (size_t)((it.cam.width + 15) / 16) * ((it.cam.height + 15) / 16);
+    for (size_t t = 0; t < nt; ++t) tile_scene[tile_off[i] + t] = (uint16_t)i;
+  }
+  char* w = (char*)b->ws;
+  size_t o0 = 0, o1 = rt_align256(sd * count), o2 = o1 + rt_align256(vb * count);
+  hipError_t e = hipMemcpy(w + o0, table.data(), table.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(w + o1, views.data(), views.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(w + o2, tile_scene.data(), tile_scene.size() * 2, hipMemcpyHostToDevice);
+  // The overflow counter accumulates over the batch's lifetime.
+  if (e == hipSuccess)
+    e = hipMemset(rt_wavefront_counts(b->ws, b->npix, b->ntiles, b->nscenes) + 8, 0, sizeof(uint32_t));
+  for (int k = 0; e == hipSuccess && k < kBuildStreams; ++k) {
+    e = hipStreamCreateWithFlags(&b->streams[k], hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->joins[k], hipEventDisableTiming);
+  }
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e != hipSuccess) {
+    rt_batch_destroy(b);
+    return rt_set_error(ctx, RT_ERR_DEVICE, "batch setup failed: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return RT_OK;
+}
+
+rt_status rt_batch_rebuild(rt_batch* b, void* stream) {
+  if (!b) return RT_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  RT_HIP(b->ctx, hipEventRecord(b->fork, s));
+  int used = (int)(b->nscenes < (size_t)kBuildStreams ? b->nscenes : (size_t)kBuildStreams);
+  for (int k = 0; k < used; ++k) RT_HIP(b->ctx, hipStreamWaitEvent(b->streams[k], b->fork, 0));
+  for (size_t i = 0; i < b->nscenes; ++i) {
+    rt_batch_item& it = b->items[i];
+    RT_HIP(b->ctx, rt_launch_build(const_cast<rt_bvh*>(it.bvh), &it.mesh, b->streams[i % kBuildStreams]));
+  }
+  for (int k = 0; k < used; ++k) {
+    RT_HIP(b->ctx, hipEventRecord(b->joins[k], b->streams[k]));
+    RT_HIP(b->ctx, hipStreamWaitEvent(s, b->joins[k], 0));
+  }
+  return RT_OK;
+}
+
+rt_status rt_batch_render(rt_batch* b, uint64_t* counters_dev, void* stream) {
+  if (!b) return RT_ERR_INVALID;
+  RT_HIP(b->ctx, rt_launch_wavefront(b->ws, b->npix, b->ntiles, b->nscenes, b->max_spp, b->max_depth,
+                                     b->ctx->num_cus, &b->tuning, counters_dev, nullptr, (hipStream_t)stream));
+  return RT_OK;
+}
+
+rt_status rt_batch_set_tuning(rt_batch* b, const rt_tuning* t) {
+  if (!b) return RT_ERR_INVALID;
+  if (!rt_tuning_valid(t)) return rt_set_error(b->ctx, RT_ERR_INVALID, "invalid tuning");
+  b->tuning = *t;
+  return RT_OK;
+}
+
+rt_status rt_batch_get_tuning(const rt_batch* b, rt_tuning* t) {
+  if (!b || !t) return RT_ERR_INVALID;
+  *t = b->tuning;
+  return RT_OK;
+}
+
+rt_status rt_batch_autotune(rt_batch* b, void* stream, rt_tuning* best, float* best_ms) {
+  if (!b) return RT_ERR_INVALID;
+  hipStream_t s = (hipStream_t)stream;
+  const int rings[] = {16, 32};
+  const int chunks[] = {128, 256};
+  const int waves[] = {3, 4, 5, 8};
+  hipEvent_t e0, e1;
+  RT_HIP(b->ctx, hipEventCreate(&e0));
+  RT_HIP(b->ctx, hipEventCreate(&e1));
+  rt_tuning keep = b->tuning, win = b->tuning;
+  float win_ms = -1.f;
+  rt_status st = RT_OK;
+  for (int r : rings)
+    for (int c : chunks)
+      for (int w : waves) {
+        rt_tuning t{r, c, w, 2};
+        b->tuning = t;
+        if ((st = rt_batch_render(b, nullptr, stream)) != RT_OK) goto done;  // warm-up
+        if (hipEventRecord(e0, s) != hipSuccess || (st = rt_batch_render(b, nullptr, stream)) != RT_OK ||
+            hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+          st = st != RT_OK ? st : rt_set_error(b->ctx, RT_ERR_DEVICE, "autotune timing failed");
+          goto done;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (win_ms < 0.f || ms < win_ms) {
+          win_ms = ms;
+          win = t;
+        }
+      }
+done:
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->tuning = st == RT_OK ? win : keep;
+  if (st == RT_OK) {
+    if (best) *best = win;
+    if (best_ms) *best_ms = win_ms;
+  }
+  return st;
+}
+
+rt_status rt_batch_diagnostics(const rt_batch* b, void* stream, uint32_t out[1]) {
+  if (!b || !out) return RT_ERR_INVALID;
+  uint32_t* c = rt_wavefront_counts(b->ws, b->npix, b->ntiles, b->nscenes);
+  RT_HIP(b->ctx, hipMemcpyAsync(out, c + 8, sizeof(uint32_t), hipMemcpyDeviceToHost, (hipStream_t)stream));
+  RT_HIP(b->ctx, hipStreamSynchronize((hipStream_t)stream));
+  return RT_OK;
+}
+
+}  // extern "C"

@@ -101,3 +101,19 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
                             uint64_t* counters, void* workspace, hipStream_t s);
 size_t rt_wavefront_bytes(int32_t width, int32_t height);
+size_t rt_wavefront_bytes_batch(size_t npix, size_t ntiles, size_t nscenes);
+size_t rt_scene_desc_bytes();
+size_t rt_bvh_view_bytes();
+void rt_fill_scene(void* table_entry, const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
+                   const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
+                   uint32_t pix_off, uint32_t tile_off);
+hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
+                               int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
+                               uint32_t* overflow, hipStream_t s);
+uint32_t* rt_wavefront_counts(void* workspace, size_t npix, size_t ntiles, size_t nscenes);
+rt_tuning rt_default_tuning();
+bool rt_tuning_valid(const rt_tuning* t);
+
+// Offsets of the per-batch tables inside a wavefront workspace (first three
+// regions: scene table, BVH views, tile -> scene map).
+constexpr size_t rt_align256(size_t v) { return (v + 255) & ~(size_t)255; }

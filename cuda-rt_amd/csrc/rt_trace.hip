@@ -130,23 +130,25 @@ struct RingStack {
   int sp = 0, lo = 0;
 };
 
+template <int RING = kRing>
 __device__ __forceinline__ bool stack_push(RingStack& S, int* spill, int v) {
-  if (S.sp - S.lo == kRing) {
+  if (S.sp - S.lo == RING) {
     if (S.lo == kSpill) return false;
-    spill[S.lo] = S.col[(S.lo & (kRing - 1)) * kThreads];
+    spill[S.lo] = S.col[(S.lo & (RING - 1)) * kThreads];
     ++S.lo;
   }
-  S.col[(S.sp & (kRing - 1)) * kThreads] = v;
+  S.col[(S.sp & (RING - 1)) * kThreads] = v;
   ++S.sp;
   return true;
 }
 
+template <int RING = kRing>
 __device__ __forceinline__ int stack_pop(RingStack& S, int* spill) {
   --S.sp;
-  int v = S.col[(S.sp & (kRing - 1)) * kThreads];
+  int v = S.col[(S.sp & (RING - 1)) * kThreads];
   if (S.sp == S.lo && S.lo > 0) {
     --S.lo;
-    S.col[(S.lo & (kRing - 1)) * kThreads] = spill[S.lo];
+    S.col[(S.lo & (RING - 1)) * kThreads] = spill[S.lo];
   }
   return v;
 }
@@ -326,12 +328,12 @@ __device__ __forceinline__ void flush_stats(unsigned long long* counters, const 
 // ------------------------------------------------------------ persistent traversal
 // Ray-queue traversal with dynamic ray fetch: each lane walks one ray a node
 // per iteration and, as soon as its ray finishes, takes the next ray index
-// from a chunk its wave reserved with one atomic (kFetchChunk rays), so the
+// from a chunk its wave reserved with one atomic (fetch_chunk rays), so the
 // wave keeps its lanes busy until the queue drains instead of idling behind
 // its longest ray. Closest-hit and any-hit use the same decisions as
 // closest_hit()/any_hit() above, one node (or, for any-hit, one popped leaf)
 // per step.
-constexpr uint32_t kFetchChunk = 256;
+constexpr uint32_t kDefaultFetchChunk = 256;
 constexpr uint32_t kSegments = 8;       // one per XCD
 constexpr uint32_t kFetchStride = 32;   // counters 128 bytes apart
 constexpr uint32_t kFetchWords = kSegments * kFetchStride;
@@ -365,13 +367,18 @@ __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uin
   }
 }
 
-template <bool ANY>
-__global__ __launch_bounds__(kThreads) void k_trace_persistent(BvhView B, RayIn R, double tmin_phys, HitOut O,
-                                                               uint32_t* fetch_ctr, uint32_t* overflow,
+// views[scene[i]] is the BVH of ray i (scene == nullptr: views[0] for all).
+template <bool ANY, int RING>
+__global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __restrict__ views,
+                                                               const uint16_t* __restrict__ scene, RayIn R,
+                                                               double tmin_phys, HitOut O, uint32_t* fetch_ctr,
+                                                               uint32_t fetch_chunk, uint32_t* overflow,
                                                                unsigned long long* counters) {
-  __shared__ int s_stack[kRing * kThreads];
+  __shared__ int s_stack[RING * kThreads];
   const uint32_t nrays = R.count_dev ? *R.count_dev : R.count;
-  const int n = B.n;
+  const uint32_t kFetchChunk = fetch_chunk;
+  BvhView B = views[0];
+  int n = B.n;
   const uint32_t lane = __lane_id();
   const uint64_t mlt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   TravStats st;
@@ -385,16 +392,10 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(BvhView B, RayIn 
   double bt = 0, bal = 0, bbe = 0;
   int bpos = -1;
   uint32_t chunk_next = 0, chunk_end = 0;
-  bool drained = (n <= 0);
+  bool drained = false;
   // ray range split into kSegments contiguous segments of whole chunks
   const uint32_t seg_len = ((nrays + kSegments - 1) / kSegments + kFetchChunk - 1) / kFetchChunk * kFetchChunk;
   uint32_t seg = blockIdx.x % kSegments, tries = 0;
-  if (n <= 0) {
-    // Empty BVH: every ray misses.
-    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < nrays; i += gridDim.x * kThreads)
-      write_hit(B, O, i, -1, 0, 0, 0);
-    return;
-  }
   while (true) {
     // ---- refill idle lanes from the wave's chunk
     uint64_t need = __ballot(!active);
@@ -436,6 +437,10 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(BvhView B, RayIn 
       }
       if (myid != 0xffffffffu) {
         rid = myid;
+        if (scene) {
+          B = views[scene[rid]];
+          n = B.n;
+        }
         const double* po = R.orig + (size_t)rid * R.stride;
         const double* pd = R.dir + (size_t)rid * R.stride;
         r = make_ray(mk(po[0], po[1], po[2]), mk(pd[0], pd[1], pd[2]));
@@ -443,7 +448,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(BvhView B, RayIn 
         bt = 0;
         double t0, t1;
         const float* rb = B.node_box;
-        if (!slab(r, rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], t0, t1)) {
+        if (n <= 0 || !slab(r, rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], t0, t1)) {
           write_hit(B, O, rid, -1, 0, 0, 0);
         } else if (n == 1) {
           double t, al, be;
@@ -498,7 +503,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(BvhView B, RayIn 
       bpos = best.pos;
       if (hl && hr) {
         int nearc = (l0 <= r0) ? cl : cr, farc = (l0 <= r0) ? cr : cl;
-        if (!stack_push(S, spill, farc)) {
+        if (!stack_push<RING>(S, spill, farc)) {
           atomicAdd(overflow, 1u);
           bpos = -1;
           done = true;
@@ -511,7 +516,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(BvhView B, RayIn 
       } else if (S.sp == 0) {
         done = true;
       } else {
-        node = stack_pop(S, spill);
+        node = stack_pop<RING>(S, spill);
       }
     } else {
       // node >= 0: internal node to enter; node < 0: leaf whose box passed.
@@ -524,7 +529,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(BvhView B, RayIn 
         } else if (S.sp == 0) {
           done = true;
         } else {
-          node = stack_pop(S, spill);
+          node = stack_pop<RING>(S, spill);
         }
       } else {
         float4 q0, q1, q2, q3;
@@ -535,7 +540,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(BvhView B, RayIn 
         bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
         int cl = __float_as_int(q3.x), cr = __float_as_int(q3.y);
         if (hl) {
-          if (hr && !stack_push(S, spill, cr)) {
+          if (hr && !stack_push<RING>(S, spill, cr)) {
             atomicAdd(overflow, 1u);
             bpos = -1;
             done = true;
@@ -546,7 +551,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(BvhView B, RayIn 
         } else if (S.sp == 0) {
           done = true;
         } else {
-          node = stack_pop(S, spill);
+          node = stack_pop<RING>(S, spill);
         }
       }
     }
@@ -814,28 +819,46 @@ __global__ __launch_bounds__(kThreads) void k_render(BvhView B, MeshShade M, con
 }
 
 // ------------------------------------------------------------ wavefront
-// The same per-pixel computation split into passes: per sample a camera
-// pass, then per bounce a persistent closest-hit traversal, a shading pass
-// (which draws the light sample and, if the path continues, the bounce
-// direction, and appends the shadow ray and the next ray to queues), a
-// persistent any-hit traversal of the shadow queue and a pass that adds the
-// shaded result. Per-pixel RNG state and partial sums stay in HBM between
-// passes; each pixel's draws and additions happen in the megakernel's order,
-// so both give the same image bit for bit.
+// The same per-pixel computation split into passes over a batch of scenes
+// (one scene for rt_render): per sample a camera pass, then per bounce a
+// persistent closest-hit traversal, a shading pass (which draws the light
+// sample and, if the path continues, the bounce direction, and appends the
+// shadow ray and the next ray to queues), a persistent any-hit traversal of
+// the shadow queue and a pass that adds the shaded result. Ray slots carry a
+// scene id; the scene table gives each scene's BVH, mesh, materials and
+// camera. Per-pixel RNG state and partial sums stay in HBM between passes;
+// each pixel's draws and additions happen in the megakernel's order, so
+// both give the same image bit for bit.
+struct SceneDesc {
+  BvhView B;
+  MeshShade M;
+  const rt_material* mats;
+  RenderParams P;
+  float* rgb;
+  uint32_t pix_off;   // first pixel of the scene in the per-pixel arrays
+  uint32_t tile_off;  // first 16x16 tile of the scene
+};
+
 struct WfState {
-  uint32_t* rng;        // np
-  double* color;        // np*3: sum over finished samples
-  double* acc;          // np*3: current sample's path sum
-  double* thr;          // np*3: current path throughput
+  const SceneDesc* scenes;
+  const BvhView* views;    // views[s] = scenes[s].B (traversal reads only these)
+  const uint16_t* tile_scene;  // scene of each tile
+  uint32_t ntiles;
+  uint32_t* rng;        // pixels
+  double* color;        // pixels*3: sum over finished samples
+  double* acc;          // pixels*3: current sample's path sum
+  double* thr;          // pixels*3: current path throughput
   double* ray[2];       // cap*6: ray origin, direction per queue slot
-  uint32_t* pix[2];     // cap: pixel of each queue slot (~0 = unused slot)
+  uint32_t* pix[2];     // cap: global pixel of each queue slot (~0 = unused slot)
+  uint16_t* rscene[2];  // cap: scene of each queue slot
   int32_t* hpos;        // cap: closest hit leaf position or -1
   double* hit;          // cap*3: t, alpha, beta
   double* shq;          // cap*16: q, L, I1, ds, thr, fo
-  uint32_t* spix;       // cap: pixel of each shadow entry
+  uint32_t* spix;       // cap: global pixel of each shadow entry
+  uint16_t* sscene;     // cap: scene of each shadow entry
   int32_t* spos;        // cap: any-hit leaf position or -1
   double* st;           // cap: any-hit t
-  uint32_t* counts;     // [0],[1] ray queue sizes, [2] shadow entries
+  uint32_t* counts;     // [0],[1] ray queue sizes, [2] shadow entries, [8] stack overflows
   uint32_t* fetch[2];   // kFetchWords each: closest-hit and any-hit fetch counters
 };
 
@@ -867,47 +890,59 @@ __device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool want, u
   return r;
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_init(RenderParams P, WfState W) {
-  int px, py;
-  pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
-  if (blockIdx.x == 0 && threadIdx.x < 8) W.counts[threadIdx.x] = 0u;
-  if (px >= P.W || py >= P.H) return;
-  size_t p = (size_t)py * P.W + px;
-  W.rng[p] = rt::wang_hash(rt::seed_index(px, py, P.W) ^ P.seed);
-  st3(W.color + 3 * p, mk(0, 0, 0));
+// Global tile -> (scene, pixel); false outside the scene's image.
+__device__ __forceinline__ bool tile_pixel(const WfState& W, uint32_t gt, int lane, int& s, int& px, int& py,
+                                           uint32_t& gp) {
+  s = W.tile_scene[gt];
+  const SceneDesc& D = W.scenes[s];
+  pixel_of_tile(D.P, (int)(gt - D.tile_off), lane, px, py);
+  if (px >= D.P.W || py >= D.P.H) return false;
+  gp = D.pix_off + (uint32_t)(py * D.P.W + px);
+  return true;
 }
 
-// Start a sample: slot = tile*256 + lane holds the pixel's primary ray.
-// Slots outside the image get a NaN ray, which misses the root box.
-__global__ __launch_bounds__(kThreads) void k_wf_camera(RenderParams P, WfState W, uint32_t nslots) {
-  int px, py;
-  pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
+__global__ __launch_bounds__(kThreads) void k_wf_init(WfState W) {
+  if (blockIdx.x == 0 && threadIdx.x < 16) W.counts[threadIdx.x] = 0u;
+  int s, px, py;
+  uint32_t gp;
+  if (!tile_pixel(W, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
+  const RenderParams& P = W.scenes[s].P;
+  W.rng[gp] = rt::wang_hash(rt::seed_index(px, py, P.W) ^ P.seed);
+  st3(W.color + 3 * (size_t)gp, mk(0, 0, 0));
+}
+
+// Start sample k: slot = tile*256 + lane holds the pixel's primary ray.
+// Slots outside an image, or of a scene that has taken all its samples,
+// get a NaN ray, which misses the root box.
+__global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, int k, uint32_t nslots) {
   const size_t slot = (size_t)blockIdx.x * kThreads + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) W.counts[0] = nslots;
+  int s, px, py;
+  uint32_t gp;
+  bool ok = tile_pixel(W, blockIdx.x, threadIdx.x, s, px, py, gp);
+  const RenderParams& P = W.scenes[s].P;
   double* rr = W.ray[0] + 6 * slot;
-  if (px >= P.W || py >= P.H) {
+  W.rscene[0][slot] = (uint16_t)s;
+  if (!ok || k >= P.spp) {
     const double nan = __builtin_nan("");
-    for (int k = 0; k < 6; ++k) rr[k] = nan;
+    for (int j = 0; j < 6; ++j) rr[j] = nan;
     W.pix[0][slot] = 0xffffffffu;
     return;
   }
-  size_t p = (size_t)py * P.W + px;
-  uint32_t rng = W.rng[p];
+  uint32_t rng = W.rng[gp];
   double xx, yy;
   pixel_xy(P, px, py, xx, yy);
   d3 o, d;
   lens_ray(P, xx, yy, rng, o, d);
-  W.rng[p] = rng;
+  W.rng[gp] = rng;
   st3(rr, o);
   st3(rr + 3, d);
-  W.pix[0][slot] = (uint32_t)p;
-  st3(W.thr + 3 * p, mk(1, 1, 1));
-  st3(W.acc + 3 * p, mk(0, 0, 0));
+  W.pix[0][slot] = gp;
+  st3(W.thr + 3 * (size_t)gp, mk(1, 1, 1));
+  st3(W.acc + 3 * (size_t)gp, mk(0, 0, 0));
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_shade(BvhView B, MeshShade M, const rt_material* __restrict__ mats,
-                                                       RenderParams P, WfState W, int bounce, int cur,
-                                                       unsigned long long* counters) {
+__global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, int bounce, int cur, unsigned long long* counters) {
   __shared__ uint32_t s_tmp[8];
   const uint32_t n_in = W.counts[cur];
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
@@ -916,7 +951,10 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(BvhView B, MeshShade M, c
   bool want_shadow = false, want_next = false;
   ShadePrep S;
   d3 thr = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
+  uint16_t sc = 0;
   if (live) {
+    sc = W.rscene[cur][i];
+    const SceneDesc& D = W.scenes[sc];
     thr = ld3(W.thr + 3 * (size_t)p);
     const int pos = W.hpos[i];
     if (pos < 0) {
@@ -927,11 +965,11 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(BvhView B, MeshShade M, c
       d3 o = ld3(rr), d = ld3(rr + 3);
       const double* hh = W.hit + 3 * (size_t)i;
       uint32_t rng = W.rng[p];
-      const int tri = leaf_tri_of(B, pos);
-      const rt_material& m = mat_of(mats, P.nmat, M, tri);
-      S = shade_prep(P, m, shading_normal(M, tri, hh[1], hh[2]), o, d, hh[0], rng);
+      const int tri = leaf_tri_of(D.B, pos);
+      const rt_material& m = mat_of(D.mats, D.P.nmat, D.M, tri);
+      S = shade_prep(D.P, m, shading_normal(D.M, tri, hh[1], hh[2]), o, d, hh[0], rng);
       want_shadow = true;
-      if (bounce < P.depth) {
+      if (bounce < D.P.depth) {
         double u1 = rt::rand_unit(rng);
         double u2 = rt::rand_unit(rng);
         d3 N = S.N;
@@ -947,6 +985,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(BvhView B, MeshShade M, c
   uint32_t ss = block_append(&W.counts[2], want_shadow, s_tmp);
   if (want_shadow) {
     W.spix[ss] = p;
+    W.sscene[ss] = sc;
     double* e = W.shq + 16 * (size_t)ss;
     st3(e + 0, S.q);
     st3(e + 3, S.L);
@@ -958,6 +997,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(BvhView B, MeshShade M, c
   uint32_t ns = block_append(&W.counts[cur ^ 1], want_next, s_tmp);
   if (want_next) {
     W.pix[cur ^ 1][ns] = p;
+    W.rscene[cur ^ 1][ns] = sc;
     double* rr = W.ray[cur ^ 1] + 6 * (size_t)ns;
     st3(rr, no);
     st3(rr + 3, nd);
@@ -969,9 +1009,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(BvhView B, MeshShade M, c
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(BvhView B, MeshShade M,
-                                                               const rt_material* __restrict__ mats, RenderParams P,
-                                                               WfState W, unsigned long long* counters) {
+__global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, unsigned long long* counters) {
   const uint32_t n_in = W.counts[2];
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   const bool live = i < n_in;
@@ -980,18 +1018,19 @@ __global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(BvhView B, MeshSh
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&counters[1], (unsigned long long)__popcll(m));
   }
   if (!live) return;
+  const SceneDesc& D = W.scenes[W.sscene[i]];
   const double* e = W.shq + 16 * (size_t)i;
   d3 q = ld3(e + 0);
   HitRec s;
   s.pos = W.spos[i];
   s.t = W.st[i];
-  d3 I = shade_finish(P, ld3(e + 6), e[15], ld3(e + 9), shadow_factor(P, B, mats, M, q, s.pos >= 0, s));
+  d3 I = shade_finish(D.P, ld3(e + 6), e[15], ld3(e + 9), shadow_factor(D.P, D.B, D.mats, D.M, q, s.pos >= 0, s));
   uint32_t p = W.spix[i];
   double* a = W.acc + 3 * (size_t)p;
   st3(a, ld3(a) + rt::hadamard(ld3(e + 12), I));
 }
 
-// Reset the listed counters and, if fetch != nullptr, kWords fetch
+// Reset the listed counters and, if fetch != nullptr, `words` fetch
 // counters (graph-safe replacement for a memset).
 __global__ void k_wf_reset(uint32_t* counts, uint32_t mask, uint32_t* fetch, uint32_t words) {
   if (counts && threadIdx.x < 32 && (mask >> threadIdx.x) & 1u) counts[threadIdx.x] = 0u;
@@ -999,26 +1038,44 @@ __global__ void k_wf_reset(uint32_t* counts, uint32_t mask, uint32_t* fetch, uin
     for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) fetch[k] = 0u;
 }
 
-// End of a sample: color += path sum.
-__global__ __launch_bounds__(kThreads) void k_wf_flush(RenderParams P, WfState W) {
-  int px, py;
-  pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
-  if (px >= P.W || py >= P.H) return;
-  size_t p = (size_t)py * P.W + px;
-  st3(W.color + 3 * p, ld3(W.color + 3 * p) + ld3(W.acc + 3 * p));
+// End of sample k: color += path sum for scenes still sampling.
+__global__ __launch_bounds__(kThreads) void k_wf_flush(WfState W, int k) {
+  int s, px, py;
+  uint32_t gp;
+  if (!tile_pixel(W, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
+  if (k >= W.scenes[s].P.spp) return;
+  st3(W.color + 3 * (size_t)gp, ld3(W.color + 3 * (size_t)gp) + ld3(W.acc + 3 * (size_t)gp));
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_resolve(RenderParams P, WfState W, float* __restrict__ rgb) {
-  int px, py;
-  pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
-  if (px >= P.W || py >= P.H) return;
-  size_t p = (size_t)py * P.W + px;
-  write_pixel(P, rgb, px, py, ld3(W.color + 3 * p));
+__global__ __launch_bounds__(kThreads) void k_wf_resolve(WfState W) {
+  int s, px, py;
+  uint32_t gp;
+  if (!tile_pixel(W, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
+  const SceneDesc& D = W.scenes[s];
+  write_pixel(D.P, D.rgb, px, py, ld3(W.color + 3 * (size_t)gp));
+}
+
+// Copy a scene table that came in as a kernel argument to device memory
+// (keeps single-scene renders free of host copies, so they can be captured).
+__global__ void k_wf_put_scene(SceneDesc* dst, BvhView* views, uint16_t* tile_scene, SceneDesc d, uint32_t ntiles) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    dst[0] = d;
+    views[0] = d.B;
+  }
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < ntiles; t += gridDim.x * blockDim.x) tile_scene[t] = 0;
+}
+
+__global__ void k_put_view(BvhView* dst, BvhView v) {
+  if (threadIdx.x == 0) *dst = v;
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
-RenderParams make_params(const rt_camera* cam, const rt_light* light, const rt_render_opts* opts, int32_t nmat) {
+size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+}  // namespace
+
+static RenderParams rt_make_params(const rt_camera* cam, const rt_light* light, const rt_render_opts* opts, int32_t nmat) {
   RenderParams P;
   P.W = cam->width;
   P.H = cam->height;
@@ -1046,45 +1103,183 @@ RenderParams make_params(const rt_camera* cam, const rt_light* light, const rt_r
   return P;
 }
 
-size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+rt_tuning rt_default_tuning() {
+  rt_tuning t;
+  t.stack_ring = kRing;
+  t.fetch_chunk = kDefaultFetchChunk;
+  t.waves_per_simd = 5;
+  t.node_layout = 2;
+  return t;
+}
 
-int persistent_blocks(const rt_bvh* b, uint32_t max_rays) {
-  int cap = (b->ctx ? b->ctx->num_cus : 256) * 5;
+bool rt_tuning_valid(const rt_tuning* t) {
+  return t && (t->stack_ring == 16 || t->stack_ring == 32) &&
+         (t->fetch_chunk == 64 || t->fetch_chunk == 128 || t->fetch_chunk == 256 || t->fetch_chunk == 512) &&
+         t->waves_per_simd >= 1 && t->waves_per_simd <= 8 && t->node_layout == 2;
+}
+
+static int persistent_grid(int num_cus, const rt_tuning& t, uint32_t max_rays) {
+  int cap = num_cus * t.waves_per_simd;  // 4 waves per workgroup, 4 SIMDs per CU
   int need = (int)((max_rays + kThreads - 1) / kThreads);
   return need < cap ? (need > 0 ? need : 1) : cap;
 }
 
-}  // namespace
+template <bool ANY>
+static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const BvhView* views,
+                              const uint16_t* scene, RayIn R, double tmin, HitOut O, uint32_t* fetch,
+                              uint32_t* overflow, unsigned long long* counters) {
+  if (t.stack_ring == 16)
+    k_trace_persistent<ANY, 16><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, t.fetch_chunk, overflow,
+                                                          counters);
+  else
+    k_trace_persistent<ANY, 32><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, t.fetch_chunk, overflow,
+                                                          counters);
+}
 
 hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays* rays, int32_t mode, double tmin,
                            const rt_hits* hits, hipStream_t s) {
   (void)m;
   if (rays->n <= 0) return hipSuccess;
   uint32_t* fetch = b->fetch;
+  // fetch[kFetchWords] holds the BvhView the kernel reads
+  BvhView* view = (BvhView*)(fetch + 512);
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, fetch, kFetchWords);
+  k_put_view<<<1, 1, 0, s>>>(view, b->view());
   RayIn R{rays->orig_dev, rays->dir_dev, 3, nullptr, (uint32_t)rays->n};
   HitOut O{nullptr, hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev, nullptr};
-  int grid = persistent_blocks(b, (uint32_t)rays->n);
+  rt_tuning t = rt_default_tuning();
+  int grid = persistent_grid(b->ctx ? b->ctx->num_cus : 256, t, (uint32_t)rays->n);
   if (mode == RT_TRACE_ANY)
-    k_trace_persistent<true><<<grid, kThreads, 0, s>>>(b->view(), R, 0.0, O, fetch, b->misc + kMiscOverflow, nullptr);
+    launch_persistent<true>(t, grid, s, view, nullptr, R, 0.0, O, fetch, b->misc + kMiscOverflow, nullptr);
   else
-    k_trace_persistent<false><<<grid, kThreads, 0, s>>>(b->view(), R, tmin, O, fetch, b->misc + kMiscOverflow,
-                                                       nullptr);
+    launch_persistent<false>(t, grid, s, view, nullptr, R, tmin, O, fetch, b->misc + kMiscOverflow, nullptr);
   return hipGetLastError();
 }
 
+// Workspace layout of a wavefront render over `nscenes` scenes with `npix`
+// pixels and `ntiles` tiles in total.
+struct WfLayout {
+  size_t off[24];
+  size_t total;
+};
+
+static WfLayout wf_layout(size_t npix, size_t ntiles, size_t nscenes) {
+  WfLayout L;
+  size_t cap = ntiles * kThreads;
+  size_t sizes[] = {nscenes * sizeof(SceneDesc), nscenes * sizeof(BvhView), ntiles * 2, npix * 4, npix * 24,
+                    npix * 24, npix * 24, cap * 48, cap * 48, cap * 4, cap * 4, cap * 2, cap * 2, cap * 4,
+                    cap * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, 256, kFetchWords * 4, kFetchWords * 4};
+  size_t off = 0;
+  for (int k = 0; k < (int)(sizeof(sizes) / sizeof(sizes[0])); ++k) {
+    L.off[k] = off;
+    off += align256(sizes[k]);
+  }
+  L.total = off;
+  return L;
+}
+
+static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes) {
+  WfLayout L = wf_layout(npix, ntiles, nscenes);
+  char* w = (char*)ws;
+  WfState W;
+  int k = 0;
+  W.scenes = (const SceneDesc*)(w + L.off[k++]);
+  W.views = (const BvhView*)(w + L.off[k++]);
+  W.tile_scene = (const uint16_t*)(w + L.off[k++]);
+  W.ntiles = (uint32_t)ntiles;
+  W.rng = (uint32_t*)(w + L.off[k++]);
+  W.color = (double*)(w + L.off[k++]);
+  W.acc = (double*)(w + L.off[k++]);
+  W.thr = (double*)(w + L.off[k++]);
+  W.ray[0] = (double*)(w + L.off[k++]);
+  W.ray[1] = (double*)(w + L.off[k++]);
+  W.pix[0] = (uint32_t*)(w + L.off[k++]);
+  W.pix[1] = (uint32_t*)(w + L.off[k++]);
+  W.rscene[0] = (uint16_t*)(w + L.off[k++]);
+  W.rscene[1] = (uint16_t*)(w + L.off[k++]);
+  W.hpos = (int32_t*)(w + L.off[k++]);
+  W.hit = (double*)(w + L.off[k++]);
+  W.shq = (double*)(w + L.off[k++]);
+  W.spix = (uint32_t*)(w + L.off[k++]);
+  W.sscene = (uint16_t*)(w + L.off[k++]);
+  W.spos = (int32_t*)(w + L.off[k++]);
+  W.st = (double*)(w + L.off[k++]);
+  W.counts = (uint32_t*)(w + L.off[k++]);
+  W.fetch[0] = (uint32_t*)(w + L.off[k++]);
+  W.fetch[1] = (uint32_t*)(w + L.off[k++]);
+  return W;
+}
+
+uint32_t* rt_wavefront_counts(void* workspace, size_t npix, size_t ntiles, size_t nscenes) {
+  return wf_state(workspace, npix, ntiles, nscenes).counts;
+}
+
+size_t rt_wavefront_bytes_batch(size_t npix, size_t ntiles, size_t nscenes) {
+  return wf_layout(npix, ntiles, nscenes).total;
+}
+
 size_t rt_wavefront_bytes(int32_t width, int32_t height) {
-  size_t np = (size_t)width * height;
-  size_t cap = (size_t)((width + 15) / 16) * ((height + 15) / 16) * kThreads;
-  return align256(np * 4) + 3 * align256(np * 24) + 2 * align256(cap * 48) + 2 * align256(cap * 4) +
-         align256(cap * 4) + align256(cap * 24) + align256(cap * 128) + 3 * align256(cap * 8) + 256 +
-         2 * align256(kFetchWords * 4);
+  size_t ntiles = (size_t)cdiv(width, 16) * cdiv(height, 16);
+  return rt_wavefront_bytes_batch((size_t)width * height, ntiles, 1);
+}
+
+// Fill the scene table of a batch (host side; the caller copies it to the device).
+void rt_fill_scene(void* table_entry, const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
+                   const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
+                   uint32_t pix_off, uint32_t tile_off) {
+  SceneDesc d;
+  d.B = b->view();
+  d.M = MeshShade{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev};
+  d.mats = mats;
+  d.P = rt_make_params(cam, light, opts, nmat);
+  d.rgb = rgb;
+  d.pix_off = pix_off;
+  d.tile_off = tile_off;
+  *(SceneDesc*)table_entry = d;
+}
+size_t rt_scene_desc_bytes() { return sizeof(SceneDesc); }
+size_t rt_bvh_view_bytes() { return sizeof(BvhView); }
+
+// The wavefront passes over a batch whose scene table, views and
+// tile->scene map are already in the workspace.
+hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
+                               int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
+                               uint32_t* overflow, hipStream_t s) {
+  WfState W = wf_state(workspace, npix, ntiles, nscenes);
+  rt_tuning t = tuning ? *tuning : rt_default_tuning();
+  unsigned long long* cnt = (unsigned long long*)counters;
+  const size_t cap = ntiles * kThreads;
+  const int tiles = (int)ntiles;
+  const int grid = persistent_grid(num_cus, t, (uint32_t)cap);
+  uint32_t* ovf = overflow ? overflow : W.counts + 8;
+  k_wf_init<<<tiles, kThreads, 0, s>>>(W);
+  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W.fetch[0], 2 * kFetchWords);
+  for (int it = 0; it < max_spp; ++it) {
+    k_wf_camera<<<tiles, kThreads, 0, s>>>(W, it, (uint32_t)cap);
+    int cur = 0;
+    for (int bounce = 0; bounce <= max_depth; ++bounce) {
+      RayIn R{W.ray[cur], W.ray[cur] + 3, 6, &W.counts[cur], 0};
+      HitOut O{W.hpos, nullptr, nullptr, nullptr, nullptr, W.hit};
+      launch_persistent<false>(t, grid, s, W.views, W.rscene[cur], R, bounce == 0 ? 0.0 : kBounceTmin, O,
+                               W.fetch[0], ovf, cnt);
+      k_wf_shade<<<tiles, kThreads, 0, s>>>(W, bounce, cur, cnt);
+      RayIn SR{W.shq, W.shq + 3, 16, &W.counts[2], 0};
+      HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr};
+      launch_persistent<true>(t, grid, s, W.views, W.sscene, SR, 0.0, SO, W.fetch[1], ovf, cnt);
+      k_wf_shadow_finish<<<tiles, kThreads, 0, s>>>(W, cnt);
+      k_wf_reset<<<1, 256, 0, s>>>(W.counts, (1u << 2) | (1u << cur), W.fetch[0], 2 * kFetchWords);
+      cur ^= 1;
+    }
+    k_wf_flush<<<tiles, kThreads, 0, s>>>(W, it);
+  }
+  k_wf_resolve<<<tiles, kThreads, 0, s>>>(W);
+  return hipGetLastError();
 }
 
 hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
                             uint64_t* counters, void* workspace, hipStream_t s) {
-  RenderParams P = make_params(cam, light, opts, nmat);
+  RenderParams P = rt_make_params(cam, light, opts, nmat);
   MeshShade M{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev};
   const int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
   unsigned long long* cnt = (unsigned long long*)counters;
@@ -1093,56 +1288,17 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
     k_render<<<tiles, kThreads, 0, s>>>(b->view(), M, mats, P, rgb, cnt, ovf);
     return hipGetLastError();
   }
-  const size_t np = (size_t)P.W * P.H;
-  const size_t cap = (size_t)tiles * kThreads;
-  char* w = (char*)workspace;
-  size_t off = 0;
-  auto take = [&](size_t bytes) {
-    char* r = w + off;
-    off += align256(bytes);
-    return r;
-  };
-  WfState W;
-  W.rng = (uint32_t*)take(np * 4);
-  W.color = (double*)take(np * 24);
-  W.acc = (double*)take(np * 24);
-  W.thr = (double*)take(np * 24);
-  W.ray[0] = (double*)take(cap * 48);
-  W.ray[1] = (double*)take(cap * 48);
-  W.pix[0] = (uint32_t*)take(cap * 4);
-  W.pix[1] = (uint32_t*)take(cap * 4);
-  W.hpos = (int32_t*)take(cap * 4);
-  W.hit = (double*)take(cap * 24);
-  W.shq = (double*)take(cap * 128);
-  W.spix = (uint32_t*)take(cap * 8);
-  W.spos = (int32_t*)take(cap * 8);
-  W.st = (double*)take(cap * 8);
-  W.counts = (uint32_t*)take(256);
-  W.fetch[0] = (uint32_t*)take(kFetchWords * 4);
-  W.fetch[1] = (uint32_t*)take(kFetchWords * 4);
-  const int qblocks = tiles;
-  const int grid = persistent_blocks(b, (uint32_t)cap);
-  const BvhView V = b->view();
-  k_wf_init<<<tiles, kThreads, 0, s>>>(P, W);
-  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W.fetch[0], 2 * kFetchWords);
-  for (int it = 0; it < P.spp; ++it) {
-    k_wf_camera<<<tiles, kThreads, 0, s>>>(P, W, (uint32_t)cap);
-    int cur = 0;
-    for (int bounce = 0; bounce <= P.depth; ++bounce) {
-      RayIn R{W.ray[cur], W.ray[cur] + 3, 6, &W.counts[cur], 0};
-      HitOut O{W.hpos, nullptr, nullptr, nullptr, nullptr, W.hit};
-      k_trace_persistent<false><<<grid, kThreads, 0, s>>>(V, R, bounce == 0 ? 0.0 : kBounceTmin, O, W.fetch[0],
-                                                          ovf, cnt);
-      k_wf_shade<<<qblocks, kThreads, 0, s>>>(V, M, mats, P, W, bounce, cur, cnt);
-      RayIn SR{W.shq, W.shq + 3, 16, &W.counts[2], 0};
-      HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr};
-      k_trace_persistent<true><<<grid, kThreads, 0, s>>>(V, SR, 0.0, SO, W.fetch[1], ovf, cnt);
-      k_wf_shadow_finish<<<qblocks, kThreads, 0, s>>>(V, M, mats, P, W, cnt);
-      k_wf_reset<<<1, 256, 0, s>>>(W.counts, (1u << 2) | (1u << cur), W.fetch[0], 2 * kFetchWords);
-      cur ^= 1;
-    }
-    k_wf_flush<<<tiles, kThreads, 0, s>>>(P, W);
-  }
-  k_wf_resolve<<<tiles, kThreads, 0, s>>>(P, W, rgb);
-  return hipGetLastError();
+  SceneDesc d;
+  d.B = b->view();
+  d.M = M;
+  d.mats = mats;
+  d.P = P;
+  d.rgb = rgb;
+  d.pix_off = 0;
+  d.tile_off = 0;
+  WfState W = wf_state(workspace, (size_t)P.W * P.H, (size_t)tiles, 1);
+  k_wf_put_scene<<<cdiv(tiles, 256), 256, 0, s>>>((SceneDesc*)W.scenes, (BvhView*)W.views,
+                                                  (uint16_t*)W.tile_scene, d, (uint32_t)tiles);
+  return rt_launch_wavefront(workspace, (size_t)P.W * P.H, (size_t)tiles, 1, P.spp, P.depth,
+                             b->ctx ? b->ctx->num_cus : 256, nullptr, counters, ovf, s);
 }

@@ -108,6 +108,7 @@ def lib():
         [c_f] * 6 + [c_i32]
     L.rt_gen_jitter.argtypes = [ctypes.c_int32, c_f, c_f, c_f, ctypes.c_double, ctypes.c_uint64, c_f, c_f, c_f]
     _lib = L
+    _bind_batch(L)
     return L
 
 
@@ -388,3 +389,98 @@ def render(ctx, bvh, mesh, mats_dev, nmat, light_, cam, opts, rgb=None, counters
                               ctypes.byref(cam), ctypes.byref(opts), rgb.data_ptr(),
                               counters.data_ptr() if counters is not None else None, _stream_handle(stream)))
     return rgb
+
+
+# ------------------------------------------------------------------ batches
+class Tuning(ctypes.Structure):
+    _fields_ = [("stack_ring", ctypes.c_int32), ("fetch_chunk", ctypes.c_int32),
+                ("waves_per_simd", ctypes.c_int32), ("node_layout", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BatchItem(ctypes.Structure):
+    _fields_ = [("bvh", vp), ("mesh", MeshView), ("mats", vp), ("nmat", ctypes.c_int32), ("light", Light),
+                ("cam", Camera), ("opts", RenderOpts), ("rgb", vp)]
+
+
+def _bind_batch(L):
+    if getattr(L, "_batch_bound", False):
+        return
+    L.rt_batch_create.argtypes = [vp, ctypes.POINTER(BatchItem), ctypes.c_int32, ctypes.POINTER(vp)]
+    L.rt_batch_rebuild.argtypes = [vp, vp]
+    L.rt_batch_render.argtypes = [vp, vp, vp]
+    L.rt_batch_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.rt_batch_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
+    L.rt_batch_autotune.argtypes = [vp, vp, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]
+    L.rt_batch_diagnostics.argtypes = [vp, vp, c_u32]
+    L.rt_batch_destroy.argtypes = [vp]
+    L._batch_bound = True
+
+
+EXPORTED += ["rt_batch_create", "rt_batch_rebuild", "rt_batch_render", "rt_batch_set_tuning", "rt_batch_get_tuning",
+             "rt_batch_autotune", "rt_batch_diagnostics", "rt_batch_destroy"]
+
+
+class Batch:
+    """Many scenes rendered by one set of wavefront launches (rt_batch_*).
+
+    scenes: list of dicts with keys mesh (DeviceMesh), bvh (BVH), mats (device
+    tensor), nmat, light, cam, opts, rgb (device float tensor)."""
+
+    def __init__(self, ctx, scenes):
+        self.ctx, self.L = ctx, ctx.L
+        _bind_batch(self.L)
+        self.scenes = scenes
+        arr = (BatchItem * len(scenes))()
+        for k, sc in enumerate(scenes):
+            it = arr[k]
+            it.bvh = sc["bvh"].h
+            it.mesh = sc["mesh"].view()
+            it.mats = sc["mats"].data_ptr()
+            it.nmat = sc["nmat"]
+            it.light = sc["light"]
+            it.cam = sc["cam"]
+            it.opts = sc["opts"]
+            it.rgb = sc["rgb"].data_ptr()
+        self._items = arr
+        self.h = vp()
+        ctx.check(self.L.rt_batch_create(ctx.h, arr, len(scenes), ctypes.byref(self.h)))
+
+    def rebuild(self, stream=None):
+        self.ctx.check(self.L.rt_batch_rebuild(self.h, _stream_handle(stream)))
+
+    def render(self, counters=None, stream=None):
+        self.ctx.check(self.L.rt_batch_render(self.h, counters.data_ptr() if counters is not None else None,
+                                              _stream_handle(stream)))
+
+    def set_tuning(self, t):
+        self.ctx.check(self.L.rt_batch_set_tuning(self.h, ctypes.byref(t)))
+
+    def tuning(self):
+        t = Tuning()
+        self.ctx.check(self.L.rt_batch_get_tuning(self.h, ctypes.byref(t)))
+        return t
+
+    def autotune(self, stream=None):
+        t = Tuning()
+        ms = ctypes.c_float()
+        self.ctx.check(self.L.rt_batch_autotune(self.h, _stream_handle(stream), ctypes.byref(t), ctypes.byref(ms)))
+        return t, ms.value
+
+    def diagnostics(self, stream=None):
+        out = (ctypes.c_uint32 * 1)()
+        self.ctx.check(self.L.rt_batch_diagnostics(self.h, _stream_handle(stream), out))
+        return int(out[0])
+
+    def close(self):
+        if self.h:
+            self.L.rt_batch_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

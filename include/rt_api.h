@@ -173,6 +173,49 @@ rt_status rt_render(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, co
                     int32_t nmat, const rt_light *light, const rt_camera *cam, const rt_render_opts *opts,
                     float *rgb_dev, uint64_t *counters_dev, void *stream);
 
+/* ---- batched rendering (no reference counterpart: the reference renders
+ * one scene per process, src/main.cpp:316-330). Many scenes, each with its
+ * own BVH, camera, light, materials, spp and depth, go through one set of
+ * wavefront launches whose persistent traversal kernels take rays of every
+ * scene from one queue. Each scene's image is identical to rt_render's. */
+typedef struct {
+  int32_t stack_ring;      /* LDS traversal stack entries per lane: 16 or 32 */
+  int32_t fetch_chunk;     /* rays a wave reserves per atomic: 64, 128, 256 or 512 */
+  int32_t waves_per_simd;  /* persistent traversal workgroups per CU, 1..8 */
+  int32_t node_layout;     /* 2: binary nodes, 64 B per internal node */
+} rt_tuning;
+
+typedef struct {
+  const rt_bvh *bvh;
+  rt_mesh_view mesh;       /* device pointers; normals required */
+  const rt_material *mats_dev;
+  int32_t nmat;
+  rt_light light;
+  rt_camera cam;
+  rt_render_opts opts;     /* RT_RENDER_WAVEFRONT is implied */
+  float *rgb_dev;          /* width*height*3 floats, as rt_render */
+} rt_batch_item;
+
+typedef struct rt_batch rt_batch;
+
+/* Allocate the batch workspace and upload the scene table (synchronous).
+ * The items' device buffers must stay valid while the batch is used. */
+rt_status rt_batch_create(rt_ctx *ctx, const rt_batch_item *items, int32_t count, rt_batch **out);
+/* Rebuild every scene's BVH; builds run on parallel internal streams that
+ * fork from and join back into `stream` (graph-capturable). */
+rt_status rt_batch_rebuild(rt_batch *batch, void *stream);
+/* Render every scene. Allocates nothing and never synchronises, so it can
+ * be captured in a hipGraph. counters_dev as in rt_render (summed). */
+rt_status rt_batch_render(rt_batch *batch, uint64_t *counters_dev, void *stream);
+rt_status rt_batch_set_tuning(rt_batch *batch, const rt_tuning *tuning);
+rt_status rt_batch_get_tuning(const rt_batch *batch, rt_tuning *tuning);
+/* Time rt_batch_render under every candidate tuning (synchronises), keep
+ * the fastest and return it and its time. Outputs do not depend on tuning. */
+rt_status rt_batch_autotune(rt_batch *batch, void *stream, rt_tuning *best, float *best_ms);
+/* out[0] = traversal stack overflows during batch renders (0 when healthy). */
+rt_status rt_batch_diagnostics(const rt_batch *batch, void *stream, uint32_t out[1]);
+void rt_batch_destroy(rt_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,3 +53,21 @@ def duplicate_key_mesh(n, seed=77):
         a[: 3 * k] = c + (a[: 3 * k] - a[: 3 * k].mean()) * np.float32(1e-3)
         a[3: 3 * 1000] = np.tile(a[0:3], 999)  # 1000 identical triangles: equal keys at any width
     return hm
+
+
+def batch_scenes(count, seed, tri_range=(1_000, 2_000_000), resolutions=(256, 512, 1024), spp_range=(1, 8),
+                 depth=0):
+    """The batched workload: `count` random soups with log-uniform triangle
+    counts, each with its own square camera and sample count."""
+    rng = np.random.default_rng(seed)
+    out = []
+    lo, hi = np.log(tri_range[0]), np.log(tri_range[1])
+    for k in range(count):
+        n = int(np.exp(rng.uniform(lo, hi)))
+        res = int(rng.choice(resolutions))
+        spp = int(rng.integers(spp_range[0], spp_range[1] + 1))
+        hm = rtamd.gen_soup(n, 0x5EED0004 + 7919 * k)
+        mats, light, cam, opts = c2_setup(res, res, spp)
+        opts.depth = depth
+        out.append(dict(hm=hm, mats=mats, light=light, cam=cam, opts=opts))
+    return out

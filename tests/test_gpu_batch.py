@@ -1,0 +1,121 @@
+"""Batched rendering (rt_batch_*): every scene's image equals the single-scene
+render bit for bit, for every tuning, eagerly and replayed from a hipGraph."""
+import numpy as np
+import pytest
+
+import rtamd
+from oracle import oracle as O
+from tests import scenes
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    return rtamd.Context(0)
+
+
+def _make(ctx, specs, bits=63):
+    out = []
+    for sp in specs:
+        dm = rtamd.DeviceMesh(sp["hm"])
+        b = rtamd.BVH(ctx, dm, bits)
+        md = rtamd.materials_tensor(sp["mats"])
+        rgb = torch.zeros(sp["cam"].width * sp["cam"].height * 3, dtype=torch.float32, device="cuda")
+        out.append(dict(mesh=dm, bvh=b, mats=md, nmat=len(sp["mats"]), light=sp["light"], cam=sp["cam"],
+                        opts=sp["opts"], rgb=rgb, hm=sp["hm"]))
+    return out
+
+
+def _single(ctx, sc):
+    o = rtamd.render_opts(sc["opts"].spp, sc["opts"].depth, sc["opts"].seed,
+                          sc["opts"].flags | rtamd.RT_RENDER_WAVEFRONT)
+    return rtamd.render(ctx, sc["bvh"], sc["mesh"], sc["mats"], sc["nmat"], sc["light"], sc["cam"], o)
+
+
+def test_batch_matches_single_and_oracle(ctx):
+    print("batch vs single", flush=True)
+    specs = scenes.batch_scenes(6, 11, tri_range=(500, 30_000), resolutions=(32, 48, 64), spp_range=(1, 3))
+    specs[2]["opts"].depth = 2
+    specs[4]["cam"].width = 40  # non-square, not a multiple of 16
+    specs[4]["cam"].height = 24
+    sc = _make(ctx, specs)
+    batch = rtamd.Batch(ctx, sc)
+    cnt = torch.zeros(5, dtype=torch.int64, device="cuda")
+    batch.render(counters=cnt)
+    torch.cuda.synchronize()
+    tot = np.zeros(3, np.int64)
+    for k, s in enumerate(sc):
+        single = _single(ctx, s)
+        torch.cuda.synchronize()
+        got = s["rgb"].cpu().numpy()
+        assert np.array_equal(got, single.cpu().numpy()), f"scene {k} differs from single render"
+        ref, counts = O.render(s["hm"], O.build(s["hm"], 63), specs[k]["mats"], s["light"], s["cam"], s["opts"])
+        assert np.abs(got - ref).max() <= 1e-4, f"scene {k} differs from the oracle"
+        tot += counts
+    c = cnt.cpu().numpy()
+    assert list(c[:3]) == list(tot)
+    assert batch.diagnostics() == 0
+
+
+def test_batch_outputs_do_not_depend_on_tuning(ctx):
+    print("batch tuning invariance", flush=True)
+    specs = scenes.batch_scenes(5, 12, tri_range=(1000, 60_000), resolutions=(64, 96), spp_range=(1, 2))
+    sc = _make(ctx, specs)
+    batch = rtamd.Batch(ctx, sc)
+    ref = None
+    for ring in (16, 32):
+        for chunk in (64, 256):
+            for waves in (1, 5, 8):
+                t = rtamd.Tuning(ring, chunk, waves, 2)
+                batch.set_tuning(t)
+                for s in sc:
+                    s["rgb"].zero_()
+                batch.render()
+                torch.cuda.synchronize()
+                imgs = [s["rgb"].cpu().numpy() for s in sc]
+                if ref is None:
+                    ref = imgs
+                else:
+                    for a, b in zip(ref, imgs):
+                        assert np.array_equal(a, b), t.as_dict()
+    best, ms = batch.autotune()
+    assert ms > 0 and best.stack_ring in (16, 32)
+    batch.render()
+    torch.cuda.synchronize()
+    for a, s in zip(ref, sc):
+        assert np.array_equal(a, s["rgb"].cpu().numpy())
+
+
+def test_batch_graph_replay_matches_eager(ctx):
+    print("batch graph replay", flush=True)
+    specs = scenes.batch_scenes(4, 13, tri_range=(2000, 50_000), resolutions=(64, 128), spp_range=(1, 3))
+    sc = _make(ctx, specs)
+    batch = rtamd.Batch(ctx, sc)
+    cnt = torch.zeros(5, dtype=torch.int64, device="cuda")
+    batch.rebuild()
+    batch.render(counters=cnt)  # warm-up outside capture
+    torch.cuda.synchronize()
+    eager = [s["rgb"].cpu().numpy().copy() for s in sc]
+    eager_cnt = cnt.cpu().numpy().copy()
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            cnt.zero_()
+            batch.rebuild()
+            batch.render(counters=cnt)
+    torch.cuda.current_stream().wait_stream(side)
+    for rep in range(3):
+        for s in sc:
+            s["rgb"].fill_(-1.0)  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for k, s in enumerate(sc):
+            assert np.array_equal(eager[k], s["rgb"].cpu().numpy()), f"replay {rep} scene {k}"
+        assert np.array_equal(cnt.cpu().numpy(), eager_cnt)
