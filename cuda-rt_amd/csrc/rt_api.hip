@@ -58,6 +58,7 @@ rt_status rt_ctx_create(int32_t device, rt_ctx** out) {
 void rt_ctx_destroy(rt_ctx* ctx) {
   if (!ctx) return;
   if (ctx->workspace) (void)hipFree(ctx->workspace);
+  if (ctx->diff) (void)hipFree(ctx->diff);
   delete ctx;
 }
 
@@ -259,6 +260,102 @@ rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, co
   }
   RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, counters_dev, ws,
                                (hipStream_t)stream));
+  return RT_OK;
+}
+
+
+static rt_status grow(rt_ctx* ctx, void** buf, size_t* have, size_t need, hipStream_t stream, const char* what) {
+  if (*have >= need) return RT_OK;
+  if (*buf) {
+    RT_HIP(ctx, hipStreamSynchronize(stream));
+    RT_HIP(ctx, hipFree(*buf));
+    *buf = nullptr;
+    *have = 0;
+  }
+  if (hipMalloc(buf, need) != hipSuccess) {
+    *buf = nullptr;
+    (void)hipGetLastError();
+    return rt_set_error(ctx, RT_ERR_OOM, "%s of %zu bytes", what, need);
+  }
+  *have = need;
+  return RT_OK;
+}
+
+static rt_status diff_check(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const double* albedo,
+                            double light_scale, const rt_light* light, const rt_camera* cam,
+                            const rt_render_opts* opts) {
+  if (!bvh || !mesh_ok(mesh) || !albedo || !light || !cam || !opts)
+    return rt_set_error(ctx, RT_ERR_INVALID, "null argument to the differentiable render");
+  if (!mesh->nx_dev || !mesh->ny_dev || !mesh->nz_dev)
+    return rt_set_error(ctx, RT_ERR_INVALID, "the differentiable render needs shading normals");
+  if (cam->width <= 0 || cam->height <= 0 || opts->spp <= 0 || opts->depth < 0)
+    return rt_set_error(ctx, RT_ERR_INVALID, "bad image size, spp or depth");
+  if (opts->depth > RT_DIFF_MAX_DEPTH)
+    return rt_set_error(ctx, RT_ERR_UNSUPPORTED, "differentiable render depth %d exceeds %d", opts->depth,
+                        RT_DIFF_MAX_DEPTH);
+  if (mesh->n != bvh->n) return rt_set_error(ctx, RT_ERR_INVALID, "mesh and BVH triangle counts differ");
+  if (!(light_scale > 0)) return rt_set_error(ctx, RT_ERR_INVALID, "light_scale must be positive");
+  if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT))
+    return rt_set_error(ctx, RT_ERR_INVALID, "unknown render flags");
+  return RT_OK;
+}
+
+size_t rt_diff_record_bytes(const rt_camera* cam, const rt_render_opts* opts) {
+  if (!cam || !opts || cam->width <= 0 || cam->height <= 0 || opts->spp <= 0 || opts->depth < 0) return 0;
+  return (size_t)cam->width * cam->height * opts->spp * (opts->depth + 1) * 2 * sizeof(int32_t);
+}
+
+rt_status rt_diff_forward(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const double* albedo_dev,
+                          double light_scale, const rt_light* light, const rt_camera* cam,
+                          const rt_render_opts* opts, double* image_dev, int32_t* records_dev, void* stream) {
+  if (!ctx) return RT_ERR_INVALID;
+  rt_status st = diff_check(ctx, bvh, mesh, albedo_dev, light_scale, light, cam, opts);
+  if (st != RT_OK) return st;
+  if (!image_dev || !records_dev) return rt_set_error(ctx, RT_ERR_INVALID, "null image or record buffer");
+  hipStream_t s = (hipStream_t)stream;
+  const size_t n = (size_t)mesh->n;
+  const size_t pix = (size_t)cam->width * cam->height;
+  const size_t mats_bytes = (n * sizeof(rt_material) + 255) & ~(size_t)255;
+  const size_t ids_bytes = (n * sizeof(int32_t) + 255) & ~(size_t)255;
+  st = grow(ctx, &ctx->diff, &ctx->diff_bytes, mats_bytes + ids_bytes + pix * 3 * sizeof(float), s,
+            "differentiable render buffers");
+  if (st != RT_OK) return st;
+  st = grow(ctx, &ctx->workspace, &ctx->workspace_bytes, rt_wavefront_bytes(cam->width, cam->height), s,
+            "wavefront workspace");
+  if (st != RT_OK) return st;
+  rt_material* mats = (rt_material*)ctx->diff;
+  int32_t* ids = (int32_t*)((char*)ctx->diff + mats_bytes);
+  float* rgb = (float*)((char*)ctx->diff + mats_bytes + ids_bytes);
+  RT_HIP(ctx, rt_launch_fill_albedo(mats, ids, albedo_dev, mesh->n, s));
+  RT_HIP(ctx, rt_launch_fill_i32(records_dev, rt_diff_record_bytes(cam, opts) / sizeof(int32_t), -1, s));
+  rt_mesh_view m = *mesh;
+  m.mat_id_dev = ids;
+  rt_light L = *light;
+  for (int k = 0; k < 3; ++k) L.color[k] *= light_scale;
+  rt_render_opts o = *opts;
+  o.flags = (o.flags & RT_RENDER_RANDVEC_RTL) | RT_RENDER_WAVEFRONT;
+  RT_HIP(ctx, rt_launch_render(bvh, &m, mats, mesh->n, &L, cam, &o, rgb, nullptr, ctx->workspace, s,
+                               records_dev, image_dev));
+  return RT_OK;
+}
+
+rt_status rt_diff_backward(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const double* albedo_dev,
+                           double light_scale, const rt_light* light, const rt_camera* cam,
+                           const rt_render_opts* opts, const int32_t* records_dev, const double* dimage_dev,
+                           double* g_albedo_dev, double* g_light_dev, double* g_pos_dev, void* stream) {
+  if (!ctx) return RT_ERR_INVALID;
+  rt_status st = diff_check(ctx, bvh, mesh, albedo_dev, light_scale, light, cam, opts);
+  if (st != RT_OK) return st;
+  if (!records_dev || !dimage_dev || !g_albedo_dev || !g_light_dev || !g_pos_dev)
+    return rt_set_error(ctx, RT_ERR_INVALID, "null record, image gradient or gradient buffer");
+  hipStream_t s = (hipStream_t)stream;
+  rt_render_opts o = *opts;
+  o.flags &= RT_RENDER_RANDVEC_RTL;
+  RT_HIP(ctx, rt_launch_zero_f64(g_albedo_dev, 3 * (size_t)mesh->n, s));
+  RT_HIP(ctx, rt_launch_zero_f64(g_light_dev, 1, s));
+  RT_HIP(ctx, rt_launch_zero_f64(g_pos_dev, 3 * (size_t)mesh->nv, s));
+  RT_HIP(ctx, rt_launch_adjoint(bvh, mesh, albedo_dev, light_scale, light, cam, &o, records_dev, dimage_dev,
+                                g_albedo_dev, g_light_dev, g_pos_dev, s));
   return RT_OK;
 }
 

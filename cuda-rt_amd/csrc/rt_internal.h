@@ -13,6 +13,8 @@ struct rt_ctx {
   std::string err;
   void* workspace = nullptr;     // wavefront render state, grown on demand
   size_t workspace_bytes = 0;
+  void* diff = nullptr;          // differentiable render: materials, identity ids, float image
+  size_t diff_bytes = 0;
 };
 
 // Device-side view of a built BVH, passed by value to the trace kernels.
@@ -97,9 +99,12 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
                            const rt_hits* hits, hipStream_t s);
 // workspace == nullptr selects the megakernel, else the wavefront passes
 // (rt_wavefront_bytes(width, height) bytes of device memory).
+// records/color64 (optional, wavefront only): path record and FP64 image
+// of the differentiable render.
 hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
-                            uint64_t* counters, void* workspace, hipStream_t s);
+                            uint64_t* counters, void* workspace, hipStream_t s, int32_t* records = nullptr,
+                            double* color64 = nullptr);
 size_t rt_wavefront_bytes(int32_t width, int32_t height);
 size_t rt_wavefront_bytes_batch(size_t npix, size_t ntiles, size_t nscenes);
 size_t rt_scene_desc_bytes();
@@ -109,7 +114,16 @@ void rt_fill_scene(void* table_entry, const rt_bvh* b, const rt_mesh_view* m, co
                    uint32_t pix_off, uint32_t tile_off);
 hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
-                               uint32_t* overflow, hipStream_t s);
+                               uint32_t* overflow, hipStream_t s, int32_t* records = nullptr,
+                               double* color64 = nullptr);
+// Differentiable render (rt_diff.hip).
+hipError_t rt_launch_zero_f64(double* p, size_t n, hipStream_t s);
+hipError_t rt_launch_fill_i32(int32_t* p, size_t n, int32_t v, hipStream_t s);
+hipError_t rt_launch_fill_albedo(rt_material* mats, int32_t* ids, const double* albedo, int32_t n, hipStream_t s);
+hipError_t rt_launch_adjoint(const rt_bvh* b, const rt_mesh_view* m, const double* albedo, double light_scale,
+                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts,
+                             const int32_t* records, const double* dcolor, double* g_albedo, double* g_light,
+                             double* g_pos, hipStream_t s);
 uint32_t* rt_wavefront_counts(void* workspace, size_t npix, size_t ntiles, size_t nscenes);
 rt_tuning rt_default_tuning();
 bool rt_tuning_valid(const rt_tuning* t);

@@ -17,6 +17,7 @@
 // must return the first hit in DFS order, so it walks left child first.
 #include <hip/hip_runtime.h>
 
+#include "rt_adjoint.h"
 #include "rt_internal.h"
 #include "rt_math.h"
 
@@ -661,17 +662,9 @@ struct ShadePrep {
   double fo;
 };
 
-__device__ __forceinline__ ShadePrep shade_prep(const RenderParams& P, const rt_material& m, d3 N, d3 o, d3 d,
-                                                double t, uint32_t& rng) {
-  ShadePrep S;
-  const d3 lpos = arr3(P.lpos), eye = arr3(P.eye);
-  d3 q = o + d * t;
-  S.q = q;
-  S.N = N;
-  S.I1 = arr3(m.ke) + rt::hadamard(arr3(m.ka), mk(0, 0, 0));
-  d3 V = eye - q;
-  V = mk(-V.x, -V.y, -V.z);
-  V = rt::normalize(V);
+// Light sample point: three draws in the order of mrand's rand_vec
+// (kernels/mrand.cu:33-35), offset by radius*r - 0.5 (kernels/Light.cu:71-81).
+__device__ __forceinline__ d3 light_point(const RenderParams& P, uint32_t& rng) {
   double r1, r2, r3;
   if (!(P.flags & RT_RENDER_RANDVEC_RTL)) {
     r1 = rt::rand_unit(rng); r2 = rt::rand_unit(rng); r3 = rt::rand_unit(rng);
@@ -680,15 +673,43 @@ __device__ __forceinline__ ShadePrep shade_prep(const RenderParams& P, const rt_
   }
   d3 off = mk(r1, r2, r3) * P.lrad;
   off = mk(off.x - 0.5, off.y - 0.5, off.z - 0.5);
-  d3 L = rt::normalize((lpos + off) - q);
-  S.L = L;
-  double dl = rt::length_f(lpos - q);
+  return arr3(P.lpos) + off;
+}
+
+// Shadow-ray direction, distance falloff fo = min(1, 1/(c0 + c1 d + c2 d^2))
+// to the light centre (kernels/Light.cu:56-68) and max(0, N . L).
+struct LightTerms {
+  d3 L;
+  double fo, nl;
+};
+__device__ __forceinline__ LightTerms light_terms(const RenderParams& P, d3 N, d3 q, d3 lpoint) {
+  LightTerms T;
+  T.L = rt::normalize(lpoint - q);
+  double dl = rt::length_f(arr3(P.lpos) - q);
   double fo = P.c0 + (P.c1 + P.c2 * dl) * dl;
   fo = (fabs(fo) < kRayEps) ? 1.0 : 1.0 / fo;
-  S.fo = fmin(1.0, fo);
-  double nl = rt::dot(N, L);
-  if (nl < 0) nl = 0;
-  d3 diffuse = arr3(m.kd) * nl;
+  T.fo = fmin(1.0, fo);
+  double nl = rt::dot(N, T.L);
+  T.nl = nl < 0 ? 0.0 : nl;
+  return T;
+}
+
+__device__ __forceinline__ ShadePrep shade_prep(const RenderParams& P, const rt_material& m, d3 N, d3 o, d3 d,
+                                                double t, uint32_t& rng) {
+  ShadePrep S;
+  const d3 eye = arr3(P.eye);
+  d3 q = o + d * t;
+  S.q = q;
+  S.N = N;
+  S.I1 = arr3(m.ke) + rt::hadamard(arr3(m.ka), mk(0, 0, 0));
+  d3 V = eye - q;
+  V = mk(-V.x, -V.y, -V.z);
+  V = rt::normalize(V);
+  LightTerms T = light_terms(P, N, q, light_point(P, rng));
+  const d3 L = T.L;
+  S.L = L;
+  S.fo = T.fo;
+  d3 diffuse = arr3(m.kd) * T.nl;
   d3 spec = mk(0, 0, 0);
   if (!(m.ks[0] == 0 && m.ks[1] == 0 && m.ks[2] == 0)) {
     d3 R = N * (-2.0 * rt::dot(L, N)) + L;
@@ -860,7 +881,15 @@ struct WfState {
   double* st;           // cap: any-hit t
   uint32_t* counts;     // [0],[1] ray queue sizes, [2] shadow entries, [8] stack overflows
   uint32_t* fetch[2];   // kFetchWords each: closest-hit and any-hit fetch counters
+  int2* rec;            // optional path record: (hit leaf, occluder code) per pixel, sample, bounce
+  int rec_spp, rec_depth;
+  double* color64;      // optional FP64 output (pixels*3, row-flipped like rgb, unclamped mean)
 };
+
+// Path record index of (global pixel, sample, bounce).
+__device__ __forceinline__ size_t rec_index(const WfState& W, uint32_t p, int sample, int bounce) {
+  return ((size_t)p * W.rec_spp + sample) * (W.rec_depth + 1) + bounce;
+}
 
 __device__ __forceinline__ d3 ld3(const double* p) { return mk(p[0], p[1], p[2]); }
 __device__ __forceinline__ void st3(double* p, d3 v) {
@@ -942,7 +971,8 @@ __global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, int k, uint32
   st3(W.acc + 3 * (size_t)gp, mk(0, 0, 0));
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, int bounce, int cur, unsigned long long* counters) {
+__global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, int bounce, int cur, int sample,
+                                                       unsigned long long* counters) {
   __shared__ uint32_t s_tmp[8];
   const uint32_t n_in = W.counts[cur];
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
@@ -957,6 +987,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, int bounce, in
     const SceneDesc& D = W.scenes[sc];
     thr = ld3(W.thr + 3 * (size_t)p);
     const int pos = W.hpos[i];
+    if (W.rec) W.rec[rec_index(W, p, sample, bounce)] = make_int2(pos, -1);
     if (pos < 0) {
       double* a = W.acc + 3 * (size_t)p;
       st3(a, ld3(a) + rt::hadamard(thr, mk(0.9, 0.9, 0.9)));
@@ -1009,7 +1040,18 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, int bounce, in
   }
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, unsigned long long* counters) {
+// Occluder code of a shadow query: -1 = unshadowed (factor 1), -2 = opaque
+// occluder (factor 0), else the leaf position of the transmitting occluder
+// (factor kd % (1 - kt), kernels/Light.cu:17-26).
+__device__ __forceinline__ int occluder_code(const SceneDesc& D, d3 q, const HitRec& s) {
+  if (s.pos < 0 || !(s.t < rt::length_f(arr3(D.P.lpos) - q)) || s.t < kRayEps) return -1;
+  const rt_material& sm = mat_of(D.mats, D.P.nmat, D.M, leaf_tri_of(D.B, s.pos));
+  bool trans = (sm.kt[0] == 0) && (sm.kt[1] == 0) && (sm.kt[2] == 0);
+  return trans ? s.pos : -2;
+}
+
+__global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, int sample, int bounce,
+                                                               unsigned long long* counters) {
   const uint32_t n_in = W.counts[2];
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   const bool live = i < n_in;
@@ -1028,6 +1070,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, unsign
   uint32_t p = W.spix[i];
   double* a = W.acc + 3 * (size_t)p;
   st3(a, ld3(a) + rt::hadamard(ld3(e + 12), I));
+  if (W.rec) W.rec[rec_index(W, p, sample, bounce)].y = occluder_code(D, q, s);
 }
 
 // Reset the listed counters and, if fetch != nullptr, `words` fetch
@@ -1052,7 +1095,12 @@ __global__ __launch_bounds__(kThreads) void k_wf_resolve(WfState W) {
   uint32_t gp;
   if (!tile_pixel(W, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
   const SceneDesc& D = W.scenes[s];
-  write_pixel(D.P, D.rgb, px, py, ld3(W.color + 3 * (size_t)gp));
+  d3 c = ld3(W.color + 3 * (size_t)gp);
+  if (D.rgb) write_pixel(D.P, D.rgb, px, py, c);
+  if (W.color64) {
+    size_t oi = 3 * ((size_t)(D.P.H - py - 1) * D.P.W + px);
+    st3(W.color64 + oi, mk(c.x / double(D.P.spp), c.y / double(D.P.spp), c.z / double(D.P.spp)));
+  }
 }
 
 // Copy a scene table that came in as a kernel argument to device memory
@@ -1244,8 +1292,12 @@ size_t rt_bvh_view_bytes() { return sizeof(BvhView); }
 // tile->scene map are already in the workspace.
 hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
-                               uint32_t* overflow, hipStream_t s) {
+                               uint32_t* overflow, hipStream_t s, int32_t* records, double* color64) {
   WfState W = wf_state(workspace, npix, ntiles, nscenes);
+  W.rec = (int2*)records;
+  W.rec_spp = max_spp;
+  W.rec_depth = max_depth;
+  W.color64 = color64;
   rt_tuning t = tuning ? *tuning : rt_default_tuning();
   unsigned long long* cnt = (unsigned long long*)counters;
   const size_t cap = ntiles * kThreads;
@@ -1262,11 +1314,11 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
       HitOut O{W.hpos, nullptr, nullptr, nullptr, nullptr, W.hit};
       launch_persistent<false>(t, grid, s, W.views, W.rscene[cur], R, bounce == 0 ? 0.0 : kBounceTmin, O,
                                W.fetch[0], ovf, cnt);
-      k_wf_shade<<<tiles, kThreads, 0, s>>>(W, bounce, cur, cnt);
+      k_wf_shade<<<tiles, kThreads, 0, s>>>(W, bounce, cur, it, cnt);
       RayIn SR{W.shq, W.shq + 3, 16, &W.counts[2], 0};
       HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr};
       launch_persistent<true>(t, grid, s, W.views, W.sscene, SR, 0.0, SO, W.fetch[1], ovf, cnt);
-      k_wf_shadow_finish<<<tiles, kThreads, 0, s>>>(W, cnt);
+      k_wf_shadow_finish<<<tiles, kThreads, 0, s>>>(W, it, bounce, cnt);
       k_wf_reset<<<1, 256, 0, s>>>(W.counts, (1u << 2) | (1u << cur), W.fetch[0], 2 * kFetchWords);
       cur ^= 1;
     }
@@ -1278,7 +1330,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
 
 hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
-                            uint64_t* counters, void* workspace, hipStream_t s) {
+                            uint64_t* counters, void* workspace, hipStream_t s, int32_t* records, double* color64) {
   RenderParams P = rt_make_params(cam, light, opts, nmat);
   MeshShade M{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev};
   const int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
@@ -1300,5 +1352,237 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   k_wf_put_scene<<<cdiv(tiles, 256), 256, 0, s>>>((SceneDesc*)W.scenes, (BvhView*)W.views,
                                                   (uint16_t*)W.tile_scene, d, (uint32_t)tiles);
   return rt_launch_wavefront(workspace, (size_t)P.W * P.H, (size_t)tiles, 1, P.spp, P.depth,
-                             b->ctx ? b->ctx->num_cus : 256, nullptr, counters, ovf, s);
+                             b->ctx ? b->ctx->num_cus : 256, nullptr, counters, ovf, s, records, color64);
+}
+
+// ------------------------------------------------------------ differentiable render
+// Per-triangle albedo as a material table: material t = (kd = albedo[t],
+// ka = ks = kt = ke = 0), so Trans() is true and an occluder attenuates
+// shadows by its albedo (kernels/Light.cu:22-24).
+namespace {
+
+__global__ void k_fill_albedo(rt_material* mats, int32_t* ids, const double* albedo, int32_t n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  ids[t] = t;
+  rt_material m;
+  for (int k = 0; k < 3; ++k) {
+    m.ka[k] = 0.0;
+    m.kd[k] = albedo[3 * (size_t)t + k];
+    m.ks[k] = 0.0;
+    m.kt[k] = 0.0;
+    m.ke[k] = 0.0;
+  }
+  m.shininess = 0.0;
+  mats[t] = m;
+}
+
+__global__ void k_zero_f64(double* p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = 0.0;
+}
+
+__global__ void k_fill_i32(int32_t* p, size_t n, int32_t v) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+constexpr int kMaxDiffBounces = 8;
+
+struct DiffBounce {
+  d3 o, d, lpoint;
+  double t, al, be;
+  double g;           // fo * max(0, N . L) as the forward rounds it
+  int tri, occ_tri;   // occ_tri: -1 unshadowed, -2 opaque, else triangle
+  bool next, flip;
+  double xs, ys, zs;
+};
+
+__device__ __forceinline__ rta::v3 to_a(d3 v) { return rta::V(v.x, v.y, v.z); }
+__device__ __forceinline__ d3 from_a(rta::v3 v) { return mk(v.x, v.y, v.z); }
+
+__device__ __forceinline__ d3 alb(const double* a, int tri) {
+  return mk(a[3 * (size_t)tri], a[3 * (size_t)tri + 1], a[3 * (size_t)tri + 2]);
+}
+__device__ __forceinline__ void atomic_add3(double* dst, d3 v) {
+  atomicAdd(dst + 0, v.x);
+  atomicAdd(dst + 1, v.y);
+  atomicAdd(dst + 2, v.z);
+}
+
+// Adjoint of the render: one thread per pixel replays each sample's path
+// from the record (same draws, no traversal) and propagates dLoss/dcolor
+// back to per-triangle albedo, light intensity and vertex positions.
+__global__ __launch_bounds__(kThreads) void k_adjoint(BvhView B, MeshShade M, const float* __restrict__ vx,
+                                                      const float* __restrict__ vy, const float* __restrict__ vz,
+                                                      RenderParams P, const double* __restrict__ albedo,
+                                                      double light_scale, const int2* __restrict__ rec,
+                                                      const double* __restrict__ dcolor, double* g_alb,
+                                                      double* g_light, double* g_pos) {
+  int px, py;
+  pixel_of_tile(P, blockIdx.x, threadIdx.x, px, py);
+  double light_acc = 0.0;
+  if (px < P.W && py < P.H) {
+    const uint32_t p = (uint32_t)(py * P.W + px);
+    uint32_t rng = rt::wang_hash(rt::seed_index(px, py, P.W) ^ P.seed);
+    double xx, yy;
+    pixel_xy(P, px, py, xx, yy);
+    const size_t oi = 3 * ((size_t)(P.H - py - 1) * P.W + px);
+    const d3 w = mk(dcolor[oi], dcolor[oi + 1], dcolor[oi + 2]);
+    const double inv_spp = 1.0 / double(P.spp);
+    const d3 color = mk(P.lcol[0] / light_scale, P.lcol[1] / light_scale, P.lcol[2] / light_scale);
+    const d3 lcol = arr3(P.lcol), lcenter = arr3(P.lpos);
+    for (int s = 0; s < P.spp; ++s) {
+      d3 o, d;
+      lens_ray(P, xx, yy, rng, o, d);
+      DiffBounce br[kMaxDiffBounces];
+      int nb = 0;
+      bool miss = false;
+      for (int k = 0; k <= P.depth; ++k) {
+        int2 r = rec[((size_t)p * P.spp + s) * (P.depth + 1) + k];
+        if (r.x < 0) {
+          miss = true;
+          break;
+        }
+        DiffBounce& b = br[nb++];
+        b.o = o;
+        b.d = d;
+        b.tri = leaf_tri_of(B, r.x);
+        RayD ray = make_ray(o, d);
+        tri_test(ray, B.ltri + 3 * (size_t)r.x, b.t, b.al, b.be);
+        b.lpoint = light_point(P, rng);
+        const d3 N0 = shading_normal(M, b.tri, b.al, b.be);
+        const LightTerms T = light_terms(P, N0, o + d * b.t, b.lpoint);
+        b.g = T.fo * T.nl;
+        b.occ_tri = r.y >= 0 ? leaf_tri_of(B, r.y) : r.y;
+        b.next = k < P.depth;
+        if (b.next) {
+          double u1 = rt::rand_unit(rng);
+          double u2 = rt::rand_unit(rng);
+          d3 N = N0;
+          b.flip = rt::dot(N, d) > 0;
+          if (b.flip) N = mk(-N.x, -N.y, -N.z);
+          double sphi, cphi;
+          rt::sincos_cw(2.0 * 3.141592653589793 * u2, &sphi, &cphi);
+          b.xs = sqrt(u1) * cphi;
+          b.ys = sqrt(1.0 - u1);
+          b.zs = sqrt(u1) * sphi;
+          o = o + d * b.t;
+          d = cosine_dir(N, u1, u2);
+        }
+      }
+      // Throughput before each bounce: thr_k = product of the albedos hit before k.
+      d3 thr[kMaxDiffBounces + 1];
+      thr[0] = mk(1, 1, 1);
+      for (int k = 0; k < nb; ++k) thr[k + 1] = rt::hadamard(thr[k], alb(albedo, br[k].tri));
+      // Reverse sweep. Each bounce term gives derivatives to its own albedo,
+      // to every albedo in its throughput (products that leave that factor
+      // out, so no division) and to its occluder's albedo; the position
+      // adjoint flows back through the shading, the next ray and the hit.
+      rta::v3 obar = rta::V(0, 0, 0), dbar = rta::V(0, 0, 0);
+      for (int k = nb - 1; k >= 0; --k) {
+        const DiffBounce& b = br[k];
+        const d3 kd = alb(albedo, b.tri);
+        d3 sh = mk(1, 1, 1);
+        if (b.occ_tri == -2) sh = mk(0, 0, 0);
+        if (b.occ_tri >= 0) sh = alb(albedo, b.occ_tri);
+        const int ia = M.idx[3 * b.tri], ib = M.idx[3 * b.tri + 1], ic = M.idx[3 * b.tri + 2];
+        const rta::v3 na = rta::V(M.nx[ia], M.ny[ia], M.nz[ia]), nbv = rta::V(M.nx[ib], M.ny[ib], M.nz[ib]),
+                      nc = rta::V(M.nx[ic], M.ny[ic], M.nz[ic]);
+        rta::Shade S = rta::shade_fwd(to_a(b.o), to_a(b.d), b.t, b.al, b.be, na, nbv, nc, to_a(b.lpoint),
+                                      to_a(lcenter), P.c0, P.c1, P.c2);
+        // term_k = inv_spp * w % thr_k % sh % kd % lcol * g
+        // term_k = inv_spp * w % thr_k % sh % kd_k % lcol * g_k
+        d3 base = rt::hadamard(rt::hadamard(w, thr[k]), rt::hadamard(sh, lcol)) * inv_spp;  // all but kd, g
+        double gbar = rt::dot(rt::hadamard(base, kd), mk(1, 1, 1));
+        light_acc += rt::dot(rt::hadamard(rt::hadamard(w, thr[k]), rt::hadamard(rt::hadamard(sh, kd), color)),
+                             mk(1, 1, 1)) * inv_spp * b.g;
+        // d/d kd_k
+        atomic_add3(g_alb + 3 * (size_t)b.tri, base * b.g);
+        // d/d albedo of earlier hits (through thr_k)
+        for (int j = 0; j < k; ++j) {
+          d3 ex = mk(1, 1, 1);
+          for (int i = 0; i < k; ++i)
+            if (i != j) ex = rt::hadamard(ex, alb(albedo, br[i].tri));
+          d3 gj = rt::hadamard(rt::hadamard(rt::hadamard(w, ex), rt::hadamard(sh, kd)), lcol) * (inv_spp * b.g);
+          atomic_add3(g_alb + 3 * (size_t)br[j].tri, gj);
+        }
+        // d/d occluder albedo
+        if (b.occ_tri >= 0)
+          atomic_add3(g_alb + 3 * (size_t)b.occ_tri,
+                      rt::hadamard(rt::hadamard(rt::hadamard(w, thr[k]), kd), lcol) * (inv_spp * b.g));
+        // positions: next-ray adjoint -> normal, then shading, then the hit
+        rta::v3 Nextra = rta::V(0, 0, 0);
+        rta::v3 qbar = obar;
+        if (b.next) {
+          rta::v3 Nf = b.flip ? -S.N : S.N;
+          rta::Frame F = rta::frame_fwd(Nf, b.xs, b.ys, b.zs);
+          rta::v3 nfbar = rta::frame_adj(F, Nf, dbar);
+          Nextra = b.flip ? -nfbar : nfbar;
+        }
+        rta::ShadeAdj a = rta::shade_adj(S, na, nbv, nc, P.c1, P.c2, gbar, Nextra);
+        qbar += a.q;
+        // q = o + t d
+        rta::v3 ob = qbar, db = qbar * b.t;
+        double tbar = rta::dot(qbar, to_a(b.d));
+        const rta::v3 A = rta::V(vx[ia], vy[ia], vz[ia]), Bv = rta::V(vx[ib], vy[ib], vz[ib]),
+                      C = rta::V(vx[ic], vy[ic], vz[ic]);
+        rta::TriAdj h = rta::hit_adj(A, Bv, C, to_a(b.o), to_a(b.d), b.t, b.al, b.be, tbar, a.al, a.be);
+        ob += h.o;
+        db += h.d;
+        atomic_add3(g_pos + 3 * (size_t)ia, from_a(h.A));
+        atomic_add3(g_pos + 3 * (size_t)ib, from_a(h.B));
+        atomic_add3(g_pos + 3 * (size_t)ic, from_a(h.C));
+        obar = ob;
+        dbar = db;
+      }
+      if (miss) {
+        // background term inv_spp * w % thr_nb * 0.9: albedos of all hits before the miss
+        for (int j = 0; j < nb; ++j) {
+          d3 ex = mk(1, 1, 1);
+          for (int i = 0; i < nb; ++i)
+            if (i != j) ex = rt::hadamard(ex, alb(albedo, br[i].tri));
+          atomic_add3(g_alb + 3 * (size_t)br[j].tri, rt::hadamard(w, ex) * (0.9 * inv_spp));
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) light_acc += __shfl_xor(light_acc, off);
+  if ((threadIdx.x & 63) == 0 && light_acc != 0.0) atomicAdd(g_light, light_acc);
+}
+
+}  // namespace
+
+hipError_t rt_launch_fill_albedo(rt_material* mats, int32_t* ids, const double* albedo, int32_t n, hipStream_t s) {
+  k_fill_albedo<<<cdiv(n, 256), 256, 0, s>>>(mats, ids, albedo, n);
+  return hipGetLastError();
+}
+
+hipError_t rt_launch_fill_i32(int32_t* p, size_t n, int32_t v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = (n + 255) / 256;
+  k_fill_i32<<<(unsigned)(blocks < 65535 ? blocks : 65535), 256, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t rt_launch_zero_f64(double* p, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const size_t blocks = (n + 255) / 256;
+  k_zero_f64<<<(unsigned)(blocks < 65535 ? blocks : 65535), 256, 0, s>>>(p, n);
+  return hipGetLastError();
+}
+
+hipError_t rt_launch_adjoint(const rt_bvh* b, const rt_mesh_view* m, const double* albedo, double light_scale,
+                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts,
+                             const int32_t* records, const double* dcolor, double* g_albedo, double* g_light,
+                             double* g_pos, hipStream_t s) {
+  rt_light L = *light;
+  for (int k = 0; k < 3; ++k) L.color[k] *= light_scale;
+  RenderParams P = rt_make_params(cam, &L, opts, m->n);
+  MeshShade M{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev};
+  const int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
+  k_adjoint<<<tiles, kThreads, 0, s>>>(b->view(), M, m->vx_dev, m->vy_dev, m->vz_dev, P, albedo, light_scale,
+                                       (const int2*)records, dcolor, g_albedo, g_light, g_pos);
+  return hipGetLastError();
 }

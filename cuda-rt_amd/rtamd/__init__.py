@@ -419,6 +419,7 @@ def _bind_batch(L):
     L._batch_bound = True
 
 
+EXPORTED += ["rt_diff_record_bytes", "rt_diff_forward", "rt_diff_backward"]
 EXPORTED += ["rt_batch_create", "rt_batch_rebuild", "rt_batch_render", "rt_batch_set_tuning", "rt_batch_get_tuning",
              "rt_batch_autotune", "rt_batch_diagnostics", "rt_batch_destroy"]
 

@@ -216,6 +216,40 @@ rt_status rt_batch_autotune(rt_batch *batch, void *stream, rt_tuning *best, floa
 rt_status rt_batch_diagnostics(const rt_batch *batch, void *stream, uint32_t out[1]);
 void rt_batch_destroy(rt_batch *batch);
 
+/* ---- differentiable render (no reference counterpart: the reference
+ * renders forward only, kernels/raytracer.cu:94-249). The scene is the
+ * rt_render scene with one material per triangle, kd = albedo[t] (RGB,
+ * doubles) and ka = ks = kt = ke = 0, and the light colour scaled by
+ * light_scale. Derivatives of the image are taken with respect to the
+ * albedos, light_scale and the vertex positions through the shading
+ * (hit point, barycentrics, interpolated normal, light direction,
+ * falloff, next bounce origin and direction). Which triangle each ray hits
+ * and whether each shadow ray is blocked are held fixed: the visibility
+ * (silhouette edge) terms are not included. Vertex normals are inputs. */
+#define RT_DIFF_MAX_DEPTH 7
+
+/* Bytes of the path record for one image: per pixel (py*width+px),
+ * sample and bounce (depth+1 slots) a pair of int32: the sorted leaf
+ * position hit (-1: miss) and the shadow ray's occluder code (-1: none,
+ * -2: opaque, else the occluding leaf position). Slots after a path ends
+ * hold -1. */
+size_t rt_diff_record_bytes(const rt_camera *cam, const rt_render_opts *opts);
+/* Forward render into image_dev (width*height*3 doubles, row-flipped like
+ * rt_render, the unclamped sample mean) and record every path in
+ * records_dev. RT_RENDER_CLAMP is ignored; the wavefront path is used. The
+ * BVH must be built or refit on the mesh's current positions. */
+rt_status rt_diff_forward(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const double *albedo_dev,
+                          double light_scale, const rt_light *light, const rt_camera *cam,
+                          const rt_render_opts *opts, double *image_dev, int32_t *records_dev, void *stream);
+/* Adjoint render: given dLoss/dimage (layout of image_dev), overwrite
+ * g_albedo_dev (n*3), g_light_dev (1) and g_pos_dev (nv*3, xyz per vertex)
+ * with dLoss/dalbedo, dLoss/dlight_scale and dLoss/dposition. Arguments
+ * other than the gradients must be those of the matching rt_diff_forward. */
+rt_status rt_diff_backward(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const double *albedo_dev,
+                           double light_scale, const rt_light *light, const rt_camera *cam,
+                           const rt_render_opts *opts, const int32_t *records_dev, const double *dimage_dev,
+                           double *g_albedo_dev, double *g_light_dev, double *g_pos_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

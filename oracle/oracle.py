@@ -87,6 +87,8 @@ def lib():
         L.orc_trace_brute.argtypes = [vp, c_i32, ctypes.c_int64, c_d, c_d, c_i32, c_d]
         L.orc_render.argtypes = [vp, vp, vp, ctypes.c_int32, vp, vp, vp, c_f, c_i64, ctypes.c_int64, c_i64,
                                  ctypes.c_int32]
+        L.orc_render64.argtypes = [vp, vp, vp, ctypes.c_int32, vp, vp, vp, c_f, c_d, c_i32, c_i32, c_i64,
+                                   ctypes.c_int64, c_i64, ctypes.c_int32]
         _lib = L
     return _lib
 
@@ -238,8 +240,13 @@ def _mat(m):
     return o
 
 
-def render(hm, b, mats, light, cam, opts, pixels=None, nthreads=0):
-    """mats/light/cam/opts may be rtamd ctypes structs (same layout) or oracle ones."""
+def render(hm, b, mats, light, cam, opts, pixels=None, nthreads=0, fp64=False, records=False, branches=False):
+    """mats/light/cam/opts may be rtamd ctypes structs (same layout) or oracle ones.
+
+    Returns (rgb float32, counts); with fp64 also the image in doubles, with
+    records also the (pixels, spp, depth+1, 2) path record (hit triangle,
+    occluder code), with branches also the (pixels, spp, depth+1) shading
+    branch bits."""
     om = OMesh(hm)
     ob = OBvh(b)
     marr = (Material * len(mats))(*[_mat(m) for m in mats])
@@ -256,9 +263,24 @@ def render(hm, b, mats, light, cam, opts, pixels=None, nthreads=0):
         pp, npix = _p(px, c_i64), px.size
     else:
         pp, npix = None, 0
-    lib().orc_render(ctypes.byref(om.s), ctypes.byref(ob.s), marr, len(mats), ctypes.byref(L), ctypes.byref(C),
-                     ctypes.byref(O), _p(rgb, c_f), pp, npix, _p(counts, c_i64), nthreads)
-    return rgb, counts
+    if not fp64 and not records and not branches:
+        lib().orc_render(ctypes.byref(om.s), ctypes.byref(ob.s), marr, len(mats), ctypes.byref(L),
+                         ctypes.byref(C), ctypes.byref(O), _p(rgb, c_f), pp, npix, _p(counts, c_i64), nthreads)
+        return rgb, counts
+    rgb64 = np.zeros(cam.width * cam.height * 3, np.float64)
+    rec = np.full((cam.width * cam.height, opts.spp, opts.depth + 1, 2), -1, np.int32)
+    br = np.full((cam.width * cam.height, opts.spp, opts.depth + 1), -1, np.int32)
+    lib().orc_render64(ctypes.byref(om.s), ctypes.byref(ob.s), marr, len(mats), ctypes.byref(L), ctypes.byref(C),
+                       ctypes.byref(O), _p(rgb, c_f), _p(rgb64, c_d), _p(rec, c_i32), _p(br, c_i32), pp, npix,
+                       _p(counts, c_i64), nthreads)
+    out = (rgb, counts)
+    if fp64:
+        out += (rgb64,)
+    if records:
+        out += (rec,)
+    if branches:
+        out += (br,)
+    return out
 
 
 def sincos(x):

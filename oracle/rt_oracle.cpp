@@ -321,14 +321,17 @@ static const orc_material &material_of(const RenderCtx &c, int32_t tri) {
 }
 
 // kernels/Light.cu:5-53 (positional light): shadow factor.
-static V3 shadow_atten(const RenderCtx &c, V3 q, V3 Ldir, int64_t *cnt) {
+// *occ (if given): -1 unshadowed, -2 opaque occluder, else the occluder's triangle.
+static V3 shadow_atten(const RenderCtx &c, V3 q, V3 Ldir, int64_t *cnt, int32_t *occ) {
   Hit h;
   ++cnt[1];
+  if (occ) *occ = -1;
   if (traverse(c.m, c.b, q, Ldir, true, 0.0, &h, nullptr)) {
     if (h.t < norm(sub(arr3(c.L->position), q))) {
       if (h.t < kRayEps) return v3(1, 1, 1);
       const orc_material &mt = material_of(c, h.tri);
       bool trans = (mt.kt[0] == 0) && (mt.kt[1] == 0) && (mt.kt[2] == 0);  // inc/material.h:31
+      if (occ) *occ = trans ? h.tri : -2;
       if (trans) return mulv(arr3(mt.kd), sub(v3(1, 1, 1), arr3(mt.kt)));
       return v3(0, 0, 0);
     }
@@ -338,15 +341,18 @@ static V3 shadow_atten(const RenderCtx &c, V3 q, V3 Ldir, int64_t *cnt) {
 }
 
 // kernels/Light.cu:56-68
-static double dist_atten(const RenderCtx &c, V3 q) {
+static double dist_atten(const RenderCtx &c, V3 q, bool *clamped) {
   double d = norm(sub(arr3(c.L->position), q));
   double f = c.L->c0 + (c.L->c1 + c.L->c2 * d) * d;
   f = (fabs(f) < kRayEps) ? 1.0 : 1.0 / f;
+  *clamped = !(f < 1.0);
   return fmin(1.0, f);
 }
 
 // kernels/material.cu:6-40 with Light::getDirection (kernels/Light.cu:71-81).
-static V3 shade(const RenderCtx &c, V3 p, V3 d, const Hit &h, uint32_t &rng, int64_t *cnt) {
+// *br (if given) gets bit 0: N.L > 0, bit 1: falloff clamped to 1.
+static V3 shade(const RenderCtx &c, V3 p, V3 d, const Hit &h, uint32_t &rng, int64_t *cnt, int32_t *occ,
+                int32_t *br) {
   const orc_material &mt = material_of(c, h.tri);
   V3 q = add(p, scale(d, h.t));
   V3 I = add(arr3(mt.ke), mulv(arr3(mt.ka), v3(0, 0, 0)));
@@ -356,8 +362,10 @@ static V3 shade(const RenderCtx &c, V3 p, V3 d, const Hit &h, uint32_t &rng, int
   V3 off = scale(rand_vec(rng, (c.o->flags & ORC_RANDVEC_RTL) != 0), c.L->radius);
   off = v3(off.x - 0.5, off.y - 0.5, off.z - 0.5);
   V3 L = normalize(sub(add(arr3(c.L->position), off), q));
-  V3 atten = scale(shadow_atten(c, q, L, cnt), dist_atten(c, q));
+  bool clamped;
+  V3 atten = scale(shadow_atten(c, q, L, cnt, occ), dist_atten(c, q, &clamped));
   double nl = dot(h.N, L);
+  if (br) *br = (nl > 0 ? 1 : 0) | (clamped ? 2 : 0);
   if (nl < 0) nl = 0;
   V3 diffuse = scale(arr3(mt.kd), nl);
   V3 R = add(scale(h.N, -2.0 * dot(L, h.N)), L);
@@ -372,19 +380,23 @@ static V3 shade(const RenderCtx &c, V3 p, V3 d, const Hit &h, uint32_t &rng, int
 // Cosine-weighted hemisphere direction about n, following the frame
 // construction of src/ray.cpp:5-32 with theta = acos(sqrt(1-u1)) written
 // as sin(theta) = sqrt(u1), cos(theta) = sqrt(1-u1).
-static V3 cosine_dir(V3 n, double u1, double u2) {
+static V3 cosine_dir(V3 n, double u1, double u2, int *axis) {
   double sphi, cphi;
   orc_math::sincos(2.0 * 3.141592653589793 * u2, &sphi, &cphi);
   double st = sqrt(u1), ct = sqrt(1.0 - u1);
   double xs = st * cphi, ys = ct, zs = st * sphi;
   V3 h = n;
   float ax = fabsf((float)h.x), ay = fabsf((float)h.y), az = fabsf((float)h.z);
-  if (ax <= ay && ax <= az)
+  if (ax <= ay && ax <= az) {
     h.x = 1.0;
-  else if (ay <= ax && ay <= az)
+    *axis = 0;
+  } else if (ay <= ax && ay <= az) {
     h.y = 1.0;
-  else
+    *axis = 1;
+  } else {
     h.z = 1.0;
+    *axis = 2;
+  }
   V3 x = normalize(cross(h, n));
   V3 z = normalize(cross(x, n));
   V3 dir = add(add(scale(x, xs), scale(n, ys)), scale(z, zs));
@@ -393,28 +405,38 @@ static V3 cosine_dir(V3 n, double u1, double u2) {
 
 static const double kBounceTmin = 1e-6;
 
-// kernels/raytracer.cu:168-249 plus the diffuse-bounce extension.
-static V3 trace_path(const RenderCtx &c, V3 p, V3 d, uint32_t &rng, int64_t *cnt) {
+// kernels/raytracer.cu:168-249 plus the diffuse-bounce extension. rec (if
+// given) receives per bounce (hit triangle or -1, occluder code of shade()),
+// br (if given) per bounce the branch bits of shade() plus bit 2: normal
+// flipped and bits 3-4: the bounce frame's replaced axis.
+static V3 trace_path(const RenderCtx &c, V3 p, V3 d, uint32_t &rng, int64_t *cnt, int32_t *rec, int32_t *br) {
   V3 color = v3(0, 0, 0), thr = v3(1, 1, 1);
   for (int bounce = 0;; ++bounce) {
     Hit h;
     bool hit = traverse(c.m, c.b, p, d, false, bounce == 0 ? 0.0 : kBounceTmin, &h, nullptr);
+    if (rec) {
+      rec[2 * bounce] = hit ? h.tri : -1;
+      rec[2 * bounce + 1] = -1;
+    }
     if (!hit) {
       color = add(color, mulv(thr, v3(0.9, 0.9, 0.9)));
       break;
     }
-    V3 s = shade(c, p, d, h, rng, cnt);
+    V3 s = shade(c, p, d, h, rng, cnt, rec ? rec + 2 * bounce + 1 : nullptr, br ? br + bounce : nullptr);
     color = add(color, mulv(thr, s));
     if (bounce >= c.o->depth) break;
     double u1 = rand_double(rng);
     double u2 = rand_double(rng);
     V3 n = h.N;
-    if (dot(n, d) > 0) n = v3(-n.x, -n.y, -n.z);
+    const bool flip = dot(n, d) > 0;
+    if (flip) n = v3(-n.x, -n.y, -n.z);
     V3 q = add(p, scale(d, h.t));
     const orc_material &mt = material_of(c, h.tri);
     thr = mulv(thr, arr3(mt.kd));
     p = q;
-    d = cosine_dir(n, u1, u2);
+    int axis;
+    d = cosine_dir(n, u1, u2, &axis);
+    if (br) br[bounce] |= (flip ? 4 : 0) | (axis << 3);
     ++cnt[2];
   }
   return color;
@@ -443,7 +465,8 @@ static void pixel_xy(const orc_camera *cam, int px, int py, double *xx, double *
   *yy = (1 - 2 * ((py + 0.5) * invH)) * angle;
 }
 
-static void render_pixel(const RenderCtx &c, int px, int py, float *rgb, int64_t *cnt) {
+static void render_pixel(const RenderCtx &c, int px, int py, float *rgb, double *rgb64, int32_t *rec,
+                         int32_t *branches, int64_t *cnt) {
   const orc_camera *cam = c.cam;
   int W = cam->width, H = cam->height;
   int64_t oidx = (int64_t)(H - py - 1) * W + px;  // kernels/raytracer.cu:106
@@ -456,7 +479,14 @@ static void render_pixel(const RenderCtx &c, int px, int py, float *rgb, int64_t
     V3 o, d;
     lens_ray(cam, xx, yy, rng, &o, &d);
     ++cnt[0];
-    color = add(color, trace_path(c, o, d, rng, cnt));
+    const int64_t slot = ((int64_t)(py * W + px) * N + it) * (c.o->depth + 1);
+    int32_t *r = rec ? rec + 2 * slot : nullptr;
+    int32_t *br = branches ? branches + slot : nullptr;
+    for (int k = 0; k <= c.o->depth; ++k) {
+      if (r) r[2 * k] = r[2 * k + 1] = -1;
+      if (br) br[k] = -1;
+    }
+    color = add(color, trace_path(c, o, d, rng, cnt, r, br));
   }
   color = v3(color.x / double(N), color.y / double(N), color.z / double(N));
   if (c.o->flags & ORC_CLAMP) {  // kernels/scene.cu:137, inc/vec.h:123-134
@@ -466,9 +496,16 @@ static void render_pixel(const RenderCtx &c, int px, int py, float *rgb, int64_t
       if (*v > 1) *v = 1.0;
     }
   }
-  rgb[3 * oidx + 0] = (float)color.x;
-  rgb[3 * oidx + 1] = (float)color.y;
-  rgb[3 * oidx + 2] = (float)color.z;
+  if (rgb) {
+    rgb[3 * oidx + 0] = (float)color.x;
+    rgb[3 * oidx + 1] = (float)color.y;
+    rgb[3 * oidx + 2] = (float)color.z;
+  }
+  if (rgb64) {
+    rgb64[3 * oidx + 0] = color.x;
+    rgb64[3 * oidx + 1] = color.y;
+    rgb64[3 * oidx + 2] = color.z;
+  }
 }
 
 static void set_threads(int32_t nthreads) {
@@ -799,6 +836,13 @@ void orc_primary_ray(const orc_camera *cam, const orc_render_opts *opts, int32_t
 void orc_render(const orc_mesh *m, const orc_bvh *b, const orc_material *mats, int32_t nmat, const orc_light *light,
                 const orc_camera *cam, const orc_render_opts *opts, float *rgb, const int64_t *pixels, int64_t npix,
                 int64_t *counts, int32_t nthreads) {
+  orc_render64(m, b, mats, nmat, light, cam, opts, rgb, nullptr, nullptr, nullptr, pixels, npix, counts, nthreads);
+}
+
+void orc_render64(const orc_mesh *m, const orc_bvh *b, const orc_material *mats, int32_t nmat,
+                  const orc_light *light, const orc_camera *cam, const orc_render_opts *opts, float *rgb,
+                  double *rgb64, int32_t *records, int32_t *branches, const int64_t *pixels, int64_t npix,
+                  int64_t *counts, int32_t nthreads) {
   set_threads(nthreads);
   RenderCtx c{m, b, mats, nmat, light, cam, opts};
   int W = cam->width, H = cam->height;
@@ -809,7 +853,7 @@ void orc_render(const orc_mesh *m, const orc_bvh *b, const orc_material *mats, i
     int64_t pix = pixels ? pixels[k] : k;
     int px = (int)(pix % W), py = (int)(pix / W);
     int64_t cnt[3] = {0, 0, 0};
-    render_pixel(c, px, py, rgb, cnt);
+    render_pixel(c, px, py, rgb, rgb64, records, branches, cnt);
     c0 += cnt[0];
     c1 += cnt[1];
     c2 += cnt[2];

@@ -134,6 +134,17 @@ void orc_primary_ray(const orc_camera *cam, const orc_render_opts *opts, int32_t
 void orc_render(const orc_mesh *m, const orc_bvh *b, const orc_material *mats, int32_t nmat,
                 const orc_light *light, const orc_camera *cam, const orc_render_opts *opts,
                 float *rgb, const int64_t *pixels, int64_t npix, int64_t *counts, int32_t nthreads);
+/* orc_render with optional outputs (each may be NULL): rgb64 the same
+ * image in doubles, records per pixel (py*width+px), sample and bounce
+ * (depth+1 slots) a pair (hit triangle or -1, occluder: -1 none, -2 opaque,
+ * else the transmitting occluder's triangle), branches per slot the
+ * shading's branch bits (bit 0: N.L > 0, bit 1: falloff clamped, bit 2:
+ * normal flipped for the bounce, bits 3-4: bounce frame axis); unused
+ * slots hold -1. */
+void orc_render64(const orc_mesh *m, const orc_bvh *b, const orc_material *mats, int32_t nmat,
+                  const orc_light *light, const orc_camera *cam, const orc_render_opts *opts, float *rgb,
+                  double *rgb64, int32_t *records, int32_t *branches, const int64_t *pixels, int64_t npix,
+                  int64_t *counts, int32_t nthreads);
 
 #ifdef __cplusplus
 }
