@@ -11,9 +11,10 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "librtamd.so")
+CLI = os.path.join(HERE, "rtamd_cli")
 
 HIP_SOURCES = ["rt_build.hip", "rt_trace.hip", "rt_api.hip", "rt_batch.hip"]
-CXX_SOURCES = ["rt_scenes.cpp"]
+CXX_SOURCES = ["rt_scenes.cpp", "rt_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("RT_OFFLOAD_ARCH", "gfx950")
@@ -54,6 +55,11 @@ def build(verbose=False):
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-fopenmp", "-o", tmp,
           *[o for _, o in jobs]])
     os.replace(tmp, LIB)
+    # command-line renderer over the C++ facade (host code only)
+    tmp = CLI + ".tmp"
+    _run([HIPCC, "-O2", "-std=c++17", "-o", tmp, os.path.join(CSRC, "rt_cli.cpp"), "-L" + HERE, "-lrtamd",
+          "-Wl,-rpath,$ORIGIN", "-Wno-unused-command-line-argument"])
+    os.replace(tmp, CLI)
     return LIB
 
 

@@ -359,4 +359,15 @@ rt_status rt_diff_backward(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* m
   return RT_OK;
 }
 
+rt_status rt_resolve_supersample(rt_ctx* ctx, const float* big_dev, int32_t width, int32_t height, int32_t ss,
+                                 int32_t flags, float* out_dev, void* stream) {
+  if (!ctx) return RT_ERR_INVALID;
+  if (!big_dev || !out_dev || width <= 0 || height <= 0 || ss <= 0)
+    return rt_set_error(ctx, RT_ERR_INVALID, "bad argument to rt_resolve_supersample");
+  if (flags & ~RT_RESOLVE_REFERENCE_DIVISOR) return rt_set_error(ctx, RT_ERR_INVALID, "unknown resolve flags");
+  RT_HIP(ctx, rt_launch_resolve(big_dev, width, height, ss, (flags & RT_RESOLVE_REFERENCE_DIVISOR) != 0, out_dev,
+                                (hipStream_t)stream));
+  return RT_OK;
+}
+
 }  // extern "C"

@@ -118,6 +118,8 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
                                double* color64 = nullptr);
 // Differentiable render (rt_diff.hip).
 hipError_t rt_launch_zero_f64(double* p, size_t n, hipStream_t s);
+hipError_t rt_launch_resolve(const float* big, int32_t W, int32_t H, int32_t ss, bool reference_divisor, float* out,
+                             hipStream_t s);
 hipError_t rt_launch_fill_i32(int32_t* p, size_t n, int32_t v, hipStream_t s);
 hipError_t rt_launch_fill_albedo(rt_material* mats, int32_t* ids, const double* albedo, int32_t n, hipStream_t s);
 hipError_t rt_launch_adjoint(const rt_bvh* b, const rt_mesh_view* m, const double* albedo, double light_scale,

@@ -1,7 +1,7 @@
 // BVH traversal with ray-triangle intersection and the thin-lens render
 // megakernel for gfx950.
 //
-// Reference path: inc/bbox.h:198-218 (slab test), kernels/bvh.cu:297-449
+// Reference path: inc/bbox.h:82-102 (slab test), kernels/bvh.cu:297-449
 // (Moller-Trumbore, closest-hit and any-hit DFS), kernels/raytracer.cu:94-249
 // (thin-lens ray generation, traceRay), kernels/material.cu:6-40 and
 // kernels/Light.cu (shading, shadow ray), kernels/mrand.cu (RNG).
@@ -47,7 +47,7 @@ __device__ __forceinline__ RayD make_ray(d3 o, d3 d) {
   return r;
 }
 
-// inc/bbox.h:198-218 on fp32 bounds widened exactly to double.
+// inc/bbox.h:82-102 on fp32 bounds widened exactly to double.
 __device__ __forceinline__ bool slab(const RayD& r, float lx, float ly, float lz, float hx, float hy, float hz,
                                      double& t0, double& t1) {
   double nx = r.sx ? (double)lx : (double)hx, fx = r.sx ? (double)hx : (double)lx;
@@ -1584,5 +1584,37 @@ hipError_t rt_launch_adjoint(const rt_bvh* b, const rt_mesh_view* m, const doubl
   const int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
   k_adjoint<<<tiles, kThreads, 0, s>>>(b->view(), M, m->vx_dev, m->vy_dev, m->vz_dev, P, albedo, light_scale,
                                        (const int2*)records, dcolor, g_albedo, g_light, g_pos);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ supersampling resolve
+namespace {
+
+// One thread per output pixel (kernels/scene.cu:127-155): clamp each of
+// the ss x ss samples, sum, divide.
+__global__ void k_resolve(const float* __restrict__ big, int W, int H, int ss, double div, float* __restrict__ out) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y * blockDim.y + threadIdx.y;
+  if (x >= W || y >= H) return;
+  double acc[3] = {0, 0, 0};
+  for (int i = 0; i < ss; ++i)
+    for (int j = 0; j < ss; ++j) {
+      const float* p = big + 3 * ((size_t)(y * ss + i) * (size_t)(W * ss) + (x * ss + j));
+      for (int c = 0; c < 3; ++c) {
+        double v = p[c];
+        v = v < 0 ? 0.0 : (v > 1 ? 1.0 : v);
+        acc[c] += v;
+      }
+    }
+  float* o = out + 3 * ((size_t)y * W + x);
+  for (int c = 0; c < 3; ++c) o[c] = (float)(ss == 1 ? acc[c] : acc[c] / div);
+}
+
+}  // namespace
+
+hipError_t rt_launch_resolve(const float* big, int32_t W, int32_t H, int32_t ss, bool reference_divisor, float* out,
+                             hipStream_t s) {
+  dim3 blk(16, 16), grd((unsigned)cdiv(W, 16), (unsigned)cdiv(H, 16));
+  const double div = reference_divisor ? (double)ss : (double)ss * (double)ss;
+  k_resolve<<<grd, blk, 0, s>>>(big, W, H, ss, div, out);
   return hipGetLastError();
 }

@@ -419,7 +419,77 @@ def _bind_batch(L):
     L._batch_bound = True
 
 
-EXPORTED += ["rt_diff_record_bytes", "rt_diff_forward", "rt_diff_backward"]
+EXPORTED += ["rt_diff_record_bytes", "rt_diff_forward", "rt_diff_backward", "rt_resolve_supersample"]
+RT_RESOLVE_REFERENCE_DIVISOR = 1
+
+
+def resolve_supersample(ctx, big, width, height, ss, flags=0, out=None, stream=None):
+    """AverageSuperSamplingKernel (kernels/scene.cu:127-155) on a device image."""
+    torch = _torch()
+    L = ctx.L
+    L.rt_resolve_supersample.argtypes = [vp, vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, vp, vp]
+    if out is None:
+        out = torch.empty(width * height * 3, dtype=torch.float32, device="cuda")
+    ctx.check(L.rt_resolve_supersample(ctx.h, big.data_ptr(), width, height, ss, flags, out.data_ptr(),
+                                       _stream_handle(stream)))
+    return out
+EXPORTED += ["rt_obj_load", "rt_obj_counts", "rt_obj_mesh", "rt_obj_positions64", "rt_obj_materials", "rt_obj_free",
+             "rt_write_bmp"]
+
+
+def _bind_io(L):
+    if getattr(L, "_io_bound", False):
+        return
+    L.rt_obj_load.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(vp), ctypes.c_char_p, ctypes.c_size_t]
+    L.rt_obj_counts.argtypes = [vp] + [c_i32] * 6
+    L.rt_obj_mesh.argtypes = [vp] + [c_f] * 6 + [c_i32, c_i32]
+    L.rt_obj_positions64.argtypes = [vp, c_d]
+    L.rt_obj_materials.argtypes = [vp, vp]
+    L.rt_obj_free.argtypes = [vp]
+    L.rt_write_bmp.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.c_int32, c_f]
+    L._io_bound = True
+
+
+def load_obj(path, mtl_basepath=None):
+    """Scene_h::LoadObj (src/scene.cpp:6-58): returns (HostMesh, [Material], info)."""
+    L = lib()
+    _bind_io(L)
+    if mtl_basepath is None:
+        mtl_basepath = os.path.dirname(os.path.abspath(path)) + os.sep
+    h = vp()
+    err = ctypes.create_string_buffer(512)
+    st = L.rt_obj_load(path.encode(), mtl_basepath.encode(), ctypes.byref(h), err, len(err))
+    if st != RT_OK:
+        raise RtError(st, err.value.decode())
+    try:
+        c = [ctypes.c_int32() for _ in range(6)]
+        L.rt_obj_counts(h, *[ctypes.byref(x) for x in c])
+        nv, n, nmat, nshapes, raw_v, raw_vn = (x.value for x in c)
+        arr = [np.empty(nv, np.float32) for _ in range(6)]
+        idx = np.empty(3 * n, np.int32)
+        mat_id = np.empty(n, np.int32)
+        L.rt_obj_mesh(h, *[_p(a, c_f) for a in arr], _p(idx, c_i32), _p(mat_id, c_i32))
+        pos64 = np.empty(3 * nv, np.float64)
+        L.rt_obj_positions64(h, _p(pos64, c_d))
+        mats = (Material * max(nmat, 1))()
+        L.rt_obj_materials(h, mats)
+    finally:
+        L.rt_obj_free(h)
+    hm = HostMesh(*arr, idx, mat_id)
+    info = {"shapes": nshapes, "raw_vertices": raw_v, "raw_normals": raw_vn, "positions64": pos64.reshape(-1, 3)}
+    return hm, [mats[k] for k in range(nmat)], info
+
+
+def write_bmp(path, width, height, rgb):
+    """24-bit BMP of a rendered image (writeBMP, src/bitmap.cpp:100-155)."""
+    L = lib()
+    _bind_io(L)
+    a = np.ascontiguousarray(rgb, np.float32).reshape(-1)
+    if a.size != width * height * 3:
+        raise ValueError("image size mismatch")
+    st = L.rt_write_bmp(path.encode(), width, height, _p(a, c_f))
+    if st != RT_OK:
+        raise RtError(st, f"cannot write {path}")
 EXPORTED += ["rt_batch_create", "rt_batch_rebuild", "rt_batch_render", "rt_batch_set_tuning", "rt_batch_get_tuning",
              "rt_batch_autotune", "rt_batch_diagnostics", "rt_batch_destroy"]
 

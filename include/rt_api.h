@@ -173,6 +173,17 @@ rt_status rt_render(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, co
                     int32_t nmat, const rt_light *light, const rt_camera *cam, const rt_render_opts *opts,
                     float *rgb_dev, uint64_t *counters_dev, void *stream);
 
+/* Supersampling resolve: AverageSuperSamplingKernel (kernels/scene.cu:127-155).
+ * big_dev is a (width*ss) x (height*ss) image as rt_render writes it (3
+ * floats per pixel, row-flipped); out_dev receives width x height. Each
+ * sample is clamped to [0, 1]; the sum of the ss*ss samples is divided by
+ * ss*ss, or by ss as the reference does when RT_RESOLVE_REFERENCE_DIVISOR
+ * is set (the reference's /ss makes images ss times too bright for ss > 1;
+ * both agree at ss = 1, the only value the reference runs). */
+enum { RT_RESOLVE_REFERENCE_DIVISOR = 1 };
+rt_status rt_resolve_supersample(rt_ctx *ctx, const float *big_dev, int32_t width, int32_t height, int32_t ss,
+                                 int32_t flags, float *out_dev, void *stream);
+
 /* ---- batched rendering (no reference counterpart: the reference renders
  * one scene per process, src/main.cpp:316-330). Many scenes, each with its
  * own BVH, camera, light, materials, spp and depth, go through one set of

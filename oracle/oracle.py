@@ -27,7 +27,7 @@ RANDVEC_RTL, CLAMP = 1, 2
 
 class Mesh(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("nv", ctypes.c_int32), ("vx", c_f), ("vy", c_f), ("vz", c_f),
-                ("nx", c_f), ("ny", c_f), ("nz", c_f), ("idx", c_i32), ("mat_id", c_i32)]
+                ("nx", c_f), ("ny", c_f), ("nz", c_f), ("idx", c_i32), ("mat_id", c_i32), ("pos64", c_d)]
 
 
 class Bvh(ctypes.Structure):
@@ -103,8 +103,11 @@ class OMesh:
     def __init__(self, hm):
         self.hm = hm
         self.mat = hm.mat_id
+        p64 = getattr(hm, "pos64", None)
+        self.p64 = None if p64 is None else np.ascontiguousarray(p64, np.float64)
         self.s = Mesh(hm.n, hm.nv, _p(hm.vx, c_f), _p(hm.vy, c_f), _p(hm.vz, c_f), _p(hm.nx, c_f), _p(hm.ny, c_f),
-                      _p(hm.nz, c_f), _p(hm.idx, c_i32), _p(hm.mat_id, c_i32) if hm.mat_id is not None else None)
+                      _p(hm.nz, c_f), _p(hm.idx, c_i32), _p(hm.mat_id, c_i32) if hm.mat_id is not None else None,
+                      _p(self.p64, c_d) if self.p64 is not None else None)
 
 
 def tri_aabb(hm):

@@ -45,7 +45,10 @@ static inline double fmax_f(double a, double b) { return (double)fmaxf((float)a,
 
 static const double kRayEps = 0.00000001;  // inc/common.h:2
 
-static inline V3 vertex(const orc_mesh *m, int32_t i) { return v3(m->vx[i], m->vy[i], m->vz[i]); }
+static inline V3 vertex(const orc_mesh *m, int32_t i) {
+  if (m->pos64) return v3(m->pos64[3 * (size_t)i], m->pos64[3 * (size_t)i + 1], m->pos64[3 * (size_t)i + 2]);
+  return v3(m->vx[i], m->vy[i], m->vz[i]);
+}
 static inline V3 vnormal(const orc_mesh *m, int32_t i) { return v3(m->nx[i], m->ny[i], m->nz[i]); }
 
 static inline int clz32(uint32_t x) { return x ? __builtin_clz(x) : 32; }
@@ -166,7 +169,7 @@ static void topdown(int32_t n, const K *codes, const uint32_t *ids, int32_t *par
   }
 }
 
-// inc/bbox.h:198-218 (Kay-Kajiya slab test, no t interval).
+// inc/bbox.h:82-102 (Kay-Kajiya slab test, no t interval).
 static inline bool box_hit(const float *lo, const float *hi, const double *e, const double *a) {
   double tmin[3], tmax[3];
   for (int i = 0; i < 3; ++i) {

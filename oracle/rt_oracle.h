@@ -25,6 +25,7 @@ typedef struct {
   const float *nx, *ny, *nz;    /* per-vertex shading normals, nv each */
   const int32_t *idx;           /* 3n vertex indices */
   const int32_t *mat_id;        /* n material ids, or NULL for material 0 */
+  const double *pos64;          /* optional: nv positions as doubles (xyz), used instead of vx/vy/vz */
 } orc_mesh;
 
 typedef struct {
@@ -97,7 +98,7 @@ void orc_refit_brute(int32_t n, const int32_t *left, const int32_t *right,
 void orc_build(const orc_mesh *m, int32_t bits, uint64_t *codes, uint32_t *ids, int32_t *parent,
                int32_t *left, int32_t *right, float *lo, float *hi, double *mn, double *mx, int32_t nthreads);
 
-/* A9-A12: traversal (kernels/bvh.cu:297-449, inc/bbox.h:198-218).
+/* A9-A12: traversal (kernels/bvh.cu:297-449, inc/bbox.h:82-102).
  * Rays are AoS xyz doubles. Outputs: hit triangle (-1 = miss), t, alpha,
  * beta, shading normal (3 per ray), node-visit and triangle-test counts. */
 void orc_trace(const orc_mesh *m, const orc_bvh *b, int64_t nrays, const double *orig, const double *dir,
