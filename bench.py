@@ -26,9 +26,11 @@ HBM_PEAK_GBS = 8000.0
 # (the published 78.6 TFLOP/s counts an FMA as two flops).
 FP64_LANE_OPS_PEAK_T = 39.3
 # Algorithmic work per unit (fp32 canonical counts; FP64 evaluation counts twice).
-OPS_PER_NODE_VISIT = 32     # two slab tests per internal node entered
-OPS_PER_TRI_TEST = 42       # Moller-Trumbore
+OPS_PER_BOX_TEST = 16       # one slab test (a binary node entered = 2, a 4-wide node = 4)
+OPS_PER_TRI_TEST = 42       # Moller-Trumbore, evaluated in FP64
 FP64_FACTOR = 2
+# The default traversal layout (4-wide nodes) tests child boxes in fp32.
+BOX_TEST_FACTOR = 1
 # Algorithmic HBM bytes per triangle of a 63-bit build.
 BUILD_BYTES_PER_TRI_63 = 396
 
@@ -294,11 +296,11 @@ def main():
     build_ms = float(np.median([e[0].elapsed_time(e[1]) for e in evs]))
     render_ms = float(np.median([e[1].elapsed_time(e[2]) for e in evs]))
     c = cnt.cpu().numpy() // args.steps
-    primary, shadow, bounce, visits, tests = (int(x) for x in c)
+    primary, shadow, bounce, box_tests, tests = (int(x) for x in c)
     ms_per_step = wall * 1e3 / args.steps
     value = (primary + bounce) / (ms_per_step * 1e-3) / 1e6
 
-    lane_ops = (visits * OPS_PER_NODE_VISIT + tests * OPS_PER_TRI_TEST) * FP64_FACTOR
+    lane_ops = box_tests * OPS_PER_BOX_TEST * BOX_TEST_FACTOR + tests * OPS_PER_TRI_TEST * FP64_FACTOR
     achieved = lane_ops / (render_ms * 1e-3) / 1e12
     build_bytes = BUILD_BYTES_PER_TRI_63 * args.tris
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
@@ -327,8 +329,9 @@ def main():
             "unit": "T FP64 lane-op/s",
             "frac": round(achieved / FP64_LANE_OPS_PEAK_T, 4),
             "traffic": None,
-            "work": {"node_visits": visits, "tri_tests": tests, "lane_ops": lane_ops,
-                     "ops_per_visit": OPS_PER_NODE_VISIT * FP64_FACTOR, "ops_per_test": OPS_PER_TRI_TEST * FP64_FACTOR},
+            "work": {"box_tests": box_tests, "tri_tests": tests, "lane_ops": lane_ops,
+                     "ops_per_box_test": OPS_PER_BOX_TEST * BOX_TEST_FACTOR,
+                     "ops_per_tri_test": OPS_PER_TRI_TEST * FP64_FACTOR},
         },
         "roofline_build": {
             "kernels": "k_init..k_climb_top (one rebuild)",

@@ -157,6 +157,7 @@ rt_status rt_batch_get_tuning(const rt_batch* b, rt_tuning* t) {
 rt_status rt_batch_autotune(rt_batch* b, void* stream, rt_tuning* best, float* best_ms) {
   if (!b) return RT_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
+  const int layouts[] = {2, 4};
   const int rings[] = {16, 32};
   const int chunks[] = {128, 256};
   const int waves[] = {3, 4, 5, 8};
@@ -166,10 +167,11 @@ rt_status rt_batch_autotune(rt_batch* b, void* stream, rt_tuning* best, float* b
   rt_tuning keep = b->tuning, win = b->tuning;
   float win_ms = -1.f;
   rt_status st = RT_OK;
+  for (int l : layouts)
   for (int r : rings)
     for (int c : chunks)
       for (int w : waves) {
-        rt_tuning t{r, c, w, 2};
+        rt_tuning t{r, c, w, l};
         b->tuning = t;
         if ((st = rt_batch_render(b, nullptr, stream)) != RT_OK) goto done;  // warm-up
         if (hipEventRecord(e0, s) != hipSuccess || (st = rt_batch_render(b, nullptr, stream)) != RT_OK ||

@@ -723,7 +723,76 @@ __global__ __launch_bounds__(256) void k_export(int n, const K* __restrict__ key
     for (int k = 0; k < 3; ++k) hi_out[3 * i + k] = node_box[6 * (size_t)i + 3 + k];
 }
 
+// ---- 4-wide traversal nodes
+// Depth parity of each internal node (walk to the root through the parent
+// links): the even-depth internal nodes become 4-wide nodes.
+__global__ __launch_bounds__(256) void k_wide_parity(int n, const int32_t* __restrict__ parent,
+                                                     uint8_t* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  int x = i, d = 0;
+  for (int guard = 0; x != 0 && guard < n; ++guard) {
+    x = parent[x];
+    ++d;
+  }
+  flag[i] = (d & 1) == 0;
+}
+
+// Fill the 4-wide node of every even-depth internal node from the binary
+// topology and the refit boxes (node numbering of the reference: internal
+// [0, n-2], leaf of sorted position p at n-1+p).
+__global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restrict__ left,
+                                                   const int32_t* __restrict__ right,
+                                                   const float* __restrict__ node_box,
+                                                   const uint8_t* __restrict__ flag, float4* __restrict__ wide) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1 || !flag[i]) return;
+  int slot[4];
+  int k = 0;
+  const int c[2] = {left[i], right[i]};
+  for (int j = 0; j < 2; ++j) {
+    if (c[j] >= n - 1) {
+      slot[k++] = c[j];
+    } else {
+      slot[k++] = left[c[j]];
+      slot[k++] = right[c[j]];
+    }
+  }
+  float lo[3][4], hi[3][4];
+  int ref[4];
+  for (int j = 0; j < 4; ++j) {
+    if (j < k) {
+      const float* b = node_box + 6 * (size_t)slot[j];
+      for (int a = 0; a < 3; ++a) {
+        lo[a][j] = b[a];
+        hi[a][j] = b[3 + a];
+      }
+      ref[j] = slot[j] >= n - 1 ? ~(slot[j] - (n - 1)) : slot[j];
+    } else {
+      for (int a = 0; a < 3; ++a) {
+        lo[a][j] = __int_as_float(0x7f800000);   // +inf
+        hi[a][j] = __int_as_float(0xff800000);   // -inf
+      }
+      ref[j] = kWideEmpty;
+    }
+  }
+  float4* w = wide + 8 * (size_t)i;
+  for (int a = 0; a < 3; ++a) {
+    w[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+    w[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+  }
+  w[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
+  w[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
+  const int n = b->n;
+  if (n < 2) return;
+  if (topology_changed) k_wide_parity<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->parent, b->wide_flag);
+  k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide);
+}
 
 template <typename K>
 hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
@@ -758,6 +827,7 @@ hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
       k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
                                                     b->deposits, b->arrivals, b->deferred, b->misc);
   }
+  launch_wide(b, true, s);
   return hipGetLastError();
 }
 
@@ -769,6 +839,7 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
                                                   b->tri_box, b->node_box, b->ltri, b->flags);
     if (n > 1)
       k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
+    launch_wide(b, false, s);
     return hipGetLastError();
   }
   // The keys are unchanged, so the climb re-derives the same topology and
@@ -784,6 +855,7 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
   if (n > 1)
     k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
                                                   b->deposits, b->arrivals, b->deferred, b->misc);
+  launch_wide(b, false, s);
   return hipGetLastError();
 }
 

@@ -25,7 +25,20 @@ struct BvhView {
   const float* node_box;   // (2n-1)*6: lo xyz, hi xyz
   const float4* tnode;     // (n-1)*4: child boxes and child refs (64 B per internal node)
   const float4* ltri;      // n*3: vertices a, b, c of the leaf's triangle + triangle id (48 B)
+  const float4* wide;      // (n-1)*8: 4-wide nodes (128 B), valid at even-depth internal nodes
 };
+
+// 4-wide traversal node (128 B, eight float4), stored at the index of the
+// binary internal node it replaces (the internal nodes at even depth). Its
+// children are the binary node's children, each internal one replaced by
+// its own two children, in sorted-leaf order; their fp32 boxes are the
+// exact refit boxes:
+//   [0] lo.x of children 0..3   [1] lo.y   [2] lo.z
+//   [3] hi.x of children 0..3   [4] hi.y   [5] hi.z
+//   [6] child refs (int bits): >= 0 wide node, < 0 ~(sorted leaf position),
+//       kWideEmpty for an unused slot (its box is empty: lo = +inf, hi = -inf)
+//   [7] unused
+constexpr int32_t kWideEmpty = 0x7fffffff;
 
 // Traversal record of internal node i (64 B, four float4):
 //   [0] = left.lo.xyz,  left.hi.x
@@ -62,8 +75,10 @@ struct rt_bvh {
   float* node_box = nullptr;     // (2n-1)*6
   float4* tnode = nullptr;       // (n-1)*4
   float4* ltri = nullptr;        // n*3
+  float4* wide = nullptr;        // (n-1)*8
+  uint8_t* wide_flag = nullptr;  // n-1: 1 where the internal node is at even depth
 
-  BvhView view() const { return BvhView{n, node_box, tnode, ltri}; }
+  BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide}; }
 };
 
 // misc[] slots

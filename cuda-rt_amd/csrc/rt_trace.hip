@@ -28,6 +28,22 @@ using rt::kRayEps;
 using rt::mk;
 
 
+// Loads through global-address-space pointers: the BVH arrays are reached
+// through pointers stored in memory, which the compiler would otherwise
+// load with flat instructions (whose completion is counted together with
+// the LDS traversal stack's accesses).
+typedef float f4v_t __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 ldg4(const float4* p) {
+  const f4v_t v = *(const __attribute__((address_space(1))) f4v_t*)(p);
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float ldgf(const float* p) {
+  return *(const __attribute__((address_space(1))) float*)(p);
+}
+__device__ __forceinline__ double ldgd(const double* p) {
+  return *(const __attribute__((address_space(1))) double*)(p);
+}
+
 struct RayD {
   d3 o, d;
   double ax, ay, az;
@@ -64,7 +80,7 @@ __device__ __forceinline__ bool slab(const RayD& r, float lx, float ly, float lz
 // kernels/bvh.cu:297-347 on the leaf record's vertices.
 __device__ __forceinline__ bool tri_test(const RayD& r, const float4* __restrict__ L, double& t, double& al,
                                          double& be) {
-  float4 q0 = L[0], q1 = L[1], q2 = L[2];
+  float4 q0 = ldg4(L), q1 = ldg4(L + 1), q2 = ldg4(L + 2);
   d3 a = mk(q0.x, q0.y, q0.z), b = mk(q0.w, q1.x, q1.y), c = mk(q1.z, q1.w, q2.x);
   d3 AB = b - a, AC = c - a;
   d3 P = rt::cross(r.d, AC);
@@ -87,8 +103,8 @@ struct HitRec {
   int32_t pos;  // sorted leaf position, -1 = none
 };
 
-// Work counters: internal nodes entered (two child-box tests each) and
-// triangle tests.
+// Work counters: child-box slab tests (two per binary node entered, four
+// per 4-wide node) and triangle tests.
 struct TravStats {
   uint32_t visits = 0, tests = 0;
 };
@@ -98,10 +114,10 @@ __device__ __forceinline__ double prune_margin(double t) { return 1e-4 * fabs(t)
 __device__ __forceinline__ void load_node(const float4* __restrict__ tn, int node, float4& a, float4& b, float4& c,
                                           float4& d) {
   const float4* p = tn + 4 * (size_t)node;
-  a = p[0];
-  b = p[1];
-  c = p[2];
-  d = p[3];
+  a = ldg4(p);
+  b = ldg4(p + 1);
+  c = ldg4(p + 2);
+  d = ldg4(p + 3);
 }
 
 __device__ __forceinline__ void consider_leaf(const BvhView& B, const RayD& r, int pos, double tmin_phys,
@@ -180,7 +196,7 @@ __device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, H
   for (int guard = 0; guard < n; ++guard) {
     float4 q0, q1, q2, q3;
     load_node(B.tnode, node, q0, q1, q2, q3);
-    ++st.visits;
+    st.visits += 2;
     double l0, l1, r0, r1;
     bool hl = slab(r, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, l0, l1);
     bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
@@ -249,7 +265,7 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
   for (int guard = 0; guard < n; ++guard) {
     float4 q0, q1, q2, q3;
     load_node(B.tnode, node, q0, q1, q2, q3);
-    ++st.visits;
+    st.visits += 2;
     double l0, l1, r0, r1;
     bool hl = slab(r, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, l0, l1);
     bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
@@ -309,7 +325,7 @@ __device__ __forceinline__ int32_t leaf_tri(const BvhView& B, int pos) {
 }
 
 __device__ __forceinline__ int32_t leaf_tri_of(const BvhView& B, int pos) {
-  return __float_as_int(B.ltri[3 * (size_t)pos + 2].y);
+  return __float_as_int(ldgf((const float*)(B.ltri + 3 * (size_t)pos + 2) + 1));
 }
 
 __device__ __forceinline__ void flush_stats(unsigned long long* counters, const TravStats& st) {
@@ -324,6 +340,82 @@ __device__ __forceinline__ void flush_stats(unsigned long long* counters, const 
     if (vis) atomicAdd(&counters[3], vis);
     if (tst) atomicAdd(&counters[4], tst);
   }
+}
+
+// ------------------------------------------------------------ 4-wide nodes
+// The reference tests a leaf's triangle when the FP64 slab test passes on
+// the leaf's box and on every ancestor's box. Refit boxes nest exactly and
+// the rounded FP64 test is monotone in the box (if a box passes, every box
+// containing it passes), so that set is exactly the leaves whose own box
+// passes. Internal children may therefore use any test that never rejects
+// a box the FP64 test accepts: the 4-wide nodes use an fp32 slab test whose
+// distances are widened by E = 4 eps (B + |o|) |1/d| (eps = 2^-24, B the
+// largest coordinate magnitude of the scene box), which covers the rounding
+// of 1/d, of o/d and of the fused multiply-add; a leaf child is decided by
+// the FP64 test on its exact fp32 box, as in the binary traversal.
+struct RayF {
+  float ix, iy, iz;  // 1/d (+-1e30 for a zero component)
+  float nx, ny, nz;  // o/d + E: near-plane offsets
+  float fx, fy, fz;  // o/d - E: far-plane offsets
+};
+
+__device__ __forceinline__ void rayf_axis(double o, double d, double bmax, float& iv, float& nn, float& ff) {
+  double inv = 1.0 / d;
+  if (!(fabs(inv) <= 1e30)) inv = copysign(1e30, d);
+  const double oi = o * inv;
+  const double E = 4.0 * 5.9604644775390625e-08 * (bmax + fabs(o)) * fabs(inv);
+  iv = (float)inv;
+  nn = (float)(oi + E);
+  ff = (float)(oi - E);
+}
+
+__device__ __forceinline__ RayF make_rayf(const RayD& r, const float* root_box) {
+  double bmax = 0;
+#pragma unroll
+  for (int k = 0; k < 6; ++k) bmax = fmax(bmax, fabs((double)ldgf(root_box + k)));
+  RayF f;
+  rayf_axis(r.o.x, r.d.x, bmax, f.ix, f.nx, f.fx);
+  rayf_axis(r.o.y, r.d.y, bmax, f.iy, f.ny, f.fy);
+  rayf_axis(r.o.z, r.d.z, bmax, f.iz, f.nz, f.fz);
+  return f;
+}
+
+__device__ __forceinline__ float f4c(const float4& v, int j) { return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w; }
+
+__device__ __forceinline__ void load_wide(const float4* __restrict__ wide, int node, float4 (&w)[7]) {
+  const float4* p = wide + 8 * (size_t)node;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) w[k] = ldg4(p + k);
+}
+
+// Conservative fp32 entry/exit distances of the four children.
+__device__ __forceinline__ void wide_slabs(const RayD& r, const RayF& f, const float4 (&w)[7], float (&t0)[4],
+                                           float (&t1)[4]) {
+  const float4 nx = r.sx ? w[0] : w[3], fx = r.sx ? w[3] : w[0];
+  const float4 ny = r.sy ? w[1] : w[4], fy = r.sy ? w[4] : w[1];
+  const float4 nz = r.sz ? w[2] : w[5], fz = r.sz ? w[5] : w[2];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const float a = fmaf(f4c(nx, j), f.ix, -f.nx), b = fmaf(f4c(ny, j), f.iy, -f.ny), c = fmaf(f4c(nz, j), f.iz, -f.nz);
+    const float d = fmaf(f4c(fx, j), f.ix, -f.fx), e = fmaf(f4c(fy, j), f.iy, -f.fy), g = fmaf(f4c(fz, j), f.iz, -f.fz);
+    t0[j] = fmaxf(fmaxf(a, b), c);
+    t1[j] = fminf(fminf(d, e), g);
+  }
+}
+
+// The FP64 slab test of child j's exact box.
+__device__ __forceinline__ bool wide_leaf_slab(const RayD& r, const float4 (&w)[7], int j, double& t0, double& t1) {
+  return slab(r, f4c(w[0], j), f4c(w[1], j), f4c(w[2], j), f4c(w[3], j), f4c(w[4], j), f4c(w[5], j), t0, t1);
+}
+
+// Float bounds of a double, rounded outward.
+__device__ __forceinline__ float f_up(double x) {
+  float f = (float)x;
+  return (double)f < x ? __int_as_float(__float_as_int(f) + (f >= 0 ? 1 : -1)) : f;
+}
+__device__ __forceinline__ float f_down(double x) {
+  float f = (float)x;
+  return (double)f > x ? __int_as_float(__float_as_int(f) + (f > 0 ? -1 : 1)) : f;
 }
 
 // ------------------------------------------------------------ persistent traversal
@@ -369,7 +461,7 @@ __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uin
 }
 
 // views[scene[i]] is the BVH of ray i (scene == nullptr: views[0] for all).
-template <bool ANY, int RING>
+template <bool ANY, int RING, bool WIDE>
 __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __restrict__ views,
                                                                const uint16_t* __restrict__ scene, RayIn R,
                                                                double tmin_phys, HitOut O, uint32_t* fetch_ctr,
@@ -387,6 +479,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
   S.col = s_stack + threadIdx.x;
   int spill[kSpill];
   RayD r;
+  RayF rf;
   bool active = false;
   uint32_t rid = 0;
   int node = 0;
@@ -444,12 +537,13 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
         }
         const double* po = R.orig + (size_t)rid * R.stride;
         const double* pd = R.dir + (size_t)rid * R.stride;
-        r = make_ray(mk(po[0], po[1], po[2]), mk(pd[0], pd[1], pd[2]));
+        r = make_ray(mk(ldgd(po), ldgd(po + 1), ldgd(po + 2)), mk(ldgd(pd), ldgd(pd + 1), ldgd(pd + 2)));
+        if (WIDE) rf = make_rayf(r, B.node_box);
         bpos = -1;
         bt = 0;
         double t0, t1;
         const float* rb = B.node_box;
-        if (n <= 0 || !slab(r, rb[0], rb[1], rb[2], rb[3], rb[4], rb[5], t0, t1)) {
+        if (n <= 0 || !slab(r, ldgf(rb), ldgf(rb + 1), ldgf(rb + 2), ldgf(rb + 3), ldgf(rb + 4), ldgf(rb + 5), t0, t1)) {
           write_hit(B, O, rid, -1, 0, 0, 0);
         } else if (n == 1) {
           double t, al, be;
@@ -472,10 +566,127 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
     if (!active) continue;
     // ---- one traversal step
     bool done = false;
-    if (!ANY) {
+    if (WIDE && node < 0) {
+      // Leaf whose conservative test passed: the FP64 slab test on its
+      // exact box (the min/max of its vertices), then the triangle.
+      const int pos = ~node;
+      const float4* L = B.ltri + 3 * (size_t)pos;
+      // leaf record: q0 = a.xyz, b.x; q1 = b.yz, c.xy; q2 = c.z, id. The box
+      // is formed exactly as the build forms the triangle box.
+      const float4 q0 = ldg4(L), q1 = ldg4(L + 1), q2 = ldg4(L + 2);
+      const float ax = q0.x, ay = q0.y, az = q0.z, bx = q0.w, by = q1.x, bz = q1.y, cx = q1.z, cy = q1.w, cz = q2.x;
+      double l0 = 0, l1 = 0;
+      // any-hit leaves were decided when their node was entered
+      bool ok = ANY || slab(r, fminf(cx, fminf(ax, bx)), fminf(cy, fminf(ay, by)), fminf(cz, fminf(az, bz)),
+                            fmaxf(cx, fmaxf(ax, bx)), fmaxf(cy, fmaxf(ay, by)), fmaxf(cz, fmaxf(az, bz)), l0, l1);
+      if (ANY) {
+        if (ok) {
+          double t, al, be;
+          ++st.tests;
+          if (tri_test(r, L, t, al, be)) {
+            bt = t; bal = al; bbe = be; bpos = pos;
+            done = true;
+          }
+        }
+      } else {
+        if (ok && tmin_phys > 0) ok = !(l1 < tmin_phys - prune_margin(tmin_phys));
+        if (ok && bpos >= 0) ok = !(l0 > bt + prune_margin(bt));
+        if (ok) {
+          HitRec best{bt, bal, bbe, bpos};
+          consider_leaf(B, r, pos, tmin_phys, best, st);
+          bt = best.t;
+          bal = best.al;
+          bbe = best.be;
+          bpos = best.pos;
+        }
+      }
+      if (!done) {
+        if (S.sp == 0)
+          done = true;
+        else
+          node = stack_pop<RING>(S, spill);
+      }
+    } else if (WIDE && !ANY) {
+      float4 w[7];
+      load_wide(B.wide, node, w);
+      st.visits += 4;
+      float t0[4], t1[4];
+      wide_slabs(r, rf, w, t0, t1);
+      const float lim = bpos >= 0 ? f_up(bt + prune_margin(bt)) : __int_as_float(0x7f800000);
+      const float low = tmin_phys > 0 ? f_down(tmin_phys - prune_margin(tmin_phys)) : __int_as_float(0xff800000);
+      float key[4];
+      int ref[4];
+      int m = 0;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        ref[j] = __float_as_int(f4c(w[6], j));
+        const bool h = ref[j] != kWideEmpty && t0[j] <= t1[j] && t0[j] <= lim && t1[j] >= low;
+        key[j] = h ? t0[j] : __int_as_float(0x7f800000);
+        m += h ? 1 : 0;
+      }
+      // nearest child next, the others pushed far to near (leaves too:
+      // they are decided when they come off the stack)
+#define RT_CSWAP(a, b)                                  \
+  if (key[b] < key[a]) {                                \
+    float tk = key[a]; key[a] = key[b]; key[b] = tk;    \
+    int tr = ref[a]; ref[a] = ref[b]; ref[b] = tr;      \
+  }
+      RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+#undef RT_CSWAP
+      bool ovf = false;
+      if (m >= 4) ovf = ovf || !stack_push<RING>(S, spill, ref[3]);
+      if (m >= 3) ovf = ovf || !stack_push<RING>(S, spill, ref[2]);
+      if (m >= 2) ovf = ovf || !stack_push<RING>(S, spill, ref[1]);
+      if (ovf) {
+        atomicAdd(overflow, 1u);
+        bpos = -1;
+        done = true;
+      } else if (m >= 1) {
+        node = ref[0];
+      } else if (S.sp == 0) {
+        done = true;
+      } else {
+        node = stack_pop<RING>(S, spill);
+      }
+    } else if (WIDE && ANY) {
+      float4 w[7];
+      load_wide(B.wide, node, w);
+      st.visits += 4;
+      float t0[4], t1[4];
+      wide_slabs(r, rf, w, t0, t1);
+      // children in sorted-leaf order: the first passing one next, the
+      // others pushed so that they pop in order (left-first DFS)
+      int next = 0;
+      bool have = false, ovf = false;
+#pragma unroll
+      for (int j = 3; j >= 0; --j) {
+        const int ref = __float_as_int(f4c(w[6], j));
+        bool p = ref != kWideEmpty && t0[j] <= t1[j];
+        if (p && ref < 0) {
+          double l0, l1;
+          p = wide_leaf_slab(r, w, j, l0, l1);  // exact decision for a leaf child
+        }
+        if (p) {
+          if (have) ovf = ovf || !stack_push<RING>(S, spill, next);
+          next = ref;
+          have = true;
+        }
+      }
+      if (ovf) {
+        atomicAdd(overflow, 1u);
+        bpos = -1;
+        done = true;
+      } else if (have) {
+        node = next;
+      } else if (S.sp == 0) {
+        done = true;
+      } else {
+        node = stack_pop<RING>(S, spill);
+      }
+    } else if (!ANY) {
       float4 q0, q1, q2, q3;
       load_node(B.tnode, node, q0, q1, q2, q3);
-      ++st.visits;
+      st.visits += 2;
       double l0, l1, r0, r1;
       bool hl = slab(r, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, l0, l1);
       bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
@@ -535,7 +746,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
       } else {
         float4 q0, q1, q2, q3;
         load_node(B.tnode, node, q0, q1, q2, q3);
-        ++st.visits;
+        st.visits += 2;
         double l0, l1, r0, r1;
         bool hl = slab(r, q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, l0, l1);
         bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
@@ -1156,14 +1367,14 @@ rt_tuning rt_default_tuning() {
   t.stack_ring = kRing;
   t.fetch_chunk = kDefaultFetchChunk;
   t.waves_per_simd = 5;
-  t.node_layout = 2;
+  t.node_layout = 4;
   return t;
 }
 
 bool rt_tuning_valid(const rt_tuning* t) {
   return t && (t->stack_ring == 16 || t->stack_ring == 32) &&
          (t->fetch_chunk == 64 || t->fetch_chunk == 128 || t->fetch_chunk == 256 || t->fetch_chunk == 512) &&
-         t->waves_per_simd >= 1 && t->waves_per_simd <= 8 && t->node_layout == 2;
+         t->waves_per_simd >= 1 && t->waves_per_simd <= 8 && (t->node_layout == 2 || t->node_layout == 4);
 }
 
 static int persistent_grid(int num_cus, const rt_tuning& t, uint32_t max_rays) {
@@ -1176,12 +1387,22 @@ template <bool ANY>
 static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const BvhView* views,
                               const uint16_t* scene, RayIn R, double tmin, HitOut O, uint32_t* fetch,
                               uint32_t* overflow, unsigned long long* counters) {
-  if (t.stack_ring == 16)
-    k_trace_persistent<ANY, 16><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, t.fetch_chunk, overflow,
-                                                          counters);
-  else
-    k_trace_persistent<ANY, 32><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, t.fetch_chunk, overflow,
-                                                          counters);
+  const uint32_t fc = t.fetch_chunk;
+  if (t.node_layout == 4) {
+    if (t.stack_ring == 16)
+      k_trace_persistent<ANY, 16, true><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow,
+                                                                  counters);
+    else
+      k_trace_persistent<ANY, 32, true><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow,
+                                                                  counters);
+  } else {
+    if (t.stack_ring == 16)
+      k_trace_persistent<ANY, 16, false><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow,
+                                                                   counters);
+    else
+      k_trace_persistent<ANY, 32, false><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow,
+                                                                   counters);
+  }
 }
 
 hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays* rays, int32_t mode, double tmin,

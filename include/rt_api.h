@@ -166,7 +166,8 @@ rt_status rt_trace(const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_rays *r
  * rgb_dev receives width*height*3 floats, row-flipped like the reference
  * image (kernels/raytracer.cu:106). counters_dev, if not NULL, receives
  * 5 uint64 counts added atomically: primary, shadow and bounce rays,
- * internal nodes entered (two child-box tests each) and triangle tests.
+ * child-box slab tests (two per binary node entered, four per 4-wide node)
+ * and triangle tests.
  * RT_RENDER_WAVEFRONT keeps per-pixel state in a context-owned workspace
  * that the first such call for a given image size allocates. */
 rt_status rt_render(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_material *mats_dev,
@@ -193,7 +194,8 @@ typedef struct {
   int32_t stack_ring;      /* LDS traversal stack entries per lane: 16 or 32 */
   int32_t fetch_chunk;     /* rays a wave reserves per atomic: 64, 128, 256 or 512 */
   int32_t waves_per_simd;  /* persistent traversal workgroups per CU, 1..8 */
-  int32_t node_layout;     /* 2: binary nodes, 64 B per internal node */
+  int32_t node_layout;     /* 2: binary nodes, FP64 child tests, 64 B per node;
+                              4: 4-wide nodes, fp32 conservative child tests, 128 B per node (default) */
 } rt_tuning;
 
 typedef struct {

@@ -1,5 +1,7 @@
 """Batched rendering (rt_batch_*): every scene's image equals the single-scene
 render bit for bit, for every tuning, eagerly and replayed from a hipGraph."""
+import itertools
+
 import numpy as np
 import pytest
 
@@ -67,21 +69,19 @@ def test_batch_outputs_do_not_depend_on_tuning(ctx):
     sc = _make(ctx, specs)
     batch = rtamd.Batch(ctx, sc)
     ref = None
-    for ring in (16, 32):
-        for chunk in (64, 256):
-            for waves in (1, 5, 8):
-                t = rtamd.Tuning(ring, chunk, waves, 2)
-                batch.set_tuning(t)
-                for s in sc:
-                    s["rgb"].zero_()
-                batch.render()
-                torch.cuda.synchronize()
-                imgs = [s["rgb"].cpu().numpy() for s in sc]
-                if ref is None:
-                    ref = imgs
-                else:
-                    for a, b in zip(ref, imgs):
-                        assert np.array_equal(a, b), t.as_dict()
+    for layout, ring, chunk, waves in itertools.product((2, 4), (16, 32), (64, 256), (1, 5, 8)):
+        t = rtamd.Tuning(ring, chunk, waves, layout)
+        batch.set_tuning(t)
+        for s in sc:
+            s["rgb"].zero_()
+        batch.render()
+        torch.cuda.synchronize()
+        imgs = [s["rgb"].cpu().numpy() for s in sc]
+        if ref is None:
+            ref = imgs
+        else:
+            for a, b in zip(ref, imgs):
+                assert np.array_equal(a, b), t.as_dict()
     best, ms = batch.autotune()
     assert ms > 0 and best.stack_ring in (16, 32)
     batch.render()
