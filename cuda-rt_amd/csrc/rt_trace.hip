@@ -427,6 +427,15 @@ __device__ __forceinline__ float f_down(double x) {
 // closest_hit()/any_hit() above, one node (or, for any-hit, one popped leaf)
 // per step.
 constexpr uint32_t kDefaultFetchChunk = 256;
+#ifndef RT_REFILL_MIN
+#define RT_REFILL_MIN 16
+#endif
+constexpr int kRefillMin = RT_REFILL_MIN;  // idle lanes that trigger a refill (1..64)
+#ifndef RT_LEAF_BATCH
+#define RT_LEAF_BATCH 16
+#endif
+constexpr int kLeafBatch = RT_LEAF_BATCH;  // waiting leaves that trigger a batch of leaf tests
+constexpr int kNoNode = 0x7fffffff;        // no node to visit (never a node index)
 constexpr uint32_t kSegments = 8;       // one per XCD
 constexpr uint32_t kFetchStride = 32;   // counters 128 bytes apart
 constexpr uint32_t kFetchWords = kSegments * kFetchStride;
@@ -483,6 +492,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
   bool active = false;
   uint32_t rid = 0;
   int node = 0;
+  int pending = kNoNode;  // 4-wide traversal: a leaf waiting for the wave's next batch of leaf tests
   double bt = 0, bal = 0, bbe = 0;
   int bpos = -1;
   uint32_t chunk_next = 0, chunk_end = 0;
@@ -492,8 +502,11 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
   uint32_t seg = blockIdx.x % kSegments, tries = 0;
   while (true) {
     // ---- refill idle lanes from the wave's chunk
+    // Refill only once enough lanes are idle: the ray setup (three FP64
+    // divisions, the root-box test) runs for the whole wave when any lane
+    // takes a ray, so taking rays in groups keeps it off most iterations.
     uint64_t need = __ballot(!active);
-    if (need != 0 && !drained) {
+    if (need != 0 && !drained && __popcll(need) >= kRefillMin) {
       uint32_t k = (uint32_t)__popcll(need);
       uint32_t rank = (uint32_t)__popcll(need & mlt);
       uint32_t avail = chunk_end - chunk_next;
@@ -554,6 +567,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
         } else {
           active = true;
           node = 0;
+          pending = kNoNode;
           S.sp = 0;
           S.lo = 0;
         }
@@ -566,123 +580,117 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
     if (!active) continue;
     // ---- one traversal step
     bool done = false;
-    if (WIDE && node < 0) {
-      // Leaf whose conservative test passed: the FP64 slab test on its
-      // exact box (the min/max of its vertices), then the triangle.
-      const int pos = ~node;
-      const float4* L = B.ltri + 3 * (size_t)pos;
-      // leaf record: q0 = a.xyz, b.x; q1 = b.yz, c.xy; q2 = c.z, id. The box
-      // is formed exactly as the build forms the triangle box.
-      const float4 q0 = ldg4(L), q1 = ldg4(L + 1), q2 = ldg4(L + 2);
-      const float ax = q0.x, ay = q0.y, az = q0.z, bx = q0.w, by = q1.x, bz = q1.y, cx = q1.z, cy = q1.w, cz = q2.x;
-      double l0 = 0, l1 = 0;
-      // any-hit leaves were decided when their node was entered
-      bool ok = ANY || slab(r, fminf(cx, fminf(ax, bx)), fminf(cy, fminf(ay, by)), fminf(cz, fminf(az, bz)),
-                            fmaxf(cx, fmaxf(ax, bx)), fmaxf(cy, fmaxf(ay, by)), fmaxf(cz, fmaxf(az, bz)), l0, l1);
-      if (ANY) {
-        if (ok) {
-          double t, al, be;
-          ++st.tests;
-          if (tri_test(r, L, t, al, be)) {
-            bt = t; bal = al; bbe = be; bpos = pos;
-            done = true;
+    if (WIDE) {
+      // Leaf tests are batched across the wave: a lane that reaches a leaf
+      // parks it in `pending` and keeps walking its stack; the wave tests
+      // the parked leaves together once kLeafBatch lanes have one, or as
+      // soon as a lane cannot go on (a second leaf, or nothing left to
+      // walk). Leaves are still tested in the order the lane reaches them,
+      // so any-hit keeps its left-first order.
+      const bool has_pending = pending != kNoNode;
+      const bool blocked = has_pending && (node == kNoNode || node < 0);
+      const uint64_t wp = __ballot(has_pending);
+      if (__ballot(blocked) != 0 || __popcll(wp) >= kLeafBatch) {
+        if (has_pending) {
+          const int pos = ~pending;
+          pending = kNoNode;
+          const float4* L = B.ltri + 3 * (size_t)pos;
+          if (ANY) {
+            // the leaf's box was decided exactly when its node was entered
+            double t, al, be;
+            ++st.tests;
+            if (tri_test(r, L, t, al, be)) {
+              bt = t; bal = al; bbe = be; bpos = pos;
+              done = true;
+            }
+          } else {
+            // leaf record: q0 = a.xyz, b.x; q1 = b.yz, c.xy; q2 = c.z, id. The
+            // box is formed exactly as the build forms the triangle box.
+            const float4 q0 = ldg4(L), q1 = ldg4(L + 1), q2 = ldg4(L + 2);
+            const float ax = q0.x, ay = q0.y, az = q0.z, bx = q0.w, by = q1.x, bz = q1.y, cx = q1.z, cy = q1.w,
+                        cz = q2.x;
+            double l0, l1;
+            bool ok = slab(r, fminf(cx, fminf(ax, bx)), fminf(cy, fminf(ay, by)), fminf(cz, fminf(az, bz)),
+                           fmaxf(cx, fmaxf(ax, bx)), fmaxf(cy, fmaxf(ay, by)), fmaxf(cz, fmaxf(az, bz)), l0, l1);
+            if (ok && tmin_phys > 0) ok = !(l1 < tmin_phys - prune_margin(tmin_phys));
+            if (ok && bpos >= 0) ok = !(l0 > bt + prune_margin(bt));
+            if (ok) {
+              HitRec best{bt, bal, bbe, bpos};
+              consider_leaf(B, r, pos, tmin_phys, best, st);
+              bt = best.t;
+              bal = best.al;
+              bbe = best.be;
+              bpos = best.pos;
+            }
           }
         }
-      } else {
-        if (ok && tmin_phys > 0) ok = !(l1 < tmin_phys - prune_margin(tmin_phys));
-        if (ok && bpos >= 0) ok = !(l0 > bt + prune_margin(bt));
-        if (ok) {
-          HitRec best{bt, bal, bbe, bpos};
-          consider_leaf(B, r, pos, tmin_phys, best, st);
-          bt = best.t;
-          bal = best.al;
-          bbe = best.be;
-          bpos = best.pos;
-        }
       }
-      if (!done) {
-        if (S.sp == 0)
-          done = true;
-        else
-          node = stack_pop<RING>(S, spill);
+      if (!done && node != kNoNode && node < 0 && pending == kNoNode) {
+        pending = node;
+        node = S.sp == 0 ? kNoNode : stack_pop<RING>(S, spill);
       }
-    } else if (WIDE && !ANY) {
-      float4 w[7];
-      load_wide(B.wide, node, w);
-      st.visits += 4;
-      float t0[4], t1[4];
-      wide_slabs(r, rf, w, t0, t1);
-      const float lim = bpos >= 0 ? f_up(bt + prune_margin(bt)) : __int_as_float(0x7f800000);
-      const float low = tmin_phys > 0 ? f_down(tmin_phys - prune_margin(tmin_phys)) : __int_as_float(0xff800000);
-      float key[4];
-      int ref[4];
-      int m = 0;
+      if (!done && node != kNoNode && node >= 0) {
+        float4 w[7];
+        load_wide(B.wide, node, w);
+        st.visits += 4;
+        float t0[4], t1[4];
+        wide_slabs(r, rf, w, t0, t1);
+        bool ovf = false;
+        int next = kNoNode;
+        if (!ANY) {
+          const float lim = bpos >= 0 ? f_up(bt + prune_margin(bt)) : __int_as_float(0x7f800000);
+          const float low =
+              tmin_phys > 0 ? f_down(tmin_phys - prune_margin(tmin_phys)) : __int_as_float(0xff800000);
+          float key[4];
+          int ref[4];
+          int m = 0;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        ref[j] = __float_as_int(f4c(w[6], j));
-        const bool h = ref[j] != kWideEmpty && t0[j] <= t1[j] && t0[j] <= lim && t1[j] >= low;
-        key[j] = h ? t0[j] : __int_as_float(0x7f800000);
-        m += h ? 1 : 0;
-      }
-      // nearest child next, the others pushed far to near (leaves too:
-      // they are decided when they come off the stack)
+          for (int j = 0; j < 4; ++j) {
+            ref[j] = __float_as_int(f4c(w[6], j));
+            const bool h = ref[j] != kWideEmpty && t0[j] <= t1[j] && t0[j] <= lim && t1[j] >= low;
+            key[j] = h ? t0[j] : __int_as_float(0x7f800000);
+            m += h ? 1 : 0;
+          }
+          // nearest child next, the others pushed far to near
 #define RT_CSWAP(a, b)                                  \
   if (key[b] < key[a]) {                                \
     float tk = key[a]; key[a] = key[b]; key[b] = tk;    \
     int tr = ref[a]; ref[a] = ref[b]; ref[b] = tr;      \
   }
-      RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+          RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
 #undef RT_CSWAP
-      bool ovf = false;
-      if (m >= 4) ovf = ovf || !stack_push<RING>(S, spill, ref[3]);
-      if (m >= 3) ovf = ovf || !stack_push<RING>(S, spill, ref[2]);
-      if (m >= 2) ovf = ovf || !stack_push<RING>(S, spill, ref[1]);
-      if (ovf) {
-        atomicAdd(overflow, 1u);
-        bpos = -1;
-        done = true;
-      } else if (m >= 1) {
-        node = ref[0];
-      } else if (S.sp == 0) {
-        done = true;
-      } else {
-        node = stack_pop<RING>(S, spill);
-      }
-    } else if (WIDE && ANY) {
-      float4 w[7];
-      load_wide(B.wide, node, w);
-      st.visits += 4;
-      float t0[4], t1[4];
-      wide_slabs(r, rf, w, t0, t1);
-      // children in sorted-leaf order: the first passing one next, the
-      // others pushed so that they pop in order (left-first DFS)
-      int next = 0;
-      bool have = false, ovf = false;
+          if (m >= 4) ovf = ovf || !stack_push<RING>(S, spill, ref[3]);
+          if (m >= 3) ovf = ovf || !stack_push<RING>(S, spill, ref[2]);
+          if (m >= 2) ovf = ovf || !stack_push<RING>(S, spill, ref[1]);
+          if (m >= 1) next = ref[0];
+        } else {
+          // children in sorted-leaf order: the first passing one next, the
+          // others pushed so that they pop in order (left-first DFS)
 #pragma unroll
-      for (int j = 3; j >= 0; --j) {
-        const int ref = __float_as_int(f4c(w[6], j));
-        bool p = ref != kWideEmpty && t0[j] <= t1[j];
-        if (p && ref < 0) {
-          double l0, l1;
-          p = wide_leaf_slab(r, w, j, l0, l1);  // exact decision for a leaf child
+          for (int j = 3; j >= 0; --j) {
+            const int ref = __float_as_int(f4c(w[6], j));
+            bool p = ref != kWideEmpty && t0[j] <= t1[j];
+            if (p && ref < 0) {
+              double l0, l1;
+              p = wide_leaf_slab(r, w, j, l0, l1);  // exact decision for a leaf child
+            }
+            if (p) {
+              if (next != kNoNode) ovf = ovf || !stack_push<RING>(S, spill, next);
+              next = ref;
+            }
+          }
         }
-        if (p) {
-          if (have) ovf = ovf || !stack_push<RING>(S, spill, next);
-          next = ref;
-          have = true;
+        if (ovf) {
+          atomicAdd(overflow, 1u);
+          bpos = -1;
+          done = true;
+        } else if (next != kNoNode) {
+          node = next;
+        } else {
+          node = S.sp == 0 ? kNoNode : stack_pop<RING>(S, spill);
         }
       }
-      if (ovf) {
-        atomicAdd(overflow, 1u);
-        bpos = -1;
-        done = true;
-      } else if (have) {
-        node = next;
-      } else if (S.sp == 0) {
-        done = true;
-      } else {
-        node = stack_pop<RING>(S, spill);
-      }
+      if (!done && node == kNoNode && pending == kNoNode) done = true;
     } else if (!ANY) {
       float4 q0, q1, q2, q3;
       load_node(B.tnode, node, q0, q1, q2, q3);

@@ -167,7 +167,10 @@ rt_status rt_trace(const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_rays *r
  * image (kernels/raytracer.cu:106). counters_dev, if not NULL, receives
  * 5 uint64 counts added atomically: primary, shadow and bounce rays,
  * child-box slab tests (two per binary node entered, four per 4-wide node)
- * and triangle tests.
+ * and triangle tests. The ray counts are exact; the two work counts can
+ * differ slightly between identical calls, because the 4-wide traversal
+ * batches leaf tests across a wave and closest-hit pruning then depends on
+ * which rays share a wave. Images and hits never depend on it.
  * RT_RENDER_WAVEFRONT keeps per-pixel state in a context-owned workspace
  * that the first such call for a given image size allocates. */
 rt_status rt_render(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_material *mats_dev,
