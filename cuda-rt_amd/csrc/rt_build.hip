@@ -744,7 +744,8 @@ __global__ __launch_bounds__(256) void k_wide_parity(int n, const int32_t* __res
 __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restrict__ left,
                                                    const int32_t* __restrict__ right,
                                                    const float* __restrict__ node_box,
-                                                   const uint8_t* __restrict__ flag, float4* __restrict__ wide) {
+                                                   const uint8_t* __restrict__ flag, float4* __restrict__ wide,
+                                                   float4* __restrict__ wideq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1 || !flag[i]) return;
   int slot[4];
@@ -783,6 +784,44 @@ __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restr
   }
   w[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
   w[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+  // quantized form: corner = the node's box minimum, per axis the smallest
+  // power-of-two step with 255 steps covering the box
+  const float* nb = node_box + 6 * (size_t)i;
+  uint32_t ebits = 0, qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+  for (int a = 0; a < 3; ++a) {
+    const double o = nb[a], ext = (double)nb[3 + a] - o;
+    int e = 0;  // biased exponent of the step; 0 = zero extent
+    if (ext > 0) {
+      int ex2;
+      frexp(ext / 255.0, &ex2);  // ext/255 < 2^ex2
+      ex2 = ex2 < -126 ? -126 : ex2;
+      e = ex2 + 127;
+    }
+    ebits |= (uint32_t)e << (8 * a);
+    const double step = e ? ldexp(1.0, e - 127) : 0.0;
+    for (int j = 0; j < 4; ++j) {
+      uint32_t ql = 255, qh = 0;  // empty slot: lo above hi
+      if (j < k) {
+        if (step == 0) {
+          ql = 0;
+          qh = 0;
+        } else {
+          double l = floor((lo[a][j] - o) / step), h = ceil((hi[a][j] - o) / step);
+          while (l > 0 && o + l * step > lo[a][j]) l -= 1;
+          while (h < 255 && o + h * step < hi[a][j]) h += 1;
+          ql = (uint32_t)(l < 0 ? 0 : (l > 255 ? 255 : l));
+          qh = (uint32_t)(h < 0 ? 0 : (h > 255 ? 255 : h));
+        }
+      }
+      qlo[a] |= ql << (8 * j);
+      qhi[a] |= qh << (8 * j);
+    }
+  }
+  float4* q = wideq + 4 * (size_t)i;
+  q[0] = make_float4(nb[0], nb[1], nb[2], __uint_as_float(ebits));
+  q[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]), 0.f);
+  q[2] = make_float4(__uint_as_float(qhi[0]), __uint_as_float(qhi[1]), __uint_as_float(qhi[2]), 0.f);
+  q[3] = w[6];
 }
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
@@ -791,7 +830,8 @@ void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
   const int n = b->n;
   if (n < 2) return;
   if (topology_changed) k_wide_parity<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->parent, b->wide_flag);
-  k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide);
+  k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide,
+                                               b->wideq);
 }
 
 template <typename K>

@@ -26,6 +26,7 @@ struct BvhView {
   const float4* tnode;     // (n-1)*4: child boxes and child refs (64 B per internal node)
   const float4* ltri;      // n*3: vertices a, b, c of the leaf's triangle + triangle id (48 B)
   const float4* wide;      // (n-1)*8: 4-wide nodes (128 B), valid at even-depth internal nodes
+  const float4* wideq;     // (n-1)*4: the same nodes with 8-bit quantized child boxes (64 B)
 };
 
 // 4-wide traversal node (128 B, eight float4), stored at the index of the
@@ -38,6 +39,14 @@ struct BvhView {
 //   [6] child refs (int bits): >= 0 wide node, < 0 ~(sorted leaf position),
 //       kWideEmpty for an unused slot (its box is empty: lo = +inf, hi = -inf)
 //   [7] unused
+// The quantized form (64 B, four float4) stores the node's box corner and,
+// per axis, a power-of-two step; child planes are lo = corner + qlo*step
+// (rounded down) and hi = corner + qhi*step (rounded up), so the decoded
+// child boxes contain the exact ones:
+//   [0] corner.xyz, step exponents (biased as fp32 exponents) in bytes 0-2
+//   [1] qlo of children 0..3 (byte j = child j) for x, y, z; unused
+//   [2] qhi likewise
+//   [3] child refs as in the 128-B form
 constexpr int32_t kWideEmpty = 0x7fffffff;
 
 // Traversal record of internal node i (64 B, four float4):
@@ -76,9 +85,10 @@ struct rt_bvh {
   float4* tnode = nullptr;       // (n-1)*4
   float4* ltri = nullptr;        // n*3
   float4* wide = nullptr;        // (n-1)*8
+  float4* wideq = nullptr;       // (n-1)*4
   uint8_t* wide_flag = nullptr;  // n-1: 1 where the internal node is at even depth
 
-  BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide}; }
+  BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide, wideq}; }
 };
 
 // misc[] slots

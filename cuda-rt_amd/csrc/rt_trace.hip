@@ -349,10 +349,11 @@ __device__ __forceinline__ void flush_stats(unsigned long long* counters, const 
 // containing it passes), so that set is exactly the leaves whose own box
 // passes. Internal children may therefore use any test that never rejects
 // a box the FP64 test accepts: the 4-wide nodes use an fp32 slab test whose
-// distances are widened by E = 4 eps (B + |o|) |1/d| (eps = 2^-24, B the
+// distances are widened by E = 6 eps (B + |o|) |1/d| (eps = 2^-24, B the
 // largest coordinate magnitude of the scene box), which covers the rounding
-// of 1/d, of o/d and of the fused multiply-add; a leaf child is decided by
-// the FP64 test on its exact fp32 box, as in the binary traversal.
+// of 1/d, of o/d, of the decoded planes of quantized nodes and of the fused
+// multiply-adds; a leaf child is decided by the FP64 test on its exact fp32
+// box, as in the binary traversal.
 struct RayF {
   float ix, iy, iz;  // 1/d (+-1e30 for a zero component)
   float nx, ny, nz;  // o/d + E: near-plane offsets
@@ -363,7 +364,7 @@ __device__ __forceinline__ void rayf_axis(double o, double d, double bmax, float
   double inv = 1.0 / d;
   if (!(fabs(inv) <= 1e30)) inv = copysign(1e30, d);
   const double oi = o * inv;
-  const double E = 4.0 * 5.9604644775390625e-08 * (bmax + fabs(o)) * fabs(inv);
+  const double E = 6.0 * 5.9604644775390625e-08 * (bmax + fabs(o)) * fabs(inv);
   iv = (float)inv;
   nn = (float)(oi + E);
   ff = (float)(oi - E);
@@ -398,6 +399,34 @@ __device__ __forceinline__ void wide_slabs(const RayD& r, const RayF& f, const f
   for (int j = 0; j < 4; ++j) {
     const float a = fmaf(f4c(nx, j), f.ix, -f.nx), b = fmaf(f4c(ny, j), f.iy, -f.ny), c = fmaf(f4c(nz, j), f.iz, -f.nz);
     const float d = fmaf(f4c(fx, j), f.ix, -f.fx), e = fmaf(f4c(fy, j), f.iy, -f.fy), g = fmaf(f4c(fz, j), f.iz, -f.fz);
+    t0[j] = fmaxf(fmaxf(a, b), c);
+    t1[j] = fminf(fminf(d, e), g);
+  }
+}
+
+// The same distances from a quantized node: plane = corner + q * step,
+// folded into t = q * (step / d) + (corner / d - offset).
+__device__ __forceinline__ void wideq_slabs(const RayD& r, const RayF& f, const float4 (&q)[4], float (&t0)[4],
+                                            float (&t1)[4]) {
+  const uint32_t eb = __float_as_uint(q[0].w);
+  const float sx = __uint_as_float((eb & 0xffu) << 23), sy = __uint_as_float(((eb >> 8) & 0xffu) << 23),
+              sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
+  const float ax = sx * f.ix, ay = sy * f.iy, az = sz * f.iz;
+  const float cnx = fmaf(q[0].x, f.ix, -f.nx), cfx = fmaf(q[0].x, f.ix, -f.fx);
+  const float cny = fmaf(q[0].y, f.iy, -f.ny), cfy = fmaf(q[0].y, f.iy, -f.fy);
+  const float cnz = fmaf(q[0].z, f.iz, -f.nz), cfz = fmaf(q[0].z, f.iz, -f.fz);
+  const uint32_t lx = __float_as_uint(q[1].x), ly = __float_as_uint(q[1].y), lz = __float_as_uint(q[1].z);
+  const uint32_t hx = __float_as_uint(q[2].x), hy = __float_as_uint(q[2].y), hz = __float_as_uint(q[2].z);
+  const uint32_t nx = r.sx ? lx : hx, fx = r.sx ? hx : lx;
+  const uint32_t ny = r.sy ? ly : hy, fy = r.sy ? hy : ly;
+  const uint32_t nz = r.sz ? lz : hz, fz = r.sz ? hz : lz;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int sh = 8 * j;
+    const float a = fmaf((float)((nx >> sh) & 0xffu), ax, cnx), b = fmaf((float)((ny >> sh) & 0xffu), ay, cny),
+                c = fmaf((float)((nz >> sh) & 0xffu), az, cnz);
+    const float d = fmaf((float)((fx >> sh) & 0xffu), ax, cfx), e = fmaf((float)((fy >> sh) & 0xffu), ay, cfy),
+                g = fmaf((float)((fz >> sh) & 0xffu), az, cfz);
     t0[j] = fmaxf(fmaxf(a, b), c);
     t1[j] = fminf(fminf(d, e), g);
   }
@@ -470,12 +499,15 @@ __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uin
 }
 
 // views[scene[i]] is the BVH of ray i (scene == nullptr: views[0] for all).
-template <bool ANY, int RING, bool WIDE>
+// LAYOUT 2: binary nodes, FP64 child tests; 4: 4-wide nodes (128 B);
+// 5: quantized 4-wide nodes (64 B).
+template <bool ANY, int RING, int LAYOUT>
 __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __restrict__ views,
                                                                const uint16_t* __restrict__ scene, RayIn R,
                                                                double tmin_phys, HitOut O, uint32_t* fetch_ctr,
                                                                uint32_t fetch_chunk, uint32_t* overflow,
                                                                unsigned long long* counters) {
+  constexpr bool WIDE = LAYOUT != 2, QUANT = LAYOUT == 5;
   __shared__ int s_stack[RING * kThreads];
   const uint32_t nrays = R.count_dev ? *R.count_dev : R.count;
   const uint32_t kFetchChunk = fetch_chunk;
@@ -595,13 +627,25 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
           const int pos = ~pending;
           pending = kNoNode;
           const float4* L = B.ltri + 3 * (size_t)pos;
+          bool exact = true;
+          if (ANY && QUANT) {
+            // quantized nodes hold no exact leaf boxes: decide it here
+            const float4 q0 = ldg4(L), q1 = ldg4(L + 1), q2 = ldg4(L + 2);
+            const float ax = q0.x, ay = q0.y, az = q0.z, bx = q0.w, by = q1.x, bz = q1.y, cx = q1.z, cy = q1.w,
+                        cz = q2.x;
+            double l0, l1;
+            exact = slab(r, fminf(cx, fminf(ax, bx)), fminf(cy, fminf(ay, by)), fminf(cz, fminf(az, bz)),
+                         fmaxf(cx, fmaxf(ax, bx)), fmaxf(cy, fmaxf(ay, by)), fmaxf(cz, fmaxf(az, bz)), l0, l1);
+          }
           if (ANY) {
-            // the leaf's box was decided exactly when its node was entered
+            // otherwise the leaf's box was decided exactly when its node was entered
             double t, al, be;
-            ++st.tests;
-            if (tri_test(r, L, t, al, be)) {
-              bt = t; bal = al; bbe = be; bpos = pos;
-              done = true;
+            if (exact) {
+              ++st.tests;
+              if (tri_test(r, L, t, al, be)) {
+                bt = t; bal = al; bbe = be; bpos = pos;
+                done = true;
+              }
             }
           } else {
             // leaf record: q0 = a.xyz, b.x; q1 = b.yz, c.xy; q2 = c.z, id. The
@@ -631,10 +675,19 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
       }
       if (!done && node != kNoNode && node >= 0) {
         float4 w[7];
-        load_wide(B.wide, node, w);
-        st.visits += 4;
         float t0[4], t1[4];
-        wide_slabs(r, rf, w, t0, t1);
+        if (QUANT) {
+          float4 q[4];
+          const float4* p = B.wideq + 4 * (size_t)node;
+#pragma unroll
+          for (int k = 0; k < 4; ++k) q[k] = ldg4(p + k);
+          wideq_slabs(r, rf, q, t0, t1);
+          w[6] = q[3];
+        } else {
+          load_wide(B.wide, node, w);
+          wide_slabs(r, rf, w, t0, t1);
+        }
+        st.visits += 4;
         bool ovf = false;
         int next = kNoNode;
         if (!ANY) {
@@ -670,7 +723,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
           for (int j = 3; j >= 0; --j) {
             const int ref = __float_as_int(f4c(w[6], j));
             bool p = ref != kWideEmpty && t0[j] <= t1[j];
-            if (p && ref < 0) {
+            if (!QUANT && p && ref < 0) {
               double l0, l1;
               p = wide_leaf_slab(r, w, j, l0, l1);  // exact decision for a leaf child
             }
@@ -1375,14 +1428,17 @@ rt_tuning rt_default_tuning() {
   t.stack_ring = kRing;
   t.fetch_chunk = kDefaultFetchChunk;
   t.waves_per_simd = 5;
-  t.node_layout = 4;
+#ifndef RT_DEFAULT_LAYOUT
+#define RT_DEFAULT_LAYOUT 5
+#endif
+  t.node_layout = RT_DEFAULT_LAYOUT;
   return t;
 }
 
 bool rt_tuning_valid(const rt_tuning* t) {
   return t && (t->stack_ring == 16 || t->stack_ring == 32) &&
          (t->fetch_chunk == 64 || t->fetch_chunk == 128 || t->fetch_chunk == 256 || t->fetch_chunk == 512) &&
-         t->waves_per_simd >= 1 && t->waves_per_simd <= 8 && (t->node_layout == 2 || t->node_layout == 4);
+         t->waves_per_simd >= 1 && t->waves_per_simd <= 8 && (t->node_layout == 2 || t->node_layout == 4 || t->node_layout == 5);
 }
 
 static int persistent_grid(int num_cus, const rt_tuning& t, uint32_t max_rays) {
@@ -1396,21 +1452,17 @@ static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const
                               const uint16_t* scene, RayIn R, double tmin, HitOut O, uint32_t* fetch,
                               uint32_t* overflow, unsigned long long* counters) {
   const uint32_t fc = t.fetch_chunk;
-  if (t.node_layout == 4) {
-    if (t.stack_ring == 16)
-      k_trace_persistent<ANY, 16, true><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow,
-                                                                  counters);
-    else
-      k_trace_persistent<ANY, 32, true><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow,
-                                                                  counters);
+#define RT_LAUNCH(RINGV, LAYOUTV)                                                                              \
+  k_trace_persistent<ANY, RINGV, LAYOUTV><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, \
+                                                                    counters)
+  if (t.node_layout == 5) {
+    if (t.stack_ring == 16) RT_LAUNCH(16, 5); else RT_LAUNCH(32, 5);
+  } else if (t.node_layout == 4) {
+    if (t.stack_ring == 16) RT_LAUNCH(16, 4); else RT_LAUNCH(32, 4);
   } else {
-    if (t.stack_ring == 16)
-      k_trace_persistent<ANY, 16, false><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow,
-                                                                   counters);
-    else
-      k_trace_persistent<ANY, 32, false><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow,
-                                                                   counters);
+    if (t.stack_ring == 16) RT_LAUNCH(16, 2); else RT_LAUNCH(32, 2);
   }
+#undef RT_LAUNCH
 }
 
 hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays* rays, int32_t mode, double tmin,

@@ -198,7 +198,8 @@ typedef struct {
   int32_t fetch_chunk;     /* rays a wave reserves per atomic: 64, 128, 256 or 512 */
   int32_t waves_per_simd;  /* persistent traversal workgroups per CU, 1..8 */
   int32_t node_layout;     /* 2: binary nodes, FP64 child tests, 64 B per node;
-                              4: 4-wide nodes, fp32 conservative child tests, 128 B per node (default) */
+                              4: 4-wide nodes, fp32 conservative child tests, 128 B per node;
+                              5: 4-wide nodes with 8-bit quantized child boxes, 64 B per node (default) */
 } rt_tuning;
 
 typedef struct {
