@@ -501,7 +501,7 @@ __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uin
 // views[scene[i]] is the BVH of ray i (scene == nullptr: views[0] for all).
 // LAYOUT 2: binary nodes, FP64 child tests; 4: 4-wide nodes (128 B);
 // 5: quantized 4-wide nodes (64 B).
-template <bool ANY, int RING, int LAYOUT>
+template <bool ANY, int RING, int LAYOUT, bool MULTI>
 __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __restrict__ views,
                                                                const uint16_t* __restrict__ scene, RayIn R,
                                                                double tmin_phys, HitOut O, uint32_t* fetch_ctr,
@@ -576,7 +576,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
       }
       if (myid != 0xffffffffu) {
         rid = myid;
-        if (scene) {
+        if (MULTI) {
           B = views[scene[rid]];
           n = B.n;
         }
@@ -1452,9 +1452,15 @@ static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const
                               const uint16_t* scene, RayIn R, double tmin, HitOut O, uint32_t* fetch,
                               uint32_t* overflow, unsigned long long* counters) {
   const uint32_t fc = t.fetch_chunk;
-#define RT_LAUNCH(RINGV, LAYOUTV)                                                                              \
-  k_trace_persistent<ANY, RINGV, LAYOUTV><<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, \
-                                                                    counters)
+#define RT_LAUNCH(RINGV, LAYOUTV)                                                                                  \
+  do {                                                                                                              \
+    if (scene)                                                                                                      \
+      k_trace_persistent<ANY, RINGV, LAYOUTV, true>                                                                 \
+          <<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, counters);                     \
+    else                                                                                                            \
+      k_trace_persistent<ANY, RINGV, LAYOUTV, false>                                                                \
+          <<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, counters);                     \
+  } while (0)
   if (t.node_layout == 5) {
     if (t.stack_ring == 16) RT_LAUNCH(16, 5); else RT_LAUNCH(32, 5);
   } else if (t.node_layout == 4) {
@@ -1593,12 +1599,15 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     for (int bounce = 0; bounce <= max_depth; ++bounce) {
       RayIn R{W.ray[cur], W.ray[cur] + 3, 6, &W.counts[cur], 0};
       HitOut O{W.hpos, nullptr, nullptr, nullptr, nullptr, W.hit};
-      launch_persistent<false>(t, grid, s, W.views, W.rscene[cur], R, bounce == 0 ? 0.0 : kBounceTmin, O,
+      // one scene: every ray uses views[0] (no per-ray scene lookup)
+      launch_persistent<false>(t, grid, s, W.views, nscenes > 1 ? W.rscene[cur] : nullptr, R,
+                               bounce == 0 ? 0.0 : kBounceTmin, O,
                                W.fetch[0], ovf, cnt);
       k_wf_shade<<<tiles, kThreads, 0, s>>>(W, bounce, cur, it, cnt);
       RayIn SR{W.shq, W.shq + 3, 16, &W.counts[2], 0};
       HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr};
-      launch_persistent<true>(t, grid, s, W.views, W.sscene, SR, 0.0, SO, W.fetch[1], ovf, cnt);
+      launch_persistent<true>(t, grid, s, W.views, nscenes > 1 ? W.sscene : nullptr, SR, 0.0, SO, W.fetch[1], ovf,
+                              cnt);
       k_wf_shadow_finish<<<tiles, kThreads, 0, s>>>(W, it, bounce, cnt);
       k_wf_reset<<<1, 256, 0, s>>>(W.counts, (1u << 2) | (1u << cur), W.fetch[0], 2 * kFetchWords);
       cur ^= 1;
