@@ -159,8 +159,8 @@ rt_status rt_batch_autotune(rt_batch* b, void* stream, rt_tuning* best, float* b
   hipStream_t s = (hipStream_t)stream;
   const int layouts[] = {2, 4, 5};
   const int rings[] = {16, 32};
-  const int chunks[] = {128, 256};
-  const int waves[] = {3, 4, 5, 8};
+  const int chunks[] = {16, 64};
+  const int waves[] = {4, 8};
   hipEvent_t e0, e1;
   RT_HIP(b->ctx, hipEventCreate(&e0));
   RT_HIP(b->ctx, hipEventCreate(&e1));

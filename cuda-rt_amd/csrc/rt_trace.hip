@@ -1425,9 +1425,18 @@ static RenderParams rt_make_params(const rt_camera* cam, const rt_light* light, 
 
 rt_tuning rt_default_tuning() {
   rt_tuning t;
-  t.stack_ring = kRing;
-  t.fetch_chunk = kDefaultFetchChunk;
-  t.waves_per_simd = 5;
+#ifndef RT_DEF_RING
+#define RT_DEF_RING 32
+#endif
+#ifndef RT_DEF_CHUNK
+#define RT_DEF_CHUNK 16
+#endif
+#ifndef RT_DEF_WAVES
+#define RT_DEF_WAVES 4
+#endif
+  t.stack_ring = RT_DEF_RING;
+  t.fetch_chunk = RT_DEF_CHUNK;
+  t.waves_per_simd = RT_DEF_WAVES;
 #ifndef RT_DEFAULT_LAYOUT
 #define RT_DEFAULT_LAYOUT 5
 #endif
@@ -1437,7 +1446,8 @@ rt_tuning rt_default_tuning() {
 
 bool rt_tuning_valid(const rt_tuning* t) {
   return t && (t->stack_ring == 16 || t->stack_ring == 32) &&
-         (t->fetch_chunk == 64 || t->fetch_chunk == 128 || t->fetch_chunk == 256 || t->fetch_chunk == 512) &&
+         (t->fetch_chunk == 16 || t->fetch_chunk == 32 || t->fetch_chunk == 64 || t->fetch_chunk == 128 ||
+          t->fetch_chunk == 256 || t->fetch_chunk == 512) &&
          t->waves_per_simd >= 1 && t->waves_per_simd <= 8 && (t->node_layout == 2 || t->node_layout == 4 || t->node_layout == 5);
 }
 

@@ -195,7 +195,7 @@ rt_status rt_resolve_supersample(rt_ctx *ctx, const float *big_dev, int32_t widt
  * scene from one queue. Each scene's image is identical to rt_render's. */
 typedef struct {
   int32_t stack_ring;      /* LDS traversal stack entries per lane: 16 or 32 */
-  int32_t fetch_chunk;     /* rays a wave reserves per atomic: 64, 128, 256 or 512 */
+  int32_t fetch_chunk;     /* rays a wave reserves per atomic: 16, 32, 64, 128, 256 or 512 */
   int32_t waves_per_simd;  /* persistent traversal workgroups per CU, 1..8 */
   int32_t node_layout;     /* 2: binary nodes, FP64 child tests, 64 B per node;
                               4: 4-wide nodes, fp32 conservative child tests, 128 B per node;
