@@ -95,7 +95,7 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   size_t o_i0 = take((size_t)n * 4), o_i1 = take((size_t)n * 4);
   size_t o_par = take(nn * 4), o_l = take(ni * 4), o_r = take(ni * 4), o_f = take(ni * 4);
   size_t o_nb = take(nn * 6 * 4), o_tn = take(ni * 64), o_lt = take((size_t)n * 48);
-  size_t o_w = take(ni * 128), o_wq = take(ni * 64), o_wf = take(ni);
+  size_t o_w = take(ni * 128), o_wq = take(ni * 64), o_w8 = take(ni * 128), o_wf = take(ni);
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, off);
   if (e != hipSuccess || !p) {
@@ -127,6 +127,7 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   b->ltri = (float4*)(a + o_lt);
   b->wide = (float4*)(a + o_w);
   b->wideq = (float4*)(a + o_wq);
+  b->wide8 = (float4*)(a + o_w8);
   b->wide_flag = (uint8_t*)(a + o_wf);
   // The diagnostic counters in misc[] accumulate over the BVH's lifetime.
   e = hipMemset(b->misc, 0, kMiscWords * sizeof(uint32_t));

@@ -157,7 +157,7 @@ rt_status rt_batch_get_tuning(const rt_batch* b, rt_tuning* t) {
 rt_status rt_batch_autotune(rt_batch* b, void* stream, rt_tuning* best, float* best_ms) {
   if (!b) return RT_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
-  const int layouts[] = {2, 4, 5};
+  const int layouts[] = {2, 4, 5, 8};
   const int rings[] = {16, 32};
   const int chunks[] = {16, 64};
   const int waves[] = {4, 8};

@@ -735,7 +735,7 @@ __global__ __launch_bounds__(256) void k_wide_parity(int n, const int32_t* __res
     x = parent[x];
     ++d;
   }
-  flag[i] = (d & 1) == 0;
+  flag[i] = ((d & 1) == 0 ? 1 : 0) | (d % 3 == 0 ? 2 : 0);
 }
 
 // Fill the 4-wide node of every even-depth internal node from the binary
@@ -747,7 +747,7 @@ __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restr
                                                    const uint8_t* __restrict__ flag, float4* __restrict__ wide,
                                                    float4* __restrict__ wideq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n - 1 || !flag[i]) return;
+  if (i >= n - 1 || !(flag[i] & 1)) return;
   int slot[4];
   int k = 0;
   const int c[2] = {left[i], right[i]};
@@ -824,6 +824,102 @@ __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restr
   q[3] = w[6];
 }
 
+// Quantize up to W child boxes against the node box nb: per axis the
+// smallest power-of-two step with 255 steps covering the box; planes are
+// rounded outward so the decoded boxes contain the exact ones. Slots k..W-1
+// are empty (qlo 255 > qhi 0). Byte j of word (axis, j / 4) holds child j.
+template <int W>
+__device__ void quantize_children(const float* nb, int k, const float (&lo)[3][W], const float (&hi)[3][W],
+                                  uint32_t& ebits, uint32_t (&qlo)[3][W / 4], uint32_t (&qhi)[3][W / 4]) {
+  ebits = 0;
+  for (int a = 0; a < 3; ++a) {
+    const double o = nb[a], ext = (double)nb[3 + a] - o;
+    int e = 0;
+    if (ext > 0) {
+      int ex2;
+      frexp(ext / 255.0, &ex2);
+      ex2 = ex2 < -126 ? -126 : ex2;
+      e = ex2 + 127;
+    }
+    ebits |= (uint32_t)e << (8 * a);
+    const double step = e ? ldexp(1.0, e - 127) : 0.0;
+    for (int w = 0; w < W / 4; ++w) qlo[a][w] = qhi[a][w] = 0;
+    for (int j = 0; j < W; ++j) {
+      uint32_t ql = 255, qh = 0;
+      if (j < k) {
+        if (step == 0) {
+          ql = qh = 0;
+        } else {
+          double l = floor((lo[a][j] - o) / step), h = ceil((hi[a][j] - o) / step);
+          while (l > 0 && o + l * step > lo[a][j]) l -= 1;
+          while (h < 255 && o + h * step < hi[a][j]) h += 1;
+          ql = (uint32_t)(l < 0 ? 0 : (l > 255 ? 255 : l));
+          qh = (uint32_t)(h < 0 ? 0 : (h > 255 ? 255 : h));
+        }
+      }
+      qlo[a][j / 4] |= ql << (8 * (j & 3));
+      qhi[a][j / 4] |= qh << (8 * (j & 3));
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_wide8_fill(int n, const int32_t* __restrict__ left,
+                                                    const int32_t* __restrict__ right,
+                                                    const float* __restrict__ node_box,
+                                                    const uint8_t* __restrict__ flag, float4* __restrict__ wide8) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1 || !(flag[i] & 2)) return;
+  int slot[8];
+  int k = 0;
+  const int c[2] = {left[i], right[i]};
+  for (int j = 0; j < 2; ++j) {
+    if (c[j] >= n - 1) {
+      slot[k++] = c[j];
+      continue;
+    }
+    const int g[2] = {left[c[j]], right[c[j]]};
+    for (int h = 0; h < 2; ++h) {
+      if (g[h] >= n - 1) {
+        slot[k++] = g[h];
+      } else {
+        slot[k++] = left[g[h]];
+        slot[k++] = right[g[h]];
+      }
+    }
+  }
+  float lo[3][8], hi[3][8];
+  int ref[8];
+  for (int j = 0; j < 8; ++j) {
+    if (j < k) {
+      const float* b = node_box + 6 * (size_t)slot[j];
+      for (int a = 0; a < 3; ++a) {
+        lo[a][j] = b[a];
+        hi[a][j] = b[3 + a];
+      }
+      ref[j] = slot[j] >= n - 1 ? ~(slot[j] - (n - 1)) : slot[j];
+    } else {
+      for (int a = 0; a < 3; ++a) {
+        lo[a][j] = 0.f;
+        hi[a][j] = 0.f;
+      }
+      ref[j] = kWideEmpty;
+    }
+  }
+  uint32_t ebits, qlo[3][2], qhi[3][2];
+  const float* nb = node_box + 6 * (size_t)i;
+  quantize_children<8>(nb, k, lo, hi, ebits, qlo, qhi);
+  auto u = [](uint32_t v) { return __uint_as_float(v); };
+  float4* w = wide8 + 8 * (size_t)i;
+  w[0] = make_float4(nb[0], nb[1], nb[2], u(ebits));
+  w[1] = make_float4(u(qlo[0][0]), u(qlo[0][1]), u(qlo[1][0]), u(qlo[1][1]));
+  w[2] = make_float4(u(qlo[2][0]), u(qlo[2][1]), u(qhi[0][0]), u(qhi[0][1]));
+  w[3] = make_float4(u(qhi[1][0]), u(qhi[1][1]), u(qhi[2][0]), u(qhi[2][1]));
+  w[4] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
+  w[5] = make_float4(__int_as_float(ref[4]), __int_as_float(ref[5]), __int_as_float(ref[6]), __int_as_float(ref[7]));
+  w[6] = make_float4(0.f, 0.f, 0.f, 0.f);
+  w[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
@@ -832,6 +928,7 @@ void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
   if (topology_changed) k_wide_parity<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->parent, b->wide_flag);
   k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide,
                                                b->wideq);
+  k_wide8_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
 }
 
 template <typename K>

@@ -27,6 +27,7 @@ struct BvhView {
   const float4* ltri;      // n*3: vertices a, b, c of the leaf's triangle + triangle id (48 B)
   const float4* wide;      // (n-1)*8: 4-wide nodes (128 B), valid at even-depth internal nodes
   const float4* wideq;     // (n-1)*4: the same nodes with 8-bit quantized child boxes (64 B)
+  const float4* wide8;     // (n-1)*8: 8-wide quantized nodes (128 B), valid at depths divisible by 3
 };
 
 // 4-wide traversal node (128 B, eight float4), stored at the index of the
@@ -47,6 +48,15 @@ struct BvhView {
 //   [1] qlo of children 0..3 (byte j = child j) for x, y, z; unused
 //   [2] qhi likewise
 //   [3] child refs as in the 128-B form
+// 8-wide quantized node (128 B), stored at the binary internal nodes whose
+// depth is a multiple of 3; its up to eight children are the binary
+// descendants three levels down (or the leaves met before), in sorted-leaf
+// order:
+//   [0] corner.xyz, step exponents in bytes 0-2
+//   [1] qlo.x of children 0-3, 4-7, qlo.y of 0-3, 4-7
+//   [2] qlo.z of 0-3, 4-7, qhi.x of 0-3, 4-7
+//   [3] qhi.y of 0-3, 4-7, qhi.z of 0-3, 4-7
+//   [4] refs of children 0-3   [5] refs of children 4-7   [6], [7] unused
 constexpr int32_t kWideEmpty = 0x7fffffff;
 
 // Traversal record of internal node i (64 B, four float4):
@@ -86,9 +96,10 @@ struct rt_bvh {
   float4* ltri = nullptr;        // n*3
   float4* wide = nullptr;        // (n-1)*8
   float4* wideq = nullptr;       // (n-1)*4
-  uint8_t* wide_flag = nullptr;  // n-1: 1 where the internal node is at even depth
+  float4* wide8 = nullptr;       // (n-1)*8
+  uint8_t* wide_flag = nullptr;  // n-1: bit 0 = even depth (4-wide node), bit 1 = depth divisible by 3 (8-wide)
 
-  BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide, wideq}; }
+  BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide, wideq, wide8}; }
 };
 
 // misc[] slots

@@ -432,6 +432,40 @@ __device__ __forceinline__ void wideq_slabs(const RayD& r, const RayF& f, const 
   }
 }
 
+// Eight children of an 8-wide quantized node.
+__device__ __forceinline__ void wide8_slabs(const RayF& f, const float4 (&q)[6], float (&t0)[8], float (&t1)[8]) {
+  const uint32_t eb = __float_as_uint(q[0].w);
+  const float sx = __uint_as_float((eb & 0xffu) << 23), sy = __uint_as_float(((eb >> 8) & 0xffu) << 23),
+              sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
+  const float ax = sx * f.ix, ay = sy * f.iy, az = sz * f.iz;
+  const float cnx = fmaf(q[0].x, f.ix, -f.nx), cfx = fmaf(q[0].x, f.ix, -f.fx);
+  const float cny = fmaf(q[0].y, f.iy, -f.ny), cfy = fmaf(q[0].y, f.iy, -f.fy);
+  const float cnz = fmaf(q[0].z, f.iz, -f.nz), cfz = fmaf(q[0].z, f.iz, -f.fz);
+  const bool px = f.ix >= 0, py = f.iy >= 0, pz = f.iz >= 0;
+  const uint32_t lx[2] = {__float_as_uint(q[1].x), __float_as_uint(q[1].y)};
+  const uint32_t ly[2] = {__float_as_uint(q[1].z), __float_as_uint(q[1].w)};
+  const uint32_t lz[2] = {__float_as_uint(q[2].x), __float_as_uint(q[2].y)};
+  const uint32_t hx[2] = {__float_as_uint(q[2].z), __float_as_uint(q[2].w)};
+  const uint32_t hy[2] = {__float_as_uint(q[3].x), __float_as_uint(q[3].y)};
+  const uint32_t hz[2] = {__float_as_uint(q[3].z), __float_as_uint(q[3].w)};
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const uint32_t nx = px ? lx[h] : hx[h], fx = px ? hx[h] : lx[h];
+    const uint32_t ny = py ? ly[h] : hy[h], fy = py ? hy[h] : ly[h];
+    const uint32_t nz = pz ? lz[h] : hz[h], fz = pz ? hz[h] : lz[h];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int sh = 8 * j;
+      const float a = fmaf((float)((nx >> sh) & 0xffu), ax, cnx), b = fmaf((float)((ny >> sh) & 0xffu), ay, cny),
+                  c = fmaf((float)((nz >> sh) & 0xffu), az, cnz);
+      const float d = fmaf((float)((fx >> sh) & 0xffu), ax, cfx), e = fmaf((float)((fy >> sh) & 0xffu), ay, cfy),
+                  g = fmaf((float)((fz >> sh) & 0xffu), az, cfz);
+      t0[4 * h + j] = fmaxf(fmaxf(a, b), c);
+      t1[4 * h + j] = fminf(fminf(d, e), g);
+    }
+  }
+}
+
 // The FP64 slab test of child j's exact box.
 __device__ __forceinline__ bool wide_leaf_slab(const RayD& r, const float4 (&w)[7], int j, double& t0, double& t1) {
   return slab(r, f4c(w[0], j), f4c(w[1], j), f4c(w[2], j), f4c(w[3], j), f4c(w[4], j), f4c(w[5], j), t0, t1);
@@ -507,7 +541,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
                                                                double tmin_phys, HitOut O, uint32_t* fetch_ctr,
                                                                uint32_t fetch_chunk, uint32_t* overflow,
                                                                unsigned long long* counters) {
-  constexpr bool WIDE = LAYOUT != 2, QUANT = LAYOUT == 5;
+  constexpr bool WIDE = LAYOUT != 2, QUANT = LAYOUT == 5 || LAYOUT == 8;
   __shared__ int s_stack[RING * kThreads];
   const uint32_t nrays = R.count_dev ? *R.count_dev : R.count;
   const uint32_t kFetchChunk = fetch_chunk;
@@ -674,58 +708,78 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
         node = S.sp == 0 ? kNoNode : stack_pop<RING>(S, spill);
       }
       if (!done && node != kNoNode && node >= 0) {
+        constexpr int NW = LAYOUT == 8 ? 8 : 4;
         float4 w[7];
-        float t0[4], t1[4];
-        if (QUANT) {
+        float t0[NW], t1[NW];
+        int cref[NW];
+        if constexpr (LAYOUT == 8) {
+          float4 q[6];
+          const float4* p = B.wide8 + 8 * (size_t)node;
+#pragma unroll
+          for (int k = 0; k < 6; ++k) q[k] = ldg4(p + k);
+          wide8_slabs(rf, q, t0, t1);
+#pragma unroll
+          for (int j = 0; j < NW; ++j) cref[j] = __float_as_int(f4c(q[4 + j / 4], j & 3));
+        } else if constexpr (QUANT) {
           float4 q[4];
           const float4* p = B.wideq + 4 * (size_t)node;
 #pragma unroll
           for (int k = 0; k < 4; ++k) q[k] = ldg4(p + k);
           wideq_slabs(r, rf, q, t0, t1);
-          w[6] = q[3];
+#pragma unroll
+          for (int j = 0; j < NW; ++j) cref[j] = __float_as_int(f4c(q[3], j & 3));
         } else {
           load_wide(B.wide, node, w);
           wide_slabs(r, rf, w, t0, t1);
+#pragma unroll
+          for (int j = 0; j < NW; ++j) cref[j] = __float_as_int(f4c(w[6], j & 3));
         }
-        st.visits += 4;
+        st.visits += NW;
         bool ovf = false;
         int next = kNoNode;
         if (!ANY) {
           const float lim = bpos >= 0 ? f_up(bt + prune_margin(bt)) : __int_as_float(0x7f800000);
           const float low =
               tmin_phys > 0 ? f_down(tmin_phys - prune_margin(tmin_phys)) : __int_as_float(0xff800000);
-          float key[4];
-          int ref[4];
+          float key[NW];
+          int ref[NW];
           int m = 0;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) {
-            ref[j] = __float_as_int(f4c(w[6], j));
+          for (int j = 0; j < NW; ++j) {
+            ref[j] = cref[j];
             const bool h = ref[j] != kWideEmpty && t0[j] <= t1[j] && t0[j] <= lim && t1[j] >= low;
             key[j] = h ? t0[j] : __int_as_float(0x7f800000);
             m += h ? 1 : 0;
           }
-          // nearest child next, the others pushed far to near
+          // nearest child next, the others pushed far to near (sorting
+          // networks of 5 and 19 comparators)
 #define RT_CSWAP(a, b)                                  \
   if (key[b] < key[a]) {                                \
     float tk = key[a]; key[a] = key[b]; key[b] = tk;    \
     int tr = ref[a]; ref[a] = ref[b]; ref[b] = tr;      \
   }
-          RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+          if constexpr (NW == 4) {
+            RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(1, 2)
+          } else {
+            RT_CSWAP(0, 2) RT_CSWAP(1, 3) RT_CSWAP(4, 6) RT_CSWAP(5, 7) RT_CSWAP(0, 4) RT_CSWAP(1, 5) RT_CSWAP(2, 6)
+            RT_CSWAP(3, 7) RT_CSWAP(0, 1) RT_CSWAP(2, 3) RT_CSWAP(4, 5) RT_CSWAP(6, 7) RT_CSWAP(2, 4) RT_CSWAP(3, 5)
+            RT_CSWAP(1, 4) RT_CSWAP(3, 6) RT_CSWAP(1, 2) RT_CSWAP(3, 4) RT_CSWAP(5, 6)
+          }
 #undef RT_CSWAP
-          if (m >= 4) ovf = ovf || !stack_push<RING>(S, spill, ref[3]);
-          if (m >= 3) ovf = ovf || !stack_push<RING>(S, spill, ref[2]);
-          if (m >= 2) ovf = ovf || !stack_push<RING>(S, spill, ref[1]);
+#pragma unroll
+          for (int k = NW - 1; k >= 1; --k)
+            if (m > k) ovf = ovf || !stack_push<RING>(S, spill, ref[k]);
           if (m >= 1) next = ref[0];
         } else {
           // children in sorted-leaf order: the first passing one next, the
           // others pushed so that they pop in order (left-first DFS)
 #pragma unroll
-          for (int j = 3; j >= 0; --j) {
-            const int ref = __float_as_int(f4c(w[6], j));
+          for (int j = NW - 1; j >= 0; --j) {
+            const int ref = cref[j];
             bool p = ref != kWideEmpty && t0[j] <= t1[j];
-            if (!QUANT && p && ref < 0) {
+            if (LAYOUT == 4 && p && ref < 0) {
               double l0, l1;
-              p = wide_leaf_slab(r, w, j, l0, l1);  // exact decision for a leaf child
+              p = wide_leaf_slab(r, w, j & 3, l0, l1);  // exact decision for a leaf child
             }
             if (p) {
               if (next != kNoNode) ovf = ovf || !stack_push<RING>(S, spill, next);
@@ -1448,7 +1502,8 @@ bool rt_tuning_valid(const rt_tuning* t) {
   return t && (t->stack_ring == 16 || t->stack_ring == 32) &&
          (t->fetch_chunk == 16 || t->fetch_chunk == 32 || t->fetch_chunk == 64 || t->fetch_chunk == 128 ||
           t->fetch_chunk == 256 || t->fetch_chunk == 512) &&
-         t->waves_per_simd >= 1 && t->waves_per_simd <= 8 && (t->node_layout == 2 || t->node_layout == 4 || t->node_layout == 5);
+         t->waves_per_simd >= 1 && t->waves_per_simd <= 8 && (t->node_layout == 2 || t->node_layout == 4 || t->node_layout == 5 ||
+          t->node_layout == 8);
 }
 
 static int persistent_grid(int num_cus, const rt_tuning& t, uint32_t max_rays) {
@@ -1471,7 +1526,9 @@ static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const
       k_trace_persistent<ANY, RINGV, LAYOUTV, false>                                                                \
           <<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, counters);                     \
   } while (0)
-  if (t.node_layout == 5) {
+  if (t.node_layout == 8) {
+    if (t.stack_ring == 16) RT_LAUNCH(16, 8); else RT_LAUNCH(32, 8);
+  } else if (t.node_layout == 5) {
     if (t.stack_ring == 16) RT_LAUNCH(16, 5); else RT_LAUNCH(32, 5);
   } else if (t.node_layout == 4) {
     if (t.stack_ring == 16) RT_LAUNCH(16, 4); else RT_LAUNCH(32, 4);
