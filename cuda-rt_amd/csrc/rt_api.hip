@@ -58,6 +58,7 @@ rt_status rt_ctx_create(int32_t device, rt_ctx** out) {
 void rt_ctx_destroy(rt_ctx* ctx) {
   if (!ctx) return;
   if (ctx->workspace) (void)hipFree(ctx->workspace);
+  if (ctx->shade) (void)hipFree(ctx->shade);
   if (ctx->diff) (void)hipFree(ctx->diff);
   delete ctx;
 }
@@ -96,6 +97,7 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   size_t o_par = take(nn * 4), o_l = take(ni * 4), o_r = take(ni * 4), o_f = take(ni * 4);
   size_t o_nb = take(nn * 6 * 4), o_tn = take(ni * 64), o_lt = take((size_t)n * 48);
   size_t o_w = take(ni * 128), o_wq = take(ni * 64), o_w8 = take(ni * 128), o_wf = take(ni);
+  size_t o_lo = take((size_t)n * 4);
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, off);
   if (e != hipSuccess || !p) {
@@ -129,6 +131,7 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   b->wideq = (float4*)(a + o_wq);
   b->wide8 = (float4*)(a + o_w8);
   b->wide_flag = (uint8_t*)(a + o_wf);
+  b->leaf_of = (int32_t*)(a + o_lo);
   // The diagnostic counters in misc[] accumulate over the BVH's lifetime.
   e = hipMemset(b->misc, 0, kMiscWords * sizeof(uint32_t));
   if (e != hipSuccess) {
@@ -231,44 +234,6 @@ rt_status rt_trace(const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_rays* r
   return RT_OK;
 }
 
-rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_material* mats_dev,
-                    int32_t nmat, const rt_light* light, const rt_camera* cam, const rt_render_opts* opts,
-                    float* rgb_dev, uint64_t* counters_dev, void* stream) {
-  if (!ctx) return RT_ERR_INVALID;
-  if (!bvh || !mesh_ok(mesh) || !mats_dev || nmat <= 0 || !light || !cam || !opts || !rgb_dev)
-    return rt_set_error(ctx, RT_ERR_INVALID, "null argument to rt_render");
-  if (!mesh->nx_dev || !mesh->ny_dev || !mesh->nz_dev)
-    return rt_set_error(ctx, RT_ERR_INVALID, "rt_render needs shading normals");
-  if (cam->width <= 0 || cam->height <= 0 || opts->spp <= 0 || opts->depth < 0)
-    return rt_set_error(ctx, RT_ERR_INVALID, "bad image size, spp or depth");
-  if (mesh->n != bvh->n) return rt_set_error(ctx, RT_ERR_INVALID, "mesh and BVH triangle counts differ");
-  if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT))
-    return rt_set_error(ctx, RT_ERR_INVALID, "unknown render flags");
-  void* ws = nullptr;
-  if (opts->flags & RT_RENDER_WAVEFRONT) {
-    size_t need = rt_wavefront_bytes(cam->width, cam->height);
-    if (ctx->workspace_bytes < need) {
-      if (ctx->workspace) {
-        RT_HIP(ctx, hipStreamSynchronize((hipStream_t)stream));
-        RT_HIP(ctx, hipFree(ctx->workspace));
-        ctx->workspace = nullptr;
-        ctx->workspace_bytes = 0;
-      }
-      if (hipMalloc(&ctx->workspace, need) != hipSuccess) {
-        ctx->workspace = nullptr;
-        (void)hipGetLastError();
-        return rt_set_error(ctx, RT_ERR_OOM, "wavefront workspace of %zu bytes", need);
-      }
-      ctx->workspace_bytes = need;
-    }
-    ws = ctx->workspace;
-  }
-  RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, counters_dev, ws,
-                               (hipStream_t)stream));
-  return RT_OK;
-}
-
-
 static rt_status grow(rt_ctx* ctx, void** buf, size_t* have, size_t need, hipStream_t stream, const char* what) {
   if (*have >= need) return RT_OK;
   if (*buf) {
@@ -285,6 +250,36 @@ static rt_status grow(rt_ctx* ctx, void** buf, size_t* have, size_t need, hipStr
   *have = need;
   return RT_OK;
 }
+
+rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_material* mats_dev,
+                    int32_t nmat, const rt_light* light, const rt_camera* cam, const rt_render_opts* opts,
+                    float* rgb_dev, uint64_t* counters_dev, void* stream) {
+  if (!ctx) return RT_ERR_INVALID;
+  if (!bvh || !mesh_ok(mesh) || !mats_dev || nmat <= 0 || !light || !cam || !opts || !rgb_dev)
+    return rt_set_error(ctx, RT_ERR_INVALID, "null argument to rt_render");
+  if (!mesh->nx_dev || !mesh->ny_dev || !mesh->nz_dev)
+    return rt_set_error(ctx, RT_ERR_INVALID, "rt_render needs shading normals");
+  if (cam->width <= 0 || cam->height <= 0 || opts->spp <= 0 || opts->depth < 0)
+    return rt_set_error(ctx, RT_ERR_INVALID, "bad image size, spp or depth");
+  if (mesh->n != bvh->n) return rt_set_error(ctx, RT_ERR_INVALID, "mesh and BVH triangle counts differ");
+  if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT))
+    return rt_set_error(ctx, RT_ERR_INVALID, "unknown render flags");
+  void* ws = nullptr;
+  void* shade = nullptr;
+  if (opts->flags & RT_RENDER_WAVEFRONT) {
+    hipStream_t s = (hipStream_t)stream;
+    rt_status st = grow(ctx, &ctx->workspace, &ctx->workspace_bytes, rt_wavefront_bytes(cam->width, cam->height),
+                        s, "wavefront workspace");
+    if (st == RT_OK) st = grow(ctx, &ctx->shade, &ctx->shade_bytes, rt_shade_table_bytes(mesh->n), s, "shading table");
+    if (st != RT_OK) return st;
+    ws = ctx->workspace;
+    shade = ctx->shade;
+  }
+  RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, counters_dev, ws, shade,
+                               (hipStream_t)stream));
+  return RT_OK;
+}
+
 
 static rt_status diff_check(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const double* albedo,
                             double light_scale, const rt_light* light, const rt_camera* cam,
@@ -327,6 +322,7 @@ rt_status rt_diff_forward(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* me
   if (st != RT_OK) return st;
   st = grow(ctx, &ctx->workspace, &ctx->workspace_bytes, rt_wavefront_bytes(cam->width, cam->height), s,
             "wavefront workspace");
+  if (st == RT_OK) st = grow(ctx, &ctx->shade, &ctx->shade_bytes, rt_shade_table_bytes(mesh->n), s, "shading table");
   if (st != RT_OK) return st;
   rt_material* mats = (rt_material*)ctx->diff;
   int32_t* ids = (int32_t*)((char*)ctx->diff + mats_bytes);
@@ -339,7 +335,7 @@ rt_status rt_diff_forward(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* me
   for (int k = 0; k < 3; ++k) L.color[k] *= light_scale;
   rt_render_opts o = *opts;
   o.flags = (o.flags & RT_RENDER_RANDVEC_RTL) | RT_RENDER_WAVEFRONT;
-  RT_HIP(ctx, rt_launch_render(bvh, &m, mats, mesh->n, &L, cam, &o, rgb, nullptr, ctx->workspace, s,
+  RT_HIP(ctx, rt_launch_render(bvh, &m, mats, mesh->n, &L, cam, &o, rgb, nullptr, ctx->workspace, ctx->shade, s,
                                records_dev, image_dev));
   return RT_OK;
 }

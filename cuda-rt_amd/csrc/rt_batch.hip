@@ -24,6 +24,7 @@ struct rt_batch {
   rt_ctx* ctx = nullptr;
   std::vector<rt_batch_item> items;
   void* ws = nullptr;
+  void* shade = nullptr;  // the scenes' shading tables, back to back
   size_t npix = 0, ntiles = 0, nscenes = 0;
   int max_spp = 0, max_depth = 0;
   rt_tuning tuning;
@@ -42,6 +43,7 @@ void rt_batch_destroy(rt_batch* b) {
   }
   if (b->fork) (void)hipEventDestroy(b->fork);
   if (b->ws) (void)hipFree(b->ws);
+  if (b->shade) (void)hipFree(b->shade);
   delete b;
 }
 
@@ -62,9 +64,12 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
   b->items.assign(items, items + count);
   b->nscenes = (size_t)count;
   b->tuning = rt_default_tuning();
-  std::vector<size_t> pix_off(count), tile_off(count);
+  std::vector<size_t> pix_off(count), tile_off(count), shade_off(count);
+  size_t shade_bytes = 0;
   for (int i = 0; i < count; ++i) {
     const rt_camera& c = items[i].cam;
+    shade_off[i] = shade_bytes;
+    shade_bytes += rt_align256(rt_shade_table_bytes((size_t)items[i].mesh.n));
     pix_off[i] = b->npix;
     tile_off[i] = b->ntiles;
     b->npix += (size_t)c.width * c.height;
@@ -77,11 +82,10 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
     return rt_set_error(ctx, RT_ERR_INVALID, "batch has too many pixels");
   }
   size_t bytes = rt_wavefront_bytes_batch(b->npix, b->ntiles, b->nscenes);
-  if (hipMalloc(&b->ws, bytes) != hipSuccess) {
+  if (hipMalloc(&b->ws, bytes) != hipSuccess || hipMalloc(&b->shade, shade_bytes) != hipSuccess) {
     (void)hipGetLastError();
-    b->ws = nullptr;
     rt_batch_destroy(b);
-    return rt_set_error(ctx, RT_ERR_OOM, "batch workspace of %zu bytes", bytes);
+    return rt_set_error(ctx, RT_ERR_OOM, "batch workspace of %zu + %zu bytes", bytes, shade_bytes);
   }
   // Scene table, BVH views and tile -> scene map: the first three regions.
   const size_t sd = rt_scene_desc_bytes(), vb = rt_bvh_view_bytes();
@@ -90,7 +94,7 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
   for (int i = 0; i < count; ++i) {
     const rt_batch_item& it = items[i];
     rt_fill_scene(table.data() + sd * i, it.bvh, &it.mesh, it.mats_dev, it.nmat, &it.light, &it.cam, &it.opts,
-                  it.rgb_dev, (uint32_t)pix_off[i], (uint32_t)tile_off[i]);
+                  it.rgb_dev, (uint32_t)pix_off[i], (uint32_t)tile_off[i], (char*)b->shade + shade_off[i]);
     BvhView v = it.bvh->view();
     ::memcpy(views.data() + vb * i, &v, sizeof(v));
     size_t nt = (size_t)((it.cam.width + 15) / 16) * ((it.cam.height + 15) / 16);

@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) void k_hierarchy(const K* __restrict__ keys, c
                                                    const int32_t* __restrict__ idx, int32_t* __restrict__ parent,
                                                    int32_t* __restrict__ left, int32_t* __restrict__ right,
                                                    uint32_t* __restrict__ flags, float* __restrict__ node_box,
-                                                   float4* __restrict__ ltri) {
+                                                   float4* __restrict__ ltri, int32_t* __restrict__ leaf_of) {
   int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   {
@@ -365,6 +365,7 @@ __global__ __launch_bounds__(256) void k_hierarchy(const K* __restrict__ keys, c
     L[0] = make_float4(vx[a], vy[a], vz[a], vx[b]);
     L[1] = make_float4(vy[b], vz[b], vx[c], vy[c]);
     L[2] = make_float4(vz[c], __int_as_float(tri), 0.f, 0.f);
+    leaf_of[tri] = i;
   }
   if (i == 0) parent[0] = -1;
   if (i >= n - 1) return;
@@ -587,7 +588,7 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
     const int32_t* __restrict__ idx, int32_t* __restrict__ parent, int32_t* __restrict__ left,
     int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode, float4* __restrict__ ltri,
     float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, uint32_t* __restrict__ deferred,
-    uint32_t* __restrict__ misc) {
+    uint32_t* __restrict__ misc, int32_t* __restrict__ leaf_of) {
   __shared__ uint32_t s_arr[kClimbChunk];
   const int s0 = blockIdx.x * kClimbChunk;
   const int e0 = min(n, s0 + kClimbChunk);
@@ -604,6 +605,7 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
     L[0] = make_float4(vx[a], vy[a], vz[a], vx[b]);
     L[1] = make_float4(vy[b], vz[b], vx[c], vy[c]);
     L[2] = make_float4(vz[c], __int_as_float(tri), 0.f, 0.f);
+    if (leaf_of) leaf_of[tri] = i;  // only a build reorders the leaves
   }
   if (s0 == 0 && threadIdx.x == 0) parent[0] = -1;
   __syncthreads();
@@ -953,13 +955,13 @@ hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
   if (b->build_flags & RT_BUILD_KARRAS_SEARCH) {
     k_hierarchy<K><<<cdiv(n, 256), 256, 0, s>>>(keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev,
                                                 m->idx_dev, b->parent, b->left, b->right, b->flags, b->node_box,
-                                                b->ltri);
+                                                b->ltri, b->leaf_of);
     if (n > 1)
       k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
   } else {
     k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
         keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->parent, b->left, b->right,
-        b->node_box, b->tnode, b->ltri, b->deposits, b->arrivals, b->deferred, b->misc);
+        b->node_box, b->tnode, b->ltri, b->deposits, b->arrivals, b->deferred, b->misc, b->leaf_of);
     if (n > 1)
       k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
                                                     b->deposits, b->arrivals, b->deferred, b->misc);
@@ -988,7 +990,7 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
   const K* keys = (const K*)b->keys[0];
   k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
       keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->parent, b->left, b->right,
-      b->node_box, b->tnode, b->ltri, b->deposits, b->arrivals, b->deferred, b->misc);
+      b->node_box, b->tnode, b->ltri, b->deposits, b->arrivals, b->deferred, b->misc, nullptr);
   if (n > 1)
     k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
                                                   b->deposits, b->arrivals, b->deferred, b->misc);

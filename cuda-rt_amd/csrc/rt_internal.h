@@ -13,6 +13,8 @@ struct rt_ctx {
   std::string err;
   void* workspace = nullptr;     // wavefront render state, grown on demand
   size_t workspace_bytes = 0;
+  void* shade = nullptr;         // per-leaf shading table of the last render (64 B per triangle)
+  size_t shade_bytes = 0;
   void* diff = nullptr;          // differentiable render: materials, identity ids, float image
   size_t diff_bytes = 0;
 };
@@ -28,6 +30,7 @@ struct BvhView {
   const float4* wide;      // (n-1)*8: 4-wide nodes (128 B), valid at even-depth internal nodes
   const float4* wideq;     // (n-1)*4: the same nodes with 8-bit quantized child boxes (64 B)
   const float4* wide8;     // (n-1)*8: 8-wide quantized nodes (128 B), valid at depths divisible by 3
+  const int32_t* leaf_of;  // n: sorted leaf position of each triangle (inverse of the sorted ids)
 };
 
 // 4-wide traversal node (128 B, eight float4), stored at the index of the
@@ -98,8 +101,9 @@ struct rt_bvh {
   float4* wideq = nullptr;       // (n-1)*4
   float4* wide8 = nullptr;       // (n-1)*8
   uint8_t* wide_flag = nullptr;  // n-1: bit 0 = even depth (4-wide node), bit 1 = depth divisible by 3 (8-wide)
+  int32_t* leaf_of = nullptr;    // n: triangle id -> sorted leaf position
 
-  BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide, wideq, wide8}; }
+  BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide, wideq, wide8, leaf_of}; }
 };
 
 // misc[] slots
@@ -139,19 +143,22 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
 // of the differentiable render.
 hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
-                            uint64_t* counters, void* workspace, hipStream_t s, int32_t* records = nullptr,
-                            double* color64 = nullptr);
+                            uint64_t* counters, void* workspace, void* shade_table, hipStream_t s,
+                            int32_t* records = nullptr, double* color64 = nullptr);
 size_t rt_wavefront_bytes(int32_t width, int32_t height);
+// Per-leaf shading table of an n-triangle scene: 64 B per sorted leaf
+// (three vertex normals and the material id), refreshed by every render.
+inline size_t rt_shade_table_bytes(size_t n) { return n * 64; }
 size_t rt_wavefront_bytes_batch(size_t npix, size_t ntiles, size_t nscenes);
 size_t rt_scene_desc_bytes();
 size_t rt_bvh_view_bytes();
 void rt_fill_scene(void* table_entry, const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                    const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
-                   uint32_t pix_off, uint32_t tile_off);
+                   uint32_t pix_off, uint32_t tile_off, void* shade_table);
 hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
                                uint32_t* overflow, hipStream_t s, int32_t* records = nullptr,
-                               double* color64 = nullptr);
+                               double* color64 = nullptr, const void* single_scene = nullptr);
 // Differentiable render (rt_diff.hip).
 hipError_t rt_launch_zero_f64(double* p, size_t n, hipStream_t s);
 hipError_t rt_launch_resolve(const float* big, int32_t W, int32_t H, int32_t ss, bool reference_divisor, float* out,

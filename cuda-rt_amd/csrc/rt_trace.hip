@@ -504,9 +504,10 @@ constexpr uint32_t kFetchStride = 32;   // counters 128 bytes apart
 constexpr uint32_t kFetchWords = kSegments * kFetchStride;
 
 struct RayIn {
-  const double* orig;  // origin of ray i at orig + i*stride
-  const double* dir;   // direction of ray i at dir + i*stride
-  int stride;          // in doubles
+  const double* orig;  // component k of ray i's origin at orig[i*stride + k*comp]
+  const double* dir;   // component k of ray i's direction at dir[i*stride + k*comp]
+  size_t stride;       // between rays, in doubles
+  size_t comp;         // between components, in doubles (1: xyz interleaved; the queue capacity: SoA)
   const uint32_t* count_dev;  // number of rays (device), or nullptr
   uint32_t count;             // used when count_dev is nullptr
 };
@@ -515,7 +516,8 @@ struct HitOut {
   int32_t* pos;  // sorted leaf position or -1 (nullable)
   int32_t* tri;  // triangle id or -1 (nullable)
   double *t, *al, *be;  // nullable
-  double* tab;   // nullable: t, alpha, beta interleaved (3 per ray)
+  double* tab;   // nullable: t, alpha, beta of ray i at tab[i], tab[i + tab_comp], tab[i + 2*tab_comp]
+  size_t tab_comp;
 };
 
 __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uint32_t i, int pos, double t, double al,
@@ -526,9 +528,9 @@ __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uin
   if (O.al) O.al[i] = pos >= 0 ? al : 0.0;
   if (O.be) O.be[i] = pos >= 0 ? be : 0.0;
   if (O.tab) {
-    O.tab[3 * (size_t)i] = pos >= 0 ? t : 0.0;
-    O.tab[3 * (size_t)i + 1] = pos >= 0 ? al : 0.0;
-    O.tab[3 * (size_t)i + 2] = pos >= 0 ? be : 0.0;
+    O.tab[i] = pos >= 0 ? t : 0.0;
+    O.tab[i + O.tab_comp] = pos >= 0 ? al : 0.0;
+    O.tab[i + 2 * O.tab_comp] = pos >= 0 ? be : 0.0;
   }
 }
 
@@ -616,7 +618,8 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
         }
         const double* po = R.orig + (size_t)rid * R.stride;
         const double* pd = R.dir + (size_t)rid * R.stride;
-        r = make_ray(mk(ldgd(po), ldgd(po + 1), ldgd(po + 2)), mk(ldgd(pd), ldgd(pd + 1), ldgd(pd + 2)));
+        r = make_ray(mk(ldgd(po), ldgd(po + R.comp), ldgd(po + 2 * R.comp)),
+                     mk(ldgd(pd), ldgd(pd + R.comp), ldgd(pd + 2 * R.comp)));
         if (WIDE) rf = make_rayf(r, B.node_box);
         bpos = -1;
         bt = 0;
@@ -927,6 +930,11 @@ struct MeshShade {
   const float *nx, *ny, *nz;
   const int32_t* idx;
   const int32_t* mat_id;
+  // Optional per-leaf shading table, 64 B per sorted leaf position:
+  // {na.xyz, material id bits}, {nb.xyz, 0}, {nc.xyz, 0}, {0}. The same
+  // floats as the mesh's, laid out so a hit reads one record instead of
+  // gathering index, normals and material id through the triangle id.
+  const float4* lshade;
 };
 
 __device__ __forceinline__ d3 arr3(const double* p) { return mk(p[0], p[1], p[2]); }
@@ -938,11 +946,50 @@ __device__ __forceinline__ const rt_material& mat_of(const rt_material* mats, in
 }
 
 // Smooth normal (kernels/bvh.cu:357-364).
+__device__ __forceinline__ d3 blend_normal(d3 na, d3 nb, d3 nc, double al, double be) {
+  d3 N = (na * (1.0 - (al + be)) + nb * al) + nc * be;
+  return rt::normalize(N);
+}
 __device__ __forceinline__ d3 shading_normal(const MeshShade& M, int tri, double al, double be) {
   int a = M.idx[3 * tri], b = M.idx[3 * tri + 1], c = M.idx[3 * tri + 2];
   d3 na = mk(M.nx[a], M.ny[a], M.nz[a]), nb = mk(M.nx[b], M.ny[b], M.nz[b]), nc = mk(M.nx[c], M.ny[c], M.nz[c]);
-  d3 N = (na * (1.0 - (al + be)) + nb * al) + nc * be;
-  return rt::normalize(N);
+  return blend_normal(na, nb, nc, al, be);
+}
+
+// Material and smooth normal of the triangle at sorted leaf `pos`, from the
+// shading table when the render has one.
+struct LeafShade {
+  const rt_material* m;
+  d3 na, nb, nc;
+};
+__device__ __forceinline__ const rt_material& clamp_material(const rt_material* mats, int nmat, int id) {
+  if (id < 0 || id >= nmat) id = 0;
+  return mats[id];
+}
+__device__ __forceinline__ LeafShade leaf_shade(const rt_material* mats, int nmat, const MeshShade& M,
+                                                const BvhView& B, int pos) {
+  LeafShade L;
+  if (M.lshade) {
+    const float4* r = M.lshade + 4 * (size_t)pos;
+    float4 a = ldg4(r), b = ldg4(r + 1), c = ldg4(r + 2);
+    L.m = &clamp_material(mats, nmat, __float_as_int(a.w));
+    L.na = mk(a.x, a.y, a.z);
+    L.nb = mk(b.x, b.y, b.z);
+    L.nc = mk(c.x, c.y, c.z);
+  } else {
+    const int tri = leaf_tri_of(B, pos);
+    L.m = &mat_of(mats, nmat, M, tri);
+    int a = M.idx[3 * tri], b = M.idx[3 * tri + 1], c = M.idx[3 * tri + 2];
+    L.na = mk(M.nx[a], M.ny[a], M.nz[a]);
+    L.nb = mk(M.nx[b], M.ny[b], M.nz[b]);
+    L.nc = mk(M.nx[c], M.ny[c], M.nz[c]);
+  }
+  return L;
+}
+__device__ __forceinline__ const rt_material& leaf_material(const rt_material* mats, int nmat, const MeshShade& M,
+                                                            const BvhView& B, int pos) {
+  if (M.lshade) return clamp_material(mats, nmat, __float_as_int(ldgf((const float*)(M.lshade + 4 * (size_t)pos) + 3)));
+  return mat_of(mats, nmat, M, leaf_tri_of(B, pos));
 }
 
 // Cosine-weighted direction about n: the frame of src/ray.cpp:17-30 with
@@ -1054,7 +1101,7 @@ __device__ __forceinline__ d3 shadow_factor(const RenderParams& P, const BvhView
   if (!hit) return mk(1, 1, 1);
   if (!(s.t < rt::length_f(arr3(P.lpos) - q))) return mk(1, 1, 1);
   if (s.t < kRayEps) return mk(1, 1, 1);
-  const rt_material& sm = mat_of(mats, P.nmat, M, leaf_tri(B, s.pos));
+  const rt_material& sm = leaf_material(mats, P.nmat, M, B, s.pos);
   bool trans = (sm.kt[0] == 0) && (sm.kt[1] == 0) && (sm.kt[2] == 0);
   return trans ? rt::hadamard(arr3(sm.kd), mk(1, 1, 1) - arr3(sm.kt)) : mk(0, 0, 0);
 }
@@ -1210,7 +1257,21 @@ struct WfState {
   int2* rec;            // optional path record: (hit leaf, occluder code) per pixel, sample, bounce
   int rec_spp, rec_depth;
   double* color64;      // optional FP64 output (pixels*3, row-flipped like rgb, unclamped mean)
+  size_t npix, cap;     // component strides: per-pixel arrays are SoA over npix, queues SoA over cap
+  unsigned long long* rep;  // kCtrReplicas x kCtrStride ray counters, summed into the caller's at the end
 };
+
+// Ray counts of the shading passes go to one of kCtrReplicas copies of the
+// counters (by workgroup), each on its own 128-byte line: all waves of such
+// a pass finish together, and one shared address would serialise their
+// atomics.
+constexpr int kCtrReplicas = 64;
+constexpr int kCtrStride = 16;
+__device__ __forceinline__ void rep_count(unsigned long long* rep, int k, bool pred) {
+  const uint64_t m = __ballot(pred);
+  if ((threadIdx.x & 63) == 0 && m)
+    atomicAdd(&rep[(blockIdx.x & (kCtrReplicas - 1)) * kCtrStride + k], (unsigned long long)__popcll(m));
+}
 
 // Path record index of (global pixel, sample, bounce).
 __device__ __forceinline__ size_t rec_index(const WfState& W, uint32_t p, int sample, int bounce) {
@@ -1218,6 +1279,13 @@ __device__ __forceinline__ size_t rec_index(const WfState& W, uint32_t p, int sa
 }
 
 __device__ __forceinline__ d3 ld3(const double* p) { return mk(p[0], p[1], p[2]); }
+// Component-major (SoA) vectors: x at a[i], y at a[n + i], z at a[2n + i].
+__device__ __forceinline__ d3 ldv(const double* a, size_t i, size_t n) { return mk(a[i], a[n + i], a[2 * n + i]); }
+__device__ __forceinline__ void stv(double* a, size_t i, size_t n, d3 v) {
+  a[i] = v.x;
+  a[n + i] = v.y;
+  a[2 * n + i] = v.z;
+}
 __device__ __forceinline__ void st3(double* p, d3 v) {
   p[0] = v.x;
   p[1] = v.y;
@@ -1245,42 +1313,59 @@ __device__ __forceinline__ uint32_t block_append(uint32_t* counter, bool want, u
   return r;
 }
 
+// The scene table entry of scene s. A single-scene render passes its one
+// entry as a kernel argument (MULTI = false), so every parameter is a
+// uniform scalar load instead of a per-lane gather from the table.
+template <bool MULTI>
+__device__ __forceinline__ const SceneDesc& scene_of(const WfState& W, const SceneDesc& D1, int s) {
+  if constexpr (MULTI)
+    return W.scenes[s];
+  else
+    return D1;
+}
+
 // Global tile -> (scene, pixel); false outside the scene's image.
-__device__ __forceinline__ bool tile_pixel(const WfState& W, uint32_t gt, int lane, int& s, int& px, int& py,
-                                           uint32_t& gp) {
-  s = W.tile_scene[gt];
-  const SceneDesc& D = W.scenes[s];
+template <bool MULTI>
+__device__ __forceinline__ bool tile_pixel(const WfState& W, const SceneDesc& D1, uint32_t gt, int lane, int& s,
+                                           int& px, int& py, uint32_t& gp) {
+  s = MULTI ? W.tile_scene[gt] : 0;
+  const SceneDesc& D = scene_of<MULTI>(W, D1, s);
   pixel_of_tile(D.P, (int)(gt - D.tile_off), lane, px, py);
   if (px >= D.P.W || py >= D.P.H) return false;
   gp = D.pix_off + (uint32_t)(py * D.P.W + px);
   return true;
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_init(WfState W) {
-  if (blockIdx.x == 0 && threadIdx.x < 16) W.counts[threadIdx.x] = 0u;
+template <bool MULTI>
+__global__ __launch_bounds__(kThreads) void k_wf_init(WfState W, SceneDesc D1) {
+  if (blockIdx.x == 0) {
+    if (threadIdx.x < 16) W.counts[threadIdx.x] = 0u;
+    for (int k = threadIdx.x; k < kCtrReplicas * kCtrStride; k += kThreads) W.rep[k] = 0ull;
+  }
   int s, px, py;
   uint32_t gp;
-  if (!tile_pixel(W, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
-  const RenderParams& P = W.scenes[s].P;
+  if (!tile_pixel<MULTI>(W, D1, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
+  const RenderParams& P = scene_of<MULTI>(W, D1, s).P;
   W.rng[gp] = rt::wang_hash(rt::seed_index(px, py, P.W) ^ P.seed);
-  st3(W.color + 3 * (size_t)gp, mk(0, 0, 0));
+  stv(W.color, gp, W.npix, mk(0, 0, 0));
 }
 
 // Start sample k: slot = tile*256 + lane holds the pixel's primary ray.
 // Slots outside an image, or of a scene that has taken all its samples,
 // get a NaN ray, which misses the root box.
-__global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, int k, uint32_t nslots) {
+template <bool MULTI>
+__global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, SceneDesc D1, int k, uint32_t nslots) {
   const size_t slot = (size_t)blockIdx.x * kThreads + threadIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0) W.counts[0] = nslots;
   int s, px, py;
   uint32_t gp;
-  bool ok = tile_pixel(W, blockIdx.x, threadIdx.x, s, px, py, gp);
-  const RenderParams& P = W.scenes[s].P;
-  double* rr = W.ray[0] + 6 * slot;
+  bool ok = tile_pixel<MULTI>(W, D1, blockIdx.x, threadIdx.x, s, px, py, gp);
+  const RenderParams& P = scene_of<MULTI>(W, D1, s).P;
+  double* rr = W.ray[0] + slot;
   W.rscene[0][slot] = (uint16_t)s;
   if (!ok || k >= P.spp) {
     const double nan = __builtin_nan("");
-    for (int j = 0; j < 6; ++j) rr[j] = nan;
+    for (int j = 0; j < 6; ++j) rr[j * W.cap] = nan;
     W.pix[0][slot] = 0xffffffffu;
     return;
   }
@@ -1290,14 +1375,36 @@ __global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, int k, uint32
   d3 o, d;
   lens_ray(P, xx, yy, rng, o, d);
   W.rng[gp] = rng;
-  st3(rr, o);
-  st3(rr + 3, d);
+  stv(rr, 0, W.cap, o);
+  stv(rr + 3 * W.cap, 0, W.cap, d);
   W.pix[0][slot] = gp;
-  st3(W.thr + 3 * (size_t)gp, mk(1, 1, 1));
-  st3(W.acc + 3 * (size_t)gp, mk(0, 0, 0));
+  stv(W.thr, gp, W.npix, mk(1, 1, 1));
+  stv(W.acc, gp, W.npix, mk(0, 0, 0));
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, int bounce, int cur, int sample,
+// Shading table of each scene (blockIdx.y), in triangle order so the
+// index, normal and material reads are as coherent as the mesh; each
+// triangle's record goes to its sorted leaf position.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_shade_fill(const SceneDesc* scenes, SceneDesc D1) {
+  const SceneDesc& D = MULTI ? scenes[blockIdx.y] : D1;
+  const MeshShade& M = D.M;
+  float4* out = const_cast<float4*>(M.lshade);
+  if (!out) return;
+  const int n = D.B.n;
+  for (int t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) {
+    const int a = M.idx[3 * t], b = M.idx[3 * t + 1], c = M.idx[3 * t + 2];
+    const int mat = M.mat_id ? M.mat_id[t] : 0;
+    float4* r = out + 4 * (size_t)D.B.leaf_of[t];
+    r[0] = make_float4(M.nx[a], M.ny[a], M.nz[a], __int_as_float(mat));
+    r[1] = make_float4(M.nx[b], M.ny[b], M.nz[b], 0.f);
+    r[2] = make_float4(M.nx[c], M.ny[c], M.nz[c], 0.f);
+    r[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+template <bool MULTI>
+__global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, int bounce, int cur, int sample,
                                                        unsigned long long* counters) {
   __shared__ uint32_t s_tmp[8];
   const uint32_t n_in = W.counts[cur];
@@ -1309,29 +1416,27 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, int bounce, in
   d3 thr = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
   uint16_t sc = 0;
   if (live) {
-    sc = W.rscene[cur][i];
-    const SceneDesc& D = W.scenes[sc];
-    thr = ld3(W.thr + 3 * (size_t)p);
+    sc = MULTI ? W.rscene[cur][i] : 0;
+    const SceneDesc& D = scene_of<MULTI>(W, D1, sc);
+    thr = ldv(W.thr, p, W.npix);
     const int pos = W.hpos[i];
     if (W.rec) W.rec[rec_index(W, p, sample, bounce)] = make_int2(pos, -1);
     if (pos < 0) {
-      double* a = W.acc + 3 * (size_t)p;
-      st3(a, ld3(a) + rt::hadamard(thr, mk(0.9, 0.9, 0.9)));
+      stv(W.acc, p, W.npix, ldv(W.acc, p, W.npix) + rt::hadamard(thr, mk(0.9, 0.9, 0.9)));
     } else {
-      const double* rr = W.ray[cur] + 6 * (size_t)i;
-      d3 o = ld3(rr), d = ld3(rr + 3);
-      const double* hh = W.hit + 3 * (size_t)i;
+      d3 o = ldv(W.ray[cur], i, W.cap), d = ldv(W.ray[cur] + 3 * W.cap, i, W.cap);
+      const double hh[3] = {W.hit[i], W.hit[W.cap + i], W.hit[2 * W.cap + i]};
       uint32_t rng = W.rng[p];
-      const int tri = leaf_tri_of(D.B, pos);
-      const rt_material& m = mat_of(D.mats, D.P.nmat, D.M, tri);
-      S = shade_prep(D.P, m, shading_normal(D.M, tri, hh[1], hh[2]), o, d, hh[0], rng);
+      const LeafShade LS = leaf_shade(D.mats, D.P.nmat, D.M, D.B, pos);
+      const rt_material& m = *LS.m;
+      S = shade_prep(D.P, m, blend_normal(LS.na, LS.nb, LS.nc, hh[1], hh[2]), o, d, hh[0], rng);
       want_shadow = true;
       if (bounce < D.P.depth) {
         double u1 = rt::rand_unit(rng);
         double u2 = rt::rand_unit(rng);
         d3 N = S.N;
         if (rt::dot(N, d) > 0) N = mk(-N.x, -N.y, -N.z);
-        st3(W.thr + 3 * (size_t)p, rt::hadamard(thr, arr3(m.kd)));
+        stv(W.thr, p, W.npix, rt::hadamard(thr, arr3(m.kd)));
         no = S.q;
         nd = cosine_dir(N, u1, u2);
         want_next = true;
@@ -1343,27 +1448,25 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, int bounce, in
   if (want_shadow) {
     W.spix[ss] = p;
     W.sscene[ss] = sc;
-    double* e = W.shq + 16 * (size_t)ss;
-    st3(e + 0, S.q);
-    st3(e + 3, S.L);
-    st3(e + 6, S.I1);
-    st3(e + 9, S.ds);
-    st3(e + 12, thr);
-    e[15] = S.fo;
+    // shadow entry, component-major: q, L, I1, ds, thr, fo
+    double* e = W.shq + ss;
+    const size_t c = W.cap;
+    stv(e, 0, c, S.q);
+    stv(e + 3 * c, 0, c, S.L);
+    stv(e + 6 * c, 0, c, S.I1);
+    stv(e + 9 * c, 0, c, S.ds);
+    stv(e + 12 * c, 0, c, thr);
+    e[15 * c] = S.fo;
   }
   uint32_t ns = block_append(&W.counts[cur ^ 1], want_next, s_tmp);
   if (want_next) {
     W.pix[cur ^ 1][ns] = p;
     W.rscene[cur ^ 1][ns] = sc;
-    double* rr = W.ray[cur ^ 1] + 6 * (size_t)ns;
-    st3(rr, no);
-    st3(rr + 3, nd);
+    stv(W.ray[cur ^ 1], ns, W.cap, no);
+    stv(W.ray[cur ^ 1] + 3 * W.cap, ns, W.cap, nd);
   }
-  if (counters) {
-    // rays traced by the closest-hit pass of this bounce
-    uint64_t m = __ballot(live);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&counters[bounce == 0 ? 0 : 2], (unsigned long long)__popcll(m));
-  }
+  // rays traced by the closest-hit pass of this bounce
+  if (counters) rep_count(W.rep, bounce == 0 ? 0 : 2, live);
 }
 
 // Occluder code of a shadow query: -1 = unshadowed (factor 1), -2 = opaque
@@ -1371,32 +1474,40 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, int bounce, in
 // (factor kd % (1 - kt), kernels/Light.cu:17-26).
 __device__ __forceinline__ int occluder_code(const SceneDesc& D, d3 q, const HitRec& s) {
   if (s.pos < 0 || !(s.t < rt::length_f(arr3(D.P.lpos) - q)) || s.t < kRayEps) return -1;
-  const rt_material& sm = mat_of(D.mats, D.P.nmat, D.M, leaf_tri_of(D.B, s.pos));
+  const rt_material& sm = leaf_material(D.mats, D.P.nmat, D.M, D.B, s.pos);
   bool trans = (sm.kt[0] == 0) && (sm.kt[1] == 0) && (sm.kt[2] == 0);
   return trans ? s.pos : -2;
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, int sample, int bounce,
+template <bool MULTI>
+__global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, SceneDesc D1, int sample, int bounce,
                                                                unsigned long long* counters) {
   const uint32_t n_in = W.counts[2];
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   const bool live = i < n_in;
-  if (counters) {
-    uint64_t m = __ballot(live);
-    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&counters[1], (unsigned long long)__popcll(m));
-  }
+  if (counters) rep_count(W.rep, 1, live);
   if (!live) return;
-  const SceneDesc& D = W.scenes[W.sscene[i]];
-  const double* e = W.shq + 16 * (size_t)i;
-  d3 q = ld3(e + 0);
+  const SceneDesc& D = scene_of<MULTI>(W, D1, MULTI ? W.sscene[i] : 0);
+  const double* e = W.shq + i;
+  const size_t c = W.cap;
+  d3 q = ldv(e, 0, c);
   HitRec s;
   s.pos = W.spos[i];
   s.t = W.st[i];
-  d3 I = shade_finish(D.P, ld3(e + 6), e[15], ld3(e + 9), shadow_factor(D.P, D.B, D.mats, D.M, q, s.pos >= 0, s));
+  d3 I = shade_finish(D.P, ldv(e + 6 * c, 0, c), e[15 * c], ldv(e + 9 * c, 0, c),
+                      shadow_factor(D.P, D.B, D.mats, D.M, q, s.pos >= 0, s));
   uint32_t p = W.spix[i];
-  double* a = W.acc + 3 * (size_t)p;
-  st3(a, ld3(a) + rt::hadamard(ld3(e + 12), I));
+  stv(W.acc, p, W.npix, ldv(W.acc, p, W.npix) + rt::hadamard(ldv(e + 12 * c, 0, c), I));
   if (W.rec) W.rec[rec_index(W, p, sample, bounce)].y = occluder_code(D, q, s);
+}
+
+// counters[k] += sum of the replicas of ray counter k (k = 0, 1, 2).
+__global__ void k_wf_counters_out(const unsigned long long* rep, unsigned long long* counters) {
+  const int k = threadIdx.x;
+  if (k >= 3) return;
+  unsigned long long sum = 0;
+  for (int r = 0; r < kCtrReplicas; ++r) sum += rep[r * kCtrStride + k];
+  if (sum) atomicAdd(&counters[k], sum);
 }
 
 // Reset the listed counters and, if fetch != nullptr, `words` fetch
@@ -1408,20 +1519,22 @@ __global__ void k_wf_reset(uint32_t* counts, uint32_t mask, uint32_t* fetch, uin
 }
 
 // End of sample k: color += path sum for scenes still sampling.
-__global__ __launch_bounds__(kThreads) void k_wf_flush(WfState W, int k) {
+template <bool MULTI>
+__global__ __launch_bounds__(kThreads) void k_wf_flush(WfState W, SceneDesc D1, int k) {
   int s, px, py;
   uint32_t gp;
-  if (!tile_pixel(W, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
-  if (k >= W.scenes[s].P.spp) return;
-  st3(W.color + 3 * (size_t)gp, ld3(W.color + 3 * (size_t)gp) + ld3(W.acc + 3 * (size_t)gp));
+  if (!tile_pixel<MULTI>(W, D1, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
+  if (k >= scene_of<MULTI>(W, D1, s).P.spp) return;
+  stv(W.color, gp, W.npix, ldv(W.color, gp, W.npix) + ldv(W.acc, gp, W.npix));
 }
 
-__global__ __launch_bounds__(kThreads) void k_wf_resolve(WfState W) {
+template <bool MULTI>
+__global__ __launch_bounds__(kThreads) void k_wf_resolve(WfState W, SceneDesc D1) {
   int s, px, py;
   uint32_t gp;
-  if (!tile_pixel(W, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
-  const SceneDesc& D = W.scenes[s];
-  d3 c = ld3(W.color + 3 * (size_t)gp);
+  if (!tile_pixel<MULTI>(W, D1, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
+  const SceneDesc& D = scene_of<MULTI>(W, D1, s);
+  d3 c = ldv(W.color, gp, W.npix);
   if (D.rgb) write_pixel(D.P, D.rgb, px, py, c);
   if (W.color64) {
     size_t oi = 3 * ((size_t)(D.P.H - py - 1) * D.P.W + px);
@@ -1547,8 +1660,8 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
   BvhView* view = (BvhView*)(fetch + 512);
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, fetch, kFetchWords);
   k_put_view<<<1, 1, 0, s>>>(view, b->view());
-  RayIn R{rays->orig_dev, rays->dir_dev, 3, nullptr, (uint32_t)rays->n};
-  HitOut O{nullptr, hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev, nullptr};
+  RayIn R{rays->orig_dev, rays->dir_dev, 3, 1, nullptr, (uint32_t)rays->n};
+  HitOut O{nullptr, hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev, nullptr, 0};
   rt_tuning t = rt_default_tuning();
   int grid = persistent_grid(b->ctx ? b->ctx->num_cus : 256, t, (uint32_t)rays->n);
   if (mode == RT_TRACE_ANY)
@@ -1570,7 +1683,8 @@ static WfLayout wf_layout(size_t npix, size_t ntiles, size_t nscenes) {
   size_t cap = ntiles * kThreads;
   size_t sizes[] = {nscenes * sizeof(SceneDesc), nscenes * sizeof(BvhView), ntiles * 2, npix * 4, npix * 24,
                     npix * 24, npix * 24, cap * 48, cap * 48, cap * 4, cap * 4, cap * 2, cap * 2, cap * 4,
-                    cap * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, 256, kFetchWords * 4, kFetchWords * 4};
+                    cap * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, 256, kFetchWords * 4, kFetchWords * 4,
+                    kCtrReplicas * kCtrStride * 8};
   size_t off = 0;
   for (int k = 0; k < (int)(sizeof(sizes) / sizeof(sizes[0])); ++k) {
     L.off[k] = off;
@@ -1609,6 +1723,9 @@ static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes) {
   W.counts = (uint32_t*)(w + L.off[k++]);
   W.fetch[0] = (uint32_t*)(w + L.off[k++]);
   W.fetch[1] = (uint32_t*)(w + L.off[k++]);
+  W.rep = (unsigned long long*)(w + L.off[k++]);
+  W.npix = npix;
+  W.cap = ntiles * kThreads;
   return W;
 }
 
@@ -1628,10 +1745,10 @@ size_t rt_wavefront_bytes(int32_t width, int32_t height) {
 // Fill the scene table of a batch (host side; the caller copies it to the device).
 void rt_fill_scene(void* table_entry, const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                    const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
-                   uint32_t pix_off, uint32_t tile_off) {
+                   uint32_t pix_off, uint32_t tile_off, void* shade_table) {
   SceneDesc d;
   d.B = b->view();
-  d.M = MeshShade{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev};
+  d.M = MeshShade{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev, (const float4*)shade_table};
   d.mats = mats;
   d.P = rt_make_params(cam, light, opts, nmat);
   d.rgb = rgb;
@@ -1646,8 +1763,19 @@ size_t rt_bvh_view_bytes() { return sizeof(BvhView); }
 // tile->scene map are already in the workspace.
 hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
-                               uint32_t* overflow, hipStream_t s, int32_t* records, double* color64) {
+                               uint32_t* overflow, hipStream_t s, int32_t* records, double* color64,
+                               const void* single_scene) {
   WfState W = wf_state(workspace, npix, ntiles, nscenes);
+  const bool multi = single_scene == nullptr;
+  SceneDesc D1{};
+  if (!multi) D1 = *(const SceneDesc*)single_scene;
+#define RT_WF(KERNEL, ...)                                                  \
+  do {                                                                     \
+    if (multi)                                                             \
+      KERNEL<true><<<tiles, kThreads, 0, s>>>(W, D1, __VA_ARGS__);         \
+    else                                                                   \
+      KERNEL<false><<<tiles, kThreads, 0, s>>>(W, D1, __VA_ARGS__);        \
+  } while (0)
   W.rec = (int2*)records;
   W.rec_spp = max_spp;
   W.rec_depth = max_depth;
@@ -1656,40 +1784,55 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   unsigned long long* cnt = (unsigned long long*)counters;
   const size_t cap = ntiles * kThreads;
   const int tiles = (int)ntiles;
+  if (multi) {
+    const int gx = (int)std::max<size_t>(1, 4096 / nscenes);
+    k_shade_fill<true><<<dim3(gx, (unsigned)nscenes), 256, 0, s>>>(W.scenes, D1);
+  } else if (D1.M.lshade) {
+    k_shade_fill<false><<<std::min(cdiv(D1.B.n, 256), 4096), 256, 0, s>>>(W.scenes, D1);
+  }
   const int grid = persistent_grid(num_cus, t, (uint32_t)cap);
   uint32_t* ovf = overflow ? overflow : W.counts + 8;
-  k_wf_init<<<tiles, kThreads, 0, s>>>(W);
+  if (multi)
+    k_wf_init<true><<<tiles, kThreads, 0, s>>>(W, D1);
+  else
+    k_wf_init<false><<<tiles, kThreads, 0, s>>>(W, D1);
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W.fetch[0], 2 * kFetchWords);
   for (int it = 0; it < max_spp; ++it) {
-    k_wf_camera<<<tiles, kThreads, 0, s>>>(W, it, (uint32_t)cap);
+    RT_WF(k_wf_camera, it, (uint32_t)cap);
     int cur = 0;
     for (int bounce = 0; bounce <= max_depth; ++bounce) {
-      RayIn R{W.ray[cur], W.ray[cur] + 3, 6, &W.counts[cur], 0};
-      HitOut O{W.hpos, nullptr, nullptr, nullptr, nullptr, W.hit};
+      RayIn R{W.ray[cur], W.ray[cur] + 3 * cap, 1, cap, &W.counts[cur], 0};
+      HitOut O{W.hpos, nullptr, nullptr, nullptr, nullptr, W.hit, cap};
       // one scene: every ray uses views[0] (no per-ray scene lookup)
       launch_persistent<false>(t, grid, s, W.views, nscenes > 1 ? W.rscene[cur] : nullptr, R,
                                bounce == 0 ? 0.0 : kBounceTmin, O,
                                W.fetch[0], ovf, cnt);
-      k_wf_shade<<<tiles, kThreads, 0, s>>>(W, bounce, cur, it, cnt);
-      RayIn SR{W.shq, W.shq + 3, 16, &W.counts[2], 0};
-      HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr};
+      RT_WF(k_wf_shade, bounce, cur, it, cnt);
+      RayIn SR{W.shq, W.shq + 3 * cap, 1, cap, &W.counts[2], 0};
+      HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr, 0};
       launch_persistent<true>(t, grid, s, W.views, nscenes > 1 ? W.sscene : nullptr, SR, 0.0, SO, W.fetch[1], ovf,
                               cnt);
-      k_wf_shadow_finish<<<tiles, kThreads, 0, s>>>(W, it, bounce, cnt);
+      RT_WF(k_wf_shadow_finish, it, bounce, cnt);
       k_wf_reset<<<1, 256, 0, s>>>(W.counts, (1u << 2) | (1u << cur), W.fetch[0], 2 * kFetchWords);
       cur ^= 1;
     }
-    k_wf_flush<<<tiles, kThreads, 0, s>>>(W, it);
+    RT_WF(k_wf_flush, it);
   }
-  k_wf_resolve<<<tiles, kThreads, 0, s>>>(W);
+  if (multi)
+    k_wf_resolve<true><<<tiles, kThreads, 0, s>>>(W, D1);
+  else
+    k_wf_resolve<false><<<tiles, kThreads, 0, s>>>(W, D1);
+  if (cnt) k_wf_counters_out<<<1, 64, 0, s>>>(W.rep, cnt);
+#undef RT_WF
   return hipGetLastError();
 }
 
 hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
-                            uint64_t* counters, void* workspace, hipStream_t s, int32_t* records, double* color64) {
+                            uint64_t* counters, void* workspace, void* shade_table, hipStream_t s,
+                            int32_t* records, double* color64) {
   RenderParams P = rt_make_params(cam, light, opts, nmat);
-  MeshShade M{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev};
+  MeshShade M{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev, nullptr};
   const int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
   unsigned long long* cnt = (unsigned long long*)counters;
   uint32_t* ovf = b->misc + kMiscOverflow;
@@ -1700,6 +1843,7 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   SceneDesc d;
   d.B = b->view();
   d.M = M;
+  d.M.lshade = (const float4*)shade_table;
   d.mats = mats;
   d.P = P;
   d.rgb = rgb;
@@ -1709,7 +1853,7 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   k_wf_put_scene<<<cdiv(tiles, 256), 256, 0, s>>>((SceneDesc*)W.scenes, (BvhView*)W.views,
                                                   (uint16_t*)W.tile_scene, d, (uint32_t)tiles);
   return rt_launch_wavefront(workspace, (size_t)P.W * P.H, (size_t)tiles, 1, P.spp, P.depth,
-                             b->ctx ? b->ctx->num_cus : 256, nullptr, counters, ovf, s, records, color64);
+                             b->ctx ? b->ctx->num_cus : 256, nullptr, counters, ovf, s, records, color64, &d);
 }
 
 // ------------------------------------------------------------ differentiable render
@@ -1937,7 +2081,7 @@ hipError_t rt_launch_adjoint(const rt_bvh* b, const rt_mesh_view* m, const doubl
   rt_light L = *light;
   for (int k = 0; k < 3; ++k) L.color[k] *= light_scale;
   RenderParams P = rt_make_params(cam, &L, opts, m->n);
-  MeshShade M{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev};
+  MeshShade M{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev, nullptr};
   const int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
   k_adjoint<<<tiles, kThreads, 0, s>>>(b->view(), M, m->vx_dev, m->vy_dev, m->vz_dev, P, albedo, light_scale,
                                        (const int2*)records, dcolor, g_albedo, g_light, g_pos);
