@@ -641,10 +641,68 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
         }
       } else {
         uint32_t slot = atomicAdd(&misc[kMiscDeferred], 1u);
-        deferred[slot] = 2u * (uint32_t)split + (uint32_t)side;
+        if (slot < (uint32_t)n) deferred[slot] = 2u * (uint32_t)split + (uint32_t)side;
         break;
       }
     }
+  }
+}
+
+// The climber of hand-off `code` (split * 2 + side) from its deposit.
+__device__ __forceinline__ ClimbNode climber_of(uint32_t code, const float4* __restrict__ deposits) {
+  const int split = (int)(code >> 1), side = (int)(code & 1u);
+  ClimbNode c;
+  const float4* d = deposits + 2 * (size_t)code;
+  float4 q0 = d[0], q1 = d[1];
+  c.idx = __float_as_int(q0.x);
+  int far = __float_as_int(q0.y);
+  c.l = side == 0 ? far : split + 1;
+  c.r = side == 0 ? split : far;
+  c.b[0] = q0.z; c.b[1] = q0.w; c.b[2] = q1.x; c.b[3] = q1.y; c.b[4] = q1.z; c.b[5] = q1.w;
+  c.lidx = c.ridx = 0;
+  return c;
+}
+
+// One level of the top of the tree per launch, over all workgroups: each
+// hand-off counts its arrival at its split; the second arrival merges the
+// two deposits and hands the parent on to the next launch. Every deposit
+// such a launch reads was written by an earlier launch, and the arrival
+// counters are device atomics, so no cache maintenance is needed. A
+// balanced top halves the hand-offs per launch; what a bounded number of
+// launches leaves (a deep, unbalanced top) k_climb_top finishes.
+template <typename K>
+__global__ __launch_bounds__(256) void k_climb_pass(
+    const K* __restrict__ keys, int n, int32_t* __restrict__ parent, int32_t* __restrict__ left,
+    int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode,
+    float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, const uint32_t* __restrict__ in,
+    uint32_t* __restrict__ out, uint32_t* __restrict__ misc, int in_slot, int out_slot, int clear_slot) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) misc[clear_slot] = 0u;
+  const uint32_t count = min(misc[in_slot], (uint32_t)n);
+  for (uint32_t e = blockIdx.x * 256 + threadIdx.x; e < count; e += gridDim.x * 256) {
+    const uint32_t code = in[e];
+    const int split = (int)(code >> 1), side = (int)(code & 1u);
+    if (split >= n - 1) {
+      atomicAdd(&misc[kMiscBuildError], 1u);
+      continue;
+    }
+    if (atomicAdd(&arrivals[split], 1u) == 0u) continue;
+    ClimbNode c = climber_of(code, deposits);
+    merge_deposit(c, side == 0, deposits + 2 * (size_t)(code ^ 1u));
+    if (c.l == 0 && c.r == n - 1) {
+      write_node(c, 0, n, parent, left, right, node_box, tnode);
+      continue;
+    }
+    int side2;
+    const int split2 = climb_step<K>(c, keys, n, &side2, parent, left, right, node_box, tnode, deposits);
+    if (split2 < 0) {
+      atomicAdd(&misc[kMiscBuildError], 1u);
+      continue;
+    }
+    const uint32_t slot = atomicAdd(&misc[out_slot], 1u);
+    if (slot < (uint32_t)n)
+      out[slot] = 2u * (uint32_t)split2 + (uint32_t)side2;
+    else
+      atomicAdd(&misc[kMiscBuildError], 1u);  // cannot happen: hand-offs have disjoint leaf ranges
   }
 }
 
@@ -653,22 +711,12 @@ __global__ __launch_bounds__(kClimbTopThreads) void k_climb_top(
     const K* __restrict__ keys, int n, int32_t* __restrict__ parent, int32_t* __restrict__ left,
     int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode,
     float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, const uint32_t* __restrict__ deferred,
-    uint32_t* __restrict__ misc) {
-  const uint32_t count = misc[kMiscDeferred];
+    uint32_t* __restrict__ misc, int in_slot) {
+  const uint32_t count = min(misc[in_slot], (uint32_t)n);
   for (uint32_t e = threadIdx.x; e < count; e += kClimbTopThreads) {
     uint32_t code = deferred[e];
     int split = (int)(code >> 1), side = (int)(code & 1u);
-    ClimbNode c;
-    {
-      const float4* d = deposits + 2 * (size_t)code;
-      float4 q0 = d[0], q1 = d[1];
-      c.idx = __float_as_int(q0.x);
-      int far = __float_as_int(q0.y);
-      c.l = side == 0 ? far : split + 1;
-      c.r = side == 0 ? split : far;
-      c.b[0] = q0.z; c.b[1] = q0.w; c.b[2] = q1.x; c.b[3] = q1.y; c.b[4] = q1.z; c.b[5] = q1.w;
-      c.lidx = c.ridx = 0;
-    }
+    ClimbNode c = climber_of(code, deposits);
     for (int guard = 0;; ++guard) {
       if (split < 0 || guard >= n) {
         atomicAdd((uint32_t*)&misc[kMiscBuildError], 1u);
@@ -930,7 +978,32 @@ void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
   if (topology_changed) k_wide_parity<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->parent, b->wide_flag);
   k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide,
                                                b->wideq);
-  k_wide8_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
+  if (b->want_wide8)
+    k_wide8_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
+}
+
+// The top of the tree above the chunks: about log2(chunks) + 2 one-level
+// launches over the whole GPU, then one workgroup for whatever remains.
+template <typename K>
+void launch_climb_top(rt_bvh* b, const K* keys, hipStream_t s) {
+  const int n = b->n;
+  if (n <= 1) return;
+  const int chunks = cdiv(n, kClimbChunk);
+  int passes = 0;
+  if (chunks > 1) {
+    passes = 2;
+    while ((1 << (passes - 2)) < chunks && passes < 26) ++passes;
+  }
+  const int slots[3] = {kMiscDeferred, kMiscDeferredB, kMiscDeferredC};
+  uint32_t* lists[2] = {b->deferred, b->deferred2};
+  const int grid = std::max(1, std::min(cdiv(3 * chunks, 256), 1024));
+  for (int p = 0; p < passes; ++p)
+    k_climb_pass<K><<<grid, 256, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode, b->deposits,
+                                         b->arrivals, lists[p & 1], lists[(p + 1) & 1], b->misc, slots[p % 3],
+                                         slots[(p + 1) % 3], slots[(p + 2) % 3]);
+  k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
+                                                b->deposits, b->arrivals, lists[passes & 1], b->misc,
+                                                slots[passes % 3]);
 }
 
 template <typename K>
@@ -962,9 +1035,7 @@ hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
     k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
         keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->parent, b->left, b->right,
         b->node_box, b->tnode, b->ltri, b->deposits, b->arrivals, b->deferred, b->misc, b->leaf_of);
-    if (n > 1)
-      k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
-                                                    b->deposits, b->arrivals, b->deferred, b->misc);
+    launch_climb_top<K>(b, keys, s);
   }
   launch_wide(b, true, s);
   return hipGetLastError();
@@ -991,14 +1062,20 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
   k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
       keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->parent, b->left, b->right,
       b->node_box, b->tnode, b->ltri, b->deposits, b->arrivals, b->deferred, b->misc, nullptr);
-  if (n > 1)
-    k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
-                                                  b->deposits, b->arrivals, b->deferred, b->misc);
+  launch_climb_top<K>(b, keys, s);
   launch_wide(b, false, s);
   return hipGetLastError();
 }
 
 }  // namespace
+
+hipError_t rt_ensure_wide8(const rt_bvh* b, hipStream_t s) {
+  if (b->want_wide8) return hipSuccess;
+  b->want_wide8 = true;
+  if (b->n >= 2)
+    k_wide8_fill<<<cdiv(b->n - 1, 256), 256, 0, s>>>(b->n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
+  return hipGetLastError();
+}
 
 hipError_t rt_launch_build(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
   return b->bits == 30 ? build_typed<uint32_t>(b, m, s) : build_typed<uint64_t>(b, m, s);

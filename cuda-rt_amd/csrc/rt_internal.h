@@ -86,7 +86,8 @@ struct rt_bvh {
   uint32_t* status = nullptr;    // passes*ntiles*256 (contiguous after hist)
   uint32_t* arrivals = nullptr;  // n-1 arrival counters of the top-level climb (contiguous after status)
   float4* deposits = nullptr;    // (n-1)*2 child deposits, 32 B each
-  uint32_t* deferred = nullptr;  // n climb hand-offs to the top-level pass
+  uint32_t* deferred = nullptr;  // n climb hand-offs to the top-level passes
+  uint32_t* deferred2 = nullptr; // n: second hand-off list (the top-level passes alternate)
   uint32_t* fetch = nullptr;     // 1024 words: ray fetch counters of rt_trace
   void* keys[2] = {nullptr, nullptr};
   uint32_t* ids[2] = {nullptr, nullptr};
@@ -102,6 +103,9 @@ struct rt_bvh {
   float4* wide8 = nullptr;       // (n-1)*8
   uint8_t* wide_flag = nullptr;  // n-1: bit 0 = even depth (4-wide node), bit 1 = depth divisible by 3 (8-wide)
   int32_t* leaf_of = nullptr;    // n: triangle id -> sorted leaf position
+  // The 8-wide nodes are only kept up to date once a traversal has asked
+  // for them (rt_ensure_wide8); from then on every build and refit fills them.
+  mutable bool want_wide8 = false;
 
   BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide, wideq, wide8, leaf_of}; }
 };
@@ -114,6 +118,8 @@ constexpr int kMiscOverflow = 56;   // 1 word: traversal stack overflows
 constexpr int kMiscDeferred = 57;   // 1 word: number of climb hand-offs
 constexpr int kMiscBuildError = 58; // 1 word: climb consistency failures (must stay 0)
 constexpr int kMiscFetch = 59;      // 1 word: ray fetch counter of rt_trace
+constexpr int kMiscDeferredB = 60;  // 1 word each: hand-off counts of the top-level passes
+constexpr int kMiscDeferredC = 61;  //   (kMiscDeferred, kMiscDeferredB, kMiscDeferredC in rotation)
 constexpr int kMiscWords = 64;
 
 // Fused hierarchy + refit climb: one workgroup per chunk of sorted leaves.
@@ -132,6 +138,9 @@ rt_status rt_set_error(rt_ctx* ctx, rt_status s, const char* fmt, ...);
 // Launchers (rt_build.hip).
 hipError_t rt_launch_build(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
 hipError_t rt_launch_refit(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
+// Before the first 8-wide traversal of a BVH: fill its 8-wide nodes now and
+// keep them filled by every later build and refit.
+hipError_t rt_ensure_wide8(const rt_bvh* b, hipStream_t s);
 hipError_t rt_launch_export(const rt_bvh* b, const rt_bvh_export_bufs* dst, hipStream_t s);
 
 // Launchers (rt_trace.hip).

@@ -563,6 +563,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
   int pending = kNoNode;  // 4-wide traversal: a leaf waiting for the wave's next batch of leaf tests
   double bt = 0, bal = 0, bbe = 0;
   int bpos = -1;
+  float flim = __int_as_float(0x7f800000);  // 4-wide closest hit: fp32 pruning bound of the best hit so far
   uint32_t chunk_next = 0, chunk_end = 0;
   bool drained = false;
   // ray range split into kSegments contiguous segments of whole chunks
@@ -623,6 +624,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
         if (WIDE) rf = make_rayf(r, B.node_box);
         bpos = -1;
         bt = 0;
+        flim = __int_as_float(0x7f800000);
         double t0, t1;
         const float* rb = B.node_box;
         if (n <= 0 || !slab(r, ldgf(rb), ldgf(rb + 1), ldgf(rb + 2), ldgf(rb + 3), ldgf(rb + 4), ldgf(rb + 5), t0, t1)) {
@@ -702,6 +704,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
               bal = best.al;
               bbe = best.be;
               bpos = best.pos;
+              if (bpos >= 0) flim = f_up(bt + prune_margin(bt));
             }
           }
         }
@@ -741,7 +744,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
         bool ovf = false;
         int next = kNoNode;
         if (!ANY) {
-          const float lim = bpos >= 0 ? f_up(bt + prune_margin(bt)) : __int_as_float(0x7f800000);
+          const float lim = flim;
           const float low =
               tmin_phys > 0 ? f_down(tmin_phys - prune_margin(tmin_phys)) : __int_as_float(0xff800000);
           float key[NW];
@@ -750,7 +753,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
 #pragma unroll
           for (int j = 0; j < NW; ++j) {
             ref[j] = cref[j];
-            const bool h = ref[j] != kWideEmpty && t0[j] <= t1[j] && t0[j] <= lim && t1[j] >= low;
+            const bool h = (ref[j] != kWideEmpty) & (t0[j] <= t1[j]) & (t0[j] <= lim) & (t1[j] >= low);
             key[j] = h ? t0[j] : __int_as_float(0x7f800000);
             m += h ? 1 : 0;
           }

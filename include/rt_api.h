@@ -172,7 +172,10 @@ rt_status rt_trace(const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_rays *r
  * batches leaf tests across a wave and closest-hit pruning then depends on
  * which rays share a wave. Images and hits never depend on it.
  * RT_RENDER_WAVEFRONT keeps per-pixel state in a context-owned workspace
- * that the first such call for a given image size allocates. */
+ * that the first such call for a given image size allocates, and copies the
+ * mesh's normals and material ids into a per-leaf shading table (64 B per
+ * triangle, also context-owned) at the start of every call; renders that
+ * share a context must not overlap on different streams. */
 rt_status rt_render(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_material *mats_dev,
                     int32_t nmat, const rt_light *light, const rt_camera *cam, const rt_render_opts *opts,
                     float *rgb_dev, uint64_t *counters_dev, void *stream);
@@ -199,7 +202,9 @@ typedef struct {
   int32_t waves_per_simd;  /* persistent traversal workgroups per CU, 1..8 */
   int32_t node_layout;     /* 2: binary nodes, FP64 child tests, 64 B per node;
                               4: 4-wide nodes, fp32 conservative child tests, 128 B per node;
-                              5: 4-wide nodes with 8-bit quantized child boxes, 64 B per node (default) */
+                              5: 4-wide nodes with 8-bit quantized child boxes, 64 B per node (default);
+                              8: 8-wide nodes with 8-bit quantized child boxes, 128 B per node.
+                              Every layout gives the same hits and images. */
 } rt_tuning;
 
 typedef struct {
