@@ -59,6 +59,24 @@ def parse():
     return ap.parse_args()
 
 
+def committed_traffic(kind, args):
+    """HBM bytes per traversal launch of the default workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/traffic_soup1M.json),
+    scaled by the copy-kernel calibration recorded there; None for another
+    workload."""
+    path = os.path.join(ROOT, "profiles", "traffic_soup1M.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    w = rec["workload"]
+    if (w["tris"], w["width"], w["height"]) != (args.tris, args.width, args.height) or args.mode != "wavefront":
+        return None
+    k = rec["kernels"][kind]
+    return {"bytes": int(k["bytes_per_launch"]), "fetch_bytes": int(k["fetch_bytes"]),
+            "write_bytes": int(k["write_bytes"]), "source": "profiles/traffic_soup1M.json"}
+
+
 def calibrate_copy(mib):
     import torch
     a = torch.empty(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -372,12 +390,17 @@ def main():
     torch.cuda.synchronize()
     cnt.zero_()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    # HIP events around every traversal launch, on the stream the library
+    # launches on (the dominant kernel's live duration for the roofline)
+    ctx.set_kernel_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         step(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    kstats = ctx.kernel_stats()
+    ctx.set_kernel_timing(False)
     diag = bvh.diagnostics()
     build_ms = float(np.median([e[0].elapsed_time(e[1]) for e in evs]))
     render_ms = float(np.median([e[1].elapsed_time(e[2]) for e in evs]))
@@ -386,8 +409,25 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     value = (primary + bounce) / (ms_per_step * 1e-3) / 1e6
 
-    lane_ops = box_tests * OPS_PER_BOX_TEST * BOX_TEST_FACTOR + tests * OPS_PER_TRI_TEST * FP64_FACTOR
-    achieved = lane_ops / (render_ms * 1e-3) / 1e12
+    def kernel_roofline(name, ks):
+        # algorithmic lane-ops per launch / the launch's average duration
+        launches = max(ks["launches"], 1)
+        avg_ms = ks["ms"] / launches
+        ops = (ks["box_tests"] * OPS_PER_BOX_TEST * BOX_TEST_FACTOR + ks["tri_tests"] * OPS_PER_TRI_TEST * FP64_FACTOR)
+        ach = ops / launches / (avg_ms * 1e-3) / 1e12
+        return {"kernel": name, "bound": "valu", "achieved": round(ach, 3), "peak": FP64_LANE_OPS_PEAK_T,
+                "unit": "T lane-op/s", "frac": round(ach / FP64_LANE_OPS_PEAK_T, 4),
+                "avg_launch_ms": round(avg_ms, 4), "launches": ks["launches"],
+                "work_per_launch": {"box_tests": ks["box_tests"] // launches, "tri_tests": ks["tri_tests"] // launches,
+                                    "lane_ops": ops // launches, "ops_per_box_test": OPS_PER_BOX_TEST * BOX_TEST_FACTOR,
+                                    "ops_per_tri_test": OPS_PER_TRI_TEST * FP64_FACTOR}}
+
+    roof = kernel_roofline("k_trace_persistent<closest hit, 4-wide quantized> (one launch = one sample's primary rays)",
+                           kstats["closest"])
+    roof["traffic"] = committed_traffic("closest", args)
+    roof_any = kernel_roofline("k_trace_persistent<any hit, 4-wide quantized> (one launch = one sample's shadow rays)",
+                               kstats["any"])
+    roof_any["traffic"] = committed_traffic("any", args)
     build_bytes = BUILD_BYTES_PER_TRI_63 * args.tris
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
     out = {
@@ -407,18 +447,9 @@ def main():
                    "tris": args.tris, "width": args.width, "height": args.height, "spp": args.spp,
                    "morton_bits": 63, "aperture": 0.05, "focus": 1.5},
         "render_mode": args.mode,
-        "roofline": {
-            "kernel": "render (all traversal kernels of one frame)",
-            "bound": "valu",
-            "achieved": round(achieved, 3),
-            "peak": FP64_LANE_OPS_PEAK_T,
-            "unit": "T FP64 lane-op/s",
-            "frac": round(achieved / FP64_LANE_OPS_PEAK_T, 4),
-            "traffic": None,
-            "work": {"box_tests": box_tests, "tri_tests": tests, "lane_ops": lane_ops,
-                     "ops_per_box_test": OPS_PER_BOX_TEST * BOX_TEST_FACTOR,
-                     "ops_per_tri_test": OPS_PER_TRI_TEST * FP64_FACTOR},
-        },
+        "roofline": roof,
+        "roofline_any_hit": roof_any,
+        "render_work": {"box_tests": box_tests, "tri_tests": tests},
         "roofline_build": {
             "kernels": "k_init..k_climb_top (one rebuild)",
             "bound": "hbm",

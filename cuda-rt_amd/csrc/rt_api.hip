@@ -60,10 +60,57 @@ void rt_ctx_destroy(rt_ctx* ctx) {
   if (ctx->workspace) (void)hipFree(ctx->workspace);
   if (ctx->shade) (void)hipFree(ctx->shade);
   if (ctx->diff) (void)hipFree(ctx->diff);
+  for (auto& r : ctx->timer.recs) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (hipEvent_t e : ctx->timer.pool) (void)hipEventDestroy(e);
+  if (ctx->timer.dev_work) (void)hipFree(ctx->timer.dev_work);
   delete ctx;
 }
 
 const char* rt_last_error(const rt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+rt_status rt_ctx_set_kernel_timing(rt_ctx* ctx, int32_t enable) {
+  if (!ctx) return RT_ERR_INVALID;
+  if (enable && !ctx->timer.dev_work) {
+    void* p = nullptr;
+    if (hipMalloc(&p, 4 * sizeof(unsigned long long)) != hipSuccess) {
+      (void)hipGetLastError();
+      return rt_set_error(ctx, RT_ERR_OOM, "kernel timing counters");
+    }
+    RT_HIP(ctx, hipMemset(p, 0, 4 * sizeof(unsigned long long)));
+    ctx->timer.dev_work = (unsigned long long*)p;
+  }
+  ctx->timer.enabled = enable != 0;
+  return RT_OK;
+}
+
+rt_status rt_ctx_kernel_stats(rt_ctx* ctx, rt_kernel_stat out[2]) {
+  if (!ctx || !out) return RT_ERR_INVALID;
+  RT_HIP(ctx, hipDeviceSynchronize());
+  for (int k = 0; k < 2; ++k) out[k] = rt_kernel_stat{0, 0.0, 0, 0};
+  rt_kernel_timer& T = ctx->timer;
+  for (auto& r : T.recs) {
+    float ms = 0.f;
+    RT_HIP(ctx, hipEventElapsedTime(&ms, r.a, r.b));
+    out[r.kind].launches += 1;
+    out[r.kind].ms += ms;
+    T.pool.push_back(r.a);
+    T.pool.push_back(r.b);
+  }
+  T.recs.clear();
+  if (T.dev_work) {
+    unsigned long long w[4];
+    RT_HIP(ctx, hipMemcpy(w, T.dev_work, sizeof(w), hipMemcpyDeviceToHost));
+    RT_HIP(ctx, hipMemset(T.dev_work, 0, sizeof(w)));
+    out[0].box_tests = w[0];
+    out[0].tri_tests = w[1];
+    out[1].box_tests = w[2];
+    out[1].tri_tests = w[3];
+  }
+  return RT_OK;
+}
 
 static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, rt_bvh** out) {
   rt_bvh* b = new (std::nothrow) rt_bvh();
@@ -277,7 +324,7 @@ rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, co
     shade = ctx->shade;
   }
   RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, counters_dev, ws, shade,
-                               (hipStream_t)stream));
+                               (hipStream_t)stream, nullptr, nullptr, &ctx->timer));
   return RT_OK;
 }
 

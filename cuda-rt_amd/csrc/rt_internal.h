@@ -4,8 +4,24 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "../../include/rt_api.h"
+
+// Per-kernel timing of the wavefront traversal launches, for benchmarks:
+// while enabled, HIP events bracket each traversal launch on its stream and
+// the work counts are kept per kernel (dev_work: box and triangle tests of
+// the closest-hit traversal, then of the any-hit traversal).
+struct rt_kernel_timer {
+  bool enabled = false;
+  struct Rec {
+    int kind;  // 0 closest hit, 1 any hit
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> recs;
+  std::vector<hipEvent_t> pool;
+  unsigned long long* dev_work = nullptr;
+};
 
 struct rt_ctx {
   int32_t device = 0;
@@ -17,6 +33,7 @@ struct rt_ctx {
   size_t shade_bytes = 0;
   void* diff = nullptr;          // differentiable render: materials, identity ids, float image
   size_t diff_bytes = 0;
+  rt_kernel_timer timer;
 };
 
 // Device-side view of a built BVH, passed by value to the trace kernels.
@@ -153,7 +170,8 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
 hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
                             uint64_t* counters, void* workspace, void* shade_table, hipStream_t s,
-                            int32_t* records = nullptr, double* color64 = nullptr);
+                            int32_t* records = nullptr, double* color64 = nullptr,
+                            rt_kernel_timer* timer = nullptr);
 size_t rt_wavefront_bytes(int32_t width, int32_t height);
 // Per-leaf shading table of an n-triangle scene: 64 B per sorted leaf
 // (three vertex normals and the material id), refreshed by every render.
@@ -167,7 +185,8 @@ void rt_fill_scene(void* table_entry, const rt_bvh* b, const rt_mesh_view* m, co
 hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
                                uint32_t* overflow, hipStream_t s, int32_t* records = nullptr,
-                               double* color64 = nullptr, const void* single_scene = nullptr);
+                               double* color64 = nullptr, const void* single_scene = nullptr,
+                               rt_kernel_timer* timer = nullptr);
 // Differentiable render (rt_diff.hip).
 hipError_t rt_launch_zero_f64(double* p, size_t n, hipStream_t s);
 hipError_t rt_launch_resolve(const float* big, int32_t W, int32_t H, int32_t ss, bool reference_divisor, float* out,

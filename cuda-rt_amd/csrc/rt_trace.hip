@@ -328,8 +328,10 @@ __device__ __forceinline__ int32_t leaf_tri_of(const BvhView& B, int pos) {
   return __float_as_int(ldgf((const float*)(B.ltri + 3 * (size_t)pos + 2) + 1));
 }
 
-__device__ __forceinline__ void flush_stats(unsigned long long* counters, const TravStats& st) {
-  if (!counters) return;
+// Work counts of a traversal launch: child-box tests to stats[0], triangle
+// tests to stats[1] of replica (workgroup & 63), replicas 16 words apart.
+__device__ __forceinline__ void flush_stats(unsigned long long* stats, const TravStats& st) {
+  if (!stats) return;
   unsigned long long vis = st.visits, tst = st.tests;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -337,8 +339,9 @@ __device__ __forceinline__ void flush_stats(unsigned long long* counters, const 
     tst += __shfl_xor(tst, off);
   }
   if ((threadIdx.x & 63) == 0) {
-    if (vis) atomicAdd(&counters[3], vis);
-    if (tst) atomicAdd(&counters[4], tst);
+    unsigned long long* r = stats + (blockIdx.x & 63) * 16;
+    if (vis) atomicAdd(&r[0], vis);
+    if (tst) atomicAdd(&r[1], tst);
   }
 }
 
@@ -542,7 +545,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
                                                                const uint16_t* __restrict__ scene, RayIn R,
                                                                double tmin_phys, HitOut O, uint32_t* fetch_ctr,
                                                                uint32_t fetch_chunk, uint32_t* overflow,
-                                                               unsigned long long* counters) {
+                                                               unsigned long long* stats) {
   constexpr bool WIDE = LAYOUT != 2, QUANT = LAYOUT == 5 || LAYOUT == 8;
   __shared__ int s_stack[RING * kThreads];
   const uint32_t nrays = R.count_dev ? *R.count_dev : R.count;
@@ -893,7 +896,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
       active = false;
     }
   }
-  flush_stats(counters, st);
+  flush_stats(stats, st);
 }
 
 // ------------------------------------------------------------ trace kernel
@@ -1264,10 +1267,11 @@ struct WfState {
   unsigned long long* rep;  // kCtrReplicas x kCtrStride ray counters, summed into the caller's at the end
 };
 
-// Ray counts of the shading passes go to one of kCtrReplicas copies of the
-// counters (by workgroup), each on its own 128-byte line: all waves of such
-// a pass finish together, and one shared address would serialise their
-// atomics.
+// Ray and work counts go to one of kCtrReplicas copies of the counters (by
+// workgroup), each on its own 128-byte line: all waves of a shading pass
+// finish together, and one shared address would serialise their atomics.
+// Words of a replica: 0 primary, 1 shadow, 2 bounce rays; 3, 4 box and
+// triangle tests of the closest-hit traversal; 5, 6 those of the any-hit one.
 constexpr int kCtrReplicas = 64;
 constexpr int kCtrStride = 16;
 __device__ __forceinline__ void rep_count(unsigned long long* rep, int k, bool pred) {
@@ -1504,13 +1508,27 @@ __global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, SceneD
   if (W.rec) W.rec[rec_index(W, p, sample, bounce)].y = occluder_code(D, q, s);
 }
 
-// counters[k] += sum of the replicas of ray counter k (k = 0, 1, 2).
-__global__ void k_wf_counters_out(const unsigned long long* rep, unsigned long long* counters) {
+// Sum the replicas: counters[0..2] += the ray counts, counters[3..4] += the
+// work counts of both traversals; per_kernel[0..3] (if given) += the work
+// counts of the closest-hit and the any-hit traversal apart.
+__global__ void k_wf_counters_out(const unsigned long long* rep, unsigned long long* counters,
+                                  unsigned long long* per_kernel) {
+  __shared__ unsigned long long s_sum[8];
   const int k = threadIdx.x;
-  if (k >= 3) return;
-  unsigned long long sum = 0;
-  for (int r = 0; r < kCtrReplicas; ++r) sum += rep[r * kCtrStride + k];
-  if (sum) atomicAdd(&counters[k], sum);
+  if (k < 7) {
+    unsigned long long sum = 0;
+    for (int r = 0; r < kCtrReplicas; ++r) sum += rep[r * kCtrStride + k];
+    s_sum[k] = sum;
+  }
+  __syncthreads();
+  if (k == 0 && counters) {
+    for (int j = 0; j < 3; ++j)
+      if (s_sum[j]) atomicAdd(&counters[j], s_sum[j]);
+    if (s_sum[3] + s_sum[5]) atomicAdd(&counters[3], s_sum[3] + s_sum[5]);
+    if (s_sum[4] + s_sum[6]) atomicAdd(&counters[4], s_sum[4] + s_sum[6]);
+  }
+  if (k == 1 && per_kernel)
+    for (int j = 0; j < 4; ++j) per_kernel[j] += s_sum[3 + j];
 }
 
 // Reset the listed counters and, if fetch != nullptr, `words` fetch
@@ -1631,16 +1649,16 @@ static int persistent_grid(int num_cus, const rt_tuning& t, uint32_t max_rays) {
 template <bool ANY>
 static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const BvhView* views,
                               const uint16_t* scene, RayIn R, double tmin, HitOut O, uint32_t* fetch,
-                              uint32_t* overflow, unsigned long long* counters) {
+                              uint32_t* overflow, unsigned long long* stats) {
   const uint32_t fc = t.fetch_chunk;
 #define RT_LAUNCH(RINGV, LAYOUTV)                                                                                  \
   do {                                                                                                              \
     if (scene)                                                                                                      \
       k_trace_persistent<ANY, RINGV, LAYOUTV, true>                                                                 \
-          <<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, counters);                     \
+          <<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                        \
     else                                                                                                            \
       k_trace_persistent<ANY, RINGV, LAYOUTV, false>                                                                \
-          <<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, counters);                     \
+          <<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                        \
   } while (0)
   if (t.node_layout == 8) {
     if (t.stack_ring == 16) RT_LAUNCH(16, 8); else RT_LAUNCH(32, 8);
@@ -1762,13 +1780,33 @@ void rt_fill_scene(void* table_entry, const rt_bvh* b, const rt_mesh_view* m, co
 size_t rt_scene_desc_bytes() { return sizeof(SceneDesc); }
 size_t rt_bvh_view_bytes() { return sizeof(BvhView); }
 
+// Events of one timed traversal launch (recycled by rt_ctx_kernel_stats).
+static hipError_t timer_mark(rt_kernel_timer* T, int kind, bool start, hipStream_t s) {
+  if (start) {
+    hipEvent_t ev[2];
+    for (hipEvent_t& e : ev) {
+      if (T->pool.empty()) {
+        hipError_t err = hipEventCreate(&e);
+        if (err != hipSuccess) return err;
+      } else {
+        e = T->pool.back();
+        T->pool.pop_back();
+      }
+    }
+    T->recs.push_back({kind, ev[0], ev[1]});
+    return hipEventRecord(ev[0], s);
+  }
+  return hipEventRecord(T->recs.back().b, s);
+}
+
 // The wavefront passes over a batch whose scene table, views and
 // tile->scene map are already in the workspace.
 hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
                                uint32_t* overflow, hipStream_t s, int32_t* records, double* color64,
-                               const void* single_scene) {
+                               const void* single_scene, rt_kernel_timer* timer) {
   WfState W = wf_state(workspace, npix, ntiles, nscenes);
+  rt_kernel_timer* T = timer && timer->enabled ? timer : nullptr;
   const bool multi = single_scene == nullptr;
   SceneDesc D1{};
   if (!multi) D1 = *(const SceneDesc*)single_scene;
@@ -1787,6 +1825,12 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   unsigned long long* cnt = (unsigned long long*)counters;
   const size_t cap = ntiles * kThreads;
   const int tiles = (int)ntiles;
+  const bool work = cnt || T;  // keep the replicated counts
+#define RT_TRY(X)                          \
+  do {                                     \
+    hipError_t e_ = (X);                   \
+    if (e_ != hipSuccess) return e_;       \
+  } while (0)
   if (multi) {
     const int gx = (int)std::max<size_t>(1, 4096 / nscenes);
     k_shade_fill<true><<<dim3(gx, (unsigned)nscenes), 256, 0, s>>>(W.scenes, D1);
@@ -1807,14 +1851,17 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
       RayIn R{W.ray[cur], W.ray[cur] + 3 * cap, 1, cap, &W.counts[cur], 0};
       HitOut O{W.hpos, nullptr, nullptr, nullptr, nullptr, W.hit, cap};
       // one scene: every ray uses views[0] (no per-ray scene lookup)
+      if (T) RT_TRY(timer_mark(T, 0, true, s));
       launch_persistent<false>(t, grid, s, W.views, nscenes > 1 ? W.rscene[cur] : nullptr, R,
-                               bounce == 0 ? 0.0 : kBounceTmin, O,
-                               W.fetch[0], ovf, cnt);
+                               bounce == 0 ? 0.0 : kBounceTmin, O, W.fetch[0], ovf, work ? W.rep + 3 : nullptr);
+      if (T) RT_TRY(timer_mark(T, 0, false, s));
       RT_WF(k_wf_shade, bounce, cur, it, cnt);
       RayIn SR{W.shq, W.shq + 3 * cap, 1, cap, &W.counts[2], 0};
       HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr, 0};
+      if (T) RT_TRY(timer_mark(T, 1, true, s));
       launch_persistent<true>(t, grid, s, W.views, nscenes > 1 ? W.sscene : nullptr, SR, 0.0, SO, W.fetch[1], ovf,
-                              cnt);
+                              work ? W.rep + 5 : nullptr);
+      if (T) RT_TRY(timer_mark(T, 1, false, s));
       RT_WF(k_wf_shadow_finish, it, bounce, cnt);
       k_wf_reset<<<1, 256, 0, s>>>(W.counts, (1u << 2) | (1u << cur), W.fetch[0], 2 * kFetchWords);
       cur ^= 1;
@@ -1825,15 +1872,16 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     k_wf_resolve<true><<<tiles, kThreads, 0, s>>>(W, D1);
   else
     k_wf_resolve<false><<<tiles, kThreads, 0, s>>>(W, D1);
-  if (cnt) k_wf_counters_out<<<1, 64, 0, s>>>(W.rep, cnt);
+  if (work) k_wf_counters_out<<<1, 64, 0, s>>>(W.rep, cnt, T ? T->dev_work : nullptr);
 #undef RT_WF
+#undef RT_TRY
   return hipGetLastError();
 }
 
 hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
                             uint64_t* counters, void* workspace, void* shade_table, hipStream_t s,
-                            int32_t* records, double* color64) {
+                            int32_t* records, double* color64, rt_kernel_timer* timer) {
   RenderParams P = rt_make_params(cam, light, opts, nmat);
   MeshShade M{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev, nullptr};
   const int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
@@ -1856,7 +1904,8 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   k_wf_put_scene<<<cdiv(tiles, 256), 256, 0, s>>>((SceneDesc*)W.scenes, (BvhView*)W.views,
                                                   (uint16_t*)W.tile_scene, d, (uint32_t)tiles);
   return rt_launch_wavefront(workspace, (size_t)P.W * P.H, (size_t)tiles, 1, P.spp, P.depth,
-                             b->ctx ? b->ctx->num_cus : 256, nullptr, counters, ovf, s, records, color64, &d);
+                             b->ctx ? b->ctx->num_cus : 256, nullptr, counters, ovf, s, records, color64, &d,
+                             timer);
 }
 
 // ------------------------------------------------------------ differentiable render

@@ -71,7 +71,13 @@ EXPORTED = [
     "rt_version", "rt_ctx_create", "rt_ctx_destroy", "rt_last_error", "rt_bvh_build", "rt_bvh_rebuild",
     "rt_bvh_refit", "rt_bvh_export", "rt_bvh_scene_bounds", "rt_bvh_num_triangles", "rt_bvh_diagnostics", "rt_bvh_destroy",
     "rt_trace", "rt_render", "rt_gen_soup", "rt_gen_icosphere_counts", "rt_gen_icosphere", "rt_gen_jitter",
+    "rt_ctx_set_kernel_timing", "rt_ctx_kernel_stats",
 ]
+
+
+class KernelStat(ctypes.Structure):
+    _fields_ = [("launches", ctypes.c_uint64), ("ms", ctypes.c_double), ("box_tests", ctypes.c_uint64),
+                ("tri_tests", ctypes.c_uint64)]
 
 _lib = None
 
@@ -107,6 +113,8 @@ def lib():
     L.rt_gen_icosphere.argtypes = [ctypes.c_int32, ctypes.c_double, c_d, ctypes.c_double, ctypes.c_uint64] + \
         [c_f] * 6 + [c_i32]
     L.rt_gen_jitter.argtypes = [ctypes.c_int32, c_f, c_f, c_f, ctypes.c_double, ctypes.c_uint64, c_f, c_f, c_f]
+    L.rt_ctx_set_kernel_timing.argtypes = [vp, ctypes.c_int32]
+    L.rt_ctx_kernel_stats.argtypes = [vp, ctypes.POINTER(KernelStat)]
     _lib = L
     _bind_batch(L)
     return L
@@ -273,6 +281,17 @@ class Context:
     def check(self, st):
         if st != RT_OK:
             raise RtError(st, self.L.rt_last_error(self.h).decode())
+
+    def set_kernel_timing(self, enable):
+        """Bracket each wavefront traversal launch with HIP events (rt_ctx_set_kernel_timing)."""
+        self.check(self.L.rt_ctx_set_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_stats(self):
+        """Totals since the last call: {'closest': {...}, 'any': {...}} (rt_ctx_kernel_stats)."""
+        out = (KernelStat * 2)()
+        self.check(self.L.rt_ctx_kernel_stats(self.h, out))
+        return {k: {"launches": int(o.launches), "ms": float(o.ms), "box_tests": int(o.box_tests),
+                    "tri_tests": int(o.tri_tests)} for k, o in zip(("closest", "any"), out)}
 
     def close(self):
         if self.h:

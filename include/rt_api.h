@@ -180,6 +180,21 @@ rt_status rt_render(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, co
                     int32_t nmat, const rt_light *light, const rt_camera *cam, const rt_render_opts *opts,
                     float *rgb_dev, uint64_t *counters_dev, void *stream);
 
+/* Per-kernel timing for benchmarks (no reference counterpart). While
+ * enabled, every closest-hit and any-hit traversal launch of rt_render and
+ * rt_batch_render on this context is bracketed by HIP events on its stream,
+ * and the traversals' work counts are also kept per kernel.
+ * rt_ctx_kernel_stats synchronises the device, returns the totals since the
+ * last call (out[0] closest hit, out[1] any hit) and resets them. */
+typedef struct {
+  uint64_t launches;
+  double ms;            /* summed launch durations */
+  uint64_t box_tests;   /* child-box tests */
+  uint64_t tri_tests;   /* triangle tests */
+} rt_kernel_stat;
+rt_status rt_ctx_set_kernel_timing(rt_ctx *ctx, int32_t enable);
+rt_status rt_ctx_kernel_stats(rt_ctx *ctx, rt_kernel_stat out[2]);
+
 /* Supersampling resolve: AverageSuperSamplingKernel (kernels/scene.cu:127-155).
  * big_dev is a (width*ss) x (height*ss) image as rt_render writes it (3
  * floats per pixel, row-flipped); out_dev receives width x height. Each
