@@ -411,7 +411,9 @@ def main():
 
     def kernel_roofline(name, ks):
         # algorithmic lane-ops per launch / the launch's average duration
-        launches = max(ks["launches"], 1)
+        if ks["launches"] == 0:
+            return None
+        launches = ks["launches"]
         avg_ms = ks["ms"] / launches
         ops = (ks["box_tests"] * OPS_PER_BOX_TEST * BOX_TEST_FACTOR + ks["tri_tests"] * OPS_PER_TRI_TEST * FP64_FACTOR)
         ach = ops / launches / (avg_ms * 1e-3) / 1e12
@@ -424,10 +426,18 @@ def main():
 
     roof = kernel_roofline("k_trace_persistent<closest hit, 4-wide quantized> (one launch = one sample's primary rays)",
                            kstats["closest"])
-    roof["traffic"] = committed_traffic("closest", args)
     roof_any = kernel_roofline("k_trace_persistent<any hit, 4-wide quantized> (one launch = one sample's shadow rays)",
                                kstats["any"])
-    roof_any["traffic"] = committed_traffic("any", args)
+    if roof is None:
+        # megakernel: one k_render launch per step does all traversal and shading
+        ops = box_tests * OPS_PER_BOX_TEST * BOX_TEST_FACTOR + tests * OPS_PER_TRI_TEST * FP64_FACTOR
+        ach = ops / (render_ms * 1e-3) / 1e12
+        roof = {"kernel": "k_render (megakernel, one launch per step)", "bound": "valu", "achieved": round(ach, 3),
+                "peak": FP64_LANE_OPS_PEAK_T, "unit": "T lane-op/s", "frac": round(ach / FP64_LANE_OPS_PEAK_T, 4),
+                "avg_launch_ms": round(render_ms, 4), "traffic": None}
+    else:
+        roof["traffic"] = committed_traffic("closest", args)
+        roof_any["traffic"] = committed_traffic("any", args)
     build_bytes = BUILD_BYTES_PER_TRI_63 * args.tris
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
     out = {

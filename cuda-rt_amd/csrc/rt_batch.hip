@@ -140,8 +140,8 @@ rt_status rt_batch_rebuild(rt_batch* b, void* stream) {
 
 rt_status rt_batch_render(rt_batch* b, uint64_t* counters_dev, void* stream) {
   if (!b) return RT_ERR_INVALID;
-  if (b->tuning.node_layout == 8)
-    for (const rt_batch_item& it : b->items) RT_HIP(b->ctx, rt_ensure_wide8(it.bvh, (hipStream_t)stream));
+  for (const rt_batch_item& it : b->items)
+    RT_HIP(b->ctx, rt_ensure_layout(it.bvh, b->tuning.node_layout, (hipStream_t)stream));
   RT_HIP(b->ctx, rt_launch_wavefront(b->ws, b->npix, b->ntiles, b->nscenes, b->max_spp, b->max_depth,
                                      b->ctx->num_cus, &b->tuning, counters_dev, nullptr, (hipStream_t)stream,
                                      nullptr, nullptr, nullptr, &b->ctx->timer));

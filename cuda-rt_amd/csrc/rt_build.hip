@@ -46,11 +46,13 @@ __global__ void k_init(uint32_t* __restrict__ misc, uint32_t* __restrict__ zero,
 // with fminf/fmaxf. If with_bounds, each workgroup also reduces its boxes
 // (wave shuffles, then LDS) and adds one ordered-integer atomic per axis to
 // one of kBoundsReplicas accumulator sets (kernels/scene.cu:72-85). NaN
-// coordinates are ignored, as fminf/fmaxf ignore them.
+// coordinates are ignored, as fminf/fmaxf ignore them. trirec receives each
+// triangle's leaf record in triangle order (the climb then copies one
+// record per sorted leaf instead of gathering index and vertices).
 __global__ __launch_bounds__(256) void k_aabb_bounds(const float* __restrict__ vx, const float* __restrict__ vy,
                                                      const float* __restrict__ vz, const int32_t* __restrict__ idx,
-                                                     int n, float* __restrict__ tri_box, uint32_t* __restrict__ misc,
-                                                     int with_bounds) {
+                                                     int n, float* __restrict__ tri_box, float4* __restrict__ trirec,
+                                                     uint32_t* __restrict__ misc, int with_bounds) {
   float acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
   for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     int a = idx[3 * i + 0], b = idx[3 * i + 1], c = idx[3 * i + 2];
@@ -62,6 +64,10 @@ __global__ __launch_bounds__(256) void k_aabb_bounds(const float* __restrict__ v
     float* o = tri_box + 6 * (size_t)i;
 #pragma unroll
     for (int k = 0; k < 6; ++k) o[k] = bb[k];
+    float4* L = trirec + 3 * (size_t)i;
+    L[0] = make_float4(ax, ay, az, bx);
+    L[1] = make_float4(by, bz, cx, cy);
+    L[2] = make_float4(cz, __int_as_float(i), 0.f, 0.f);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       acc[k] = fminf(acc[k], bb[k]);
@@ -583,9 +589,8 @@ __device__ __forceinline__ int climb_step(ClimbNode& c, const K* __restrict__ ke
 
 template <typename K>
 __global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
-    const K* __restrict__ keys, const uint32_t* __restrict__ ids, int n, const float* __restrict__ tri_box,
-    const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
-    const int32_t* __restrict__ idx, int32_t* __restrict__ parent, int32_t* __restrict__ left,
+    const K* __restrict__ keys, const uint32_t* __restrict__ ids, int n, const float4* __restrict__ trirec,
+    int32_t* __restrict__ parent, int32_t* __restrict__ left,
     int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode, float4* __restrict__ ltri,
     float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, uint32_t* __restrict__ deferred,
     uint32_t* __restrict__ misc, int32_t* __restrict__ leaf_of) {
@@ -593,18 +598,24 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
   const int s0 = blockIdx.x * kClimbChunk;
   const int e0 = min(n, s0 + kClimbChunk);
   for (int k = threadIdx.x; k < kClimbChunk; k += kClimbThreads) s_arr[k] = 0u;
-  // Leaves: node n-1+i = the triangle's box, plus its vertex record.
+  // Leaves: node n-1+i = the triangle's box, plus its vertex record (the
+  // box formed from the record's vertices exactly as k_aabb_bounds forms it).
   for (int i = s0 + threadIdx.x; i < e0; i += kClimbThreads) {
     int tri = (int)ids[i];
-    const float* src = tri_box + 6 * (size_t)tri;
-    float* dst = node_box + 6 * (size_t)(n - 1 + i);
-#pragma unroll
-    for (int k = 0; k < 6; ++k) dst[k] = src[k];
-    int a = idx[3 * tri + 0], b = idx[3 * tri + 1], c = idx[3 * tri + 2];
+    const float4* R = trirec + 3 * (size_t)tri;
+    const float4 q0 = R[0], q1 = R[1], q2 = R[2];
     float4* L = ltri + 3 * (size_t)i;
-    L[0] = make_float4(vx[a], vy[a], vz[a], vx[b]);
-    L[1] = make_float4(vy[b], vz[b], vx[c], vy[c]);
-    L[2] = make_float4(vz[c], __int_as_float(tri), 0.f, 0.f);
+    L[0] = q0;
+    L[1] = q1;
+    L[2] = q2;
+    const float ax = q0.x, ay = q0.y, az = q0.z, bx = q0.w, by = q1.x, bz = q1.y, cx = q1.z, cy = q1.w, cz = q2.x;
+    float* dst = node_box + 6 * (size_t)(n - 1 + i);
+    dst[0] = fminf(cx, fminf(ax, bx));
+    dst[1] = fminf(cy, fminf(ay, by));
+    dst[2] = fminf(cz, fminf(az, bz));
+    dst[3] = fmaxf(cx, fmaxf(ax, bx));
+    dst[4] = fmaxf(cy, fmaxf(ay, by));
+    dst[5] = fmaxf(cz, fmaxf(az, bz));
     if (leaf_of) leaf_of[tri] = i;  // only a build reorders the leaves
   }
   if (s0 == 0 && threadIdx.x == 0) parent[0] = -1;
@@ -615,7 +626,7 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
     c.l = c.r = i;
     c.idx = n - 1 + i;
     c.lidx = c.ridx = 0;
-    const float* src = tri_box + 6 * (size_t)ids[i];
+    const float* src = node_box + 6 * (size_t)(n - 1 + i);  // written above by this thread
 #pragma unroll
     for (int k = 0; k < 6; ++k) c.b[k] = src[k];
     for (int guard = 0;; ++guard) {
@@ -827,13 +838,18 @@ __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restr
       ref[j] = kWideEmpty;
     }
   }
-  float4* w = wide + 8 * (size_t)i;
-  for (int a = 0; a < 3; ++a) {
-    w[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
-    w[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+  const float4 refs =
+      make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
+  if (wide) {
+    float4* w = wide + 8 * (size_t)i;
+    for (int a = 0; a < 3; ++a) {
+      w[a] = make_float4(lo[a][0], lo[a][1], lo[a][2], lo[a][3]);
+      w[3 + a] = make_float4(hi[a][0], hi[a][1], hi[a][2], hi[a][3]);
+    }
+    w[6] = refs;
+    w[7] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
-  w[6] = make_float4(__int_as_float(ref[0]), __int_as_float(ref[1]), __int_as_float(ref[2]), __int_as_float(ref[3]));
-  w[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (!wideq) return;
   // quantized form: corner = the node's box minimum, per axis the smallest
   // power-of-two step with 255 steps covering the box
   const float* nb = node_box + 6 * (size_t)i;
@@ -871,7 +887,7 @@ __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restr
   q[0] = make_float4(nb[0], nb[1], nb[2], __uint_as_float(ebits));
   q[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]), 0.f);
   q[2] = make_float4(__uint_as_float(qhi[0]), __uint_as_float(qhi[1]), __uint_as_float(qhi[2]), 0.f);
-  q[3] = w[6];
+  q[3] = refs;
 }
 
 // Quantize up to W child boxes against the node box nb: per axis the
@@ -976,8 +992,8 @@ void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
   const int n = b->n;
   if (n < 2) return;
   if (topology_changed) k_wide_parity<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->parent, b->wide_flag);
-  k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide,
-                                               b->wideq);
+  k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag,
+                                               b->want_wide4 ? b->wide : nullptr, b->wideq);
   if (b->want_wide8)
     k_wide8_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
 }
@@ -1014,7 +1030,7 @@ hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
   size_t zero_words = (size_t)P * 256 + (size_t)P * b->ntiles * 256 + (size_t)(n > 1 ? n - 1 : 1);
   k_init<<<min(cdiv((long)zero_words, 256), 2048), 256, 0, s>>>(b->misc, b->hist, zero_words, 1);
   k_aabb_bounds<<<min(cdiv(n, 256), 1024), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box,
-                                                        b->misc, 1);
+                                                        b->trirec, b->misc, 1);
   int per_block = 4096;
   k_morton<K><<<cdiv(n, per_block), 256, 0, s>>>(b->tri_box, n, b->misc, (K*)b->keys[0], b->hist, per_block);
   for (int p = 0; p < P; ++p) {
@@ -1033,8 +1049,8 @@ hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
       k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
   } else {
     k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
-        keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->parent, b->left, b->right,
-        b->node_box, b->tnode, b->ltri, b->deposits, b->arrivals, b->deferred, b->misc, b->leaf_of);
+        keys, b->ids[0], n, b->trirec, b->parent, b->left, b->right, b->node_box, b->tnode, b->ltri, b->deposits,
+        b->arrivals, b->deferred, b->misc, b->leaf_of);
     launch_climb_top<K>(b, keys, s);
   }
   launch_wide(b, true, s);
@@ -1057,11 +1073,11 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
   size_t zero_words = (size_t)(n > 1 ? n - 1 : 1);
   k_init<<<min(cdiv((long)zero_words, 256), 2048), 256, 0, s>>>(b->misc, b->arrivals, zero_words, 0);
   k_aabb_bounds<<<min(cdiv(n, 256), 1024), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box,
-                                                        b->misc, 0);
+                                                        b->trirec, b->misc, 0);
   const K* keys = (const K*)b->keys[0];
   k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
-      keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->parent, b->left, b->right,
-      b->node_box, b->tnode, b->ltri, b->deposits, b->arrivals, b->deferred, b->misc, nullptr);
+      keys, b->ids[0], n, b->trirec, b->parent, b->left, b->right, b->node_box, b->tnode, b->ltri, b->deposits,
+      b->arrivals, b->deferred, b->misc, nullptr);
   launch_climb_top<K>(b, keys, s);
   launch_wide(b, false, s);
   return hipGetLastError();
@@ -1069,11 +1085,17 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
 
 }  // namespace
 
-hipError_t rt_ensure_wide8(const rt_bvh* b, hipStream_t s) {
-  if (b->want_wide8) return hipSuccess;
-  b->want_wide8 = true;
-  if (b->n >= 2)
-    k_wide8_fill<<<cdiv(b->n - 1, 256), 256, 0, s>>>(b->n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
+hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s) {
+  const int n = b->n;
+  if (layout == 4 && !b->want_wide4) {
+    b->want_wide4 = true;
+    if (n >= 2)
+      k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide, nullptr);
+  } else if (layout == 8 && !b->want_wide8) {
+    b->want_wide8 = true;
+    if (n >= 2)
+      k_wide8_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
+  }
   return hipGetLastError();
 }
 

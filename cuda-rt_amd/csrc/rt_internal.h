@@ -115,13 +115,16 @@ struct rt_bvh {
   float* node_box = nullptr;     // (2n-1)*6
   float4* tnode = nullptr;       // (n-1)*4
   float4* ltri = nullptr;        // n*3
+  float4* trirec = nullptr;      // n*3: the leaf records in triangle order (written with the triangle boxes)
   float4* wide = nullptr;        // (n-1)*8
   float4* wideq = nullptr;       // (n-1)*4
   float4* wide8 = nullptr;       // (n-1)*8
   uint8_t* wide_flag = nullptr;  // n-1: bit 0 = even depth (4-wide node), bit 1 = depth divisible by 3 (8-wide)
   int32_t* leaf_of = nullptr;    // n: triangle id -> sorted leaf position
-  // The 8-wide nodes are only kept up to date once a traversal has asked
-  // for them (rt_ensure_wide8); from then on every build and refit fills them.
+  // The unquantized 4-wide nodes and the 8-wide nodes are only kept up to
+  // date once a traversal has asked for them (rt_ensure_layout); from then on
+  // every build and refit fills them.
+  mutable bool want_wide4 = false;
   mutable bool want_wide8 = false;
 
   BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide, wideq, wide8, leaf_of}; }
@@ -140,8 +143,11 @@ constexpr int kMiscDeferredC = 61;  //   (kMiscDeferred, kMiscDeferredB, kMiscDe
 constexpr int kMiscWords = 64;
 
 // Fused hierarchy + refit climb: one workgroup per chunk of sorted leaves.
-constexpr int kClimbThreads = 256;
-constexpr int kClimbLeavesPerThread = 4;
+#ifndef RT_CLIMB_THREADS
+#define RT_CLIMB_THREADS 1024
+#endif
+constexpr int kClimbThreads = RT_CLIMB_THREADS;
+constexpr int kClimbLeavesPerThread = 1024 / RT_CLIMB_THREADS;
 constexpr int kClimbChunk = kClimbThreads * kClimbLeavesPerThread;
 constexpr int kClimbTopThreads = 1024;
 
@@ -155,9 +161,9 @@ rt_status rt_set_error(rt_ctx* ctx, rt_status s, const char* fmt, ...);
 // Launchers (rt_build.hip).
 hipError_t rt_launch_build(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
 hipError_t rt_launch_refit(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
-// Before the first 8-wide traversal of a BVH: fill its 8-wide nodes now and
-// keep them filled by every later build and refit.
-hipError_t rt_ensure_wide8(const rt_bvh* b, hipStream_t s);
+// Before the first traversal of a BVH with node layout 4 or 8: fill those
+// nodes now and keep them filled by every later build and refit.
+hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s);
 hipError_t rt_launch_export(const rt_bvh* b, const rt_bvh_export_bufs* dst, hipStream_t s);
 
 // Launchers (rt_trace.hip).
