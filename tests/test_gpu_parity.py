@@ -175,6 +175,41 @@ def test_trace_from_inside(ctx):
     _trace_compare(ctx, hm, 63, orig, dirs)
 
 
+def test_trace_axis_aligned_and_tiny_directions(ctx):
+    """Directions with exactly zero, tiny (fp32-denormal, FP64-denormal) and
+    huge-ratio components, from origins on the ground quad's plane, on box
+    planes of the sphere and inside the scene: the reference's 1/d is then
+    +-inf or enormous and its slab terms +-inf or NaN, which the traversal's
+    fp32 child tests must never reject where the FP64 test accepts."""
+    _say("trace axis-aligned and tiny directions")
+    hm = scenes.c1_mesh(16)
+    rng = np.random.default_rng(21)
+    base = []
+    for ax in range(3):
+        for sgn in (1.0, -1.0):
+            d = np.zeros(3)
+            d[ax] = sgn
+            base.append(d)
+    for tiny in (1e-20, 1e-40, 1e-310, 0.0):
+        for ax in range(3):
+            d = rng.normal(size=3)
+            d[ax] = tiny
+            base.append(d / np.linalg.norm(d))
+            d2 = np.zeros(3)
+            d2[ax] = 1.0
+            d2[(ax + 1) % 3] = tiny
+            base.append(d2)
+    base = np.array(base)
+    origins = [np.array([0.0, 0.0, 0.0]), np.array([0.0, -1.0, -4.0]), np.array([0.25, -1.0, -2.0]),
+               np.array([0.0, 0.0, -5.0]), np.array([1.0, 0.0, -4.0]), np.array([-4.0, -1.0, -8.0])]
+    for _ in range(30):
+        origins.append(np.array([rng.uniform(-3, 3), rng.choice([-1.0, rng.uniform(-1, 1)]), rng.uniform(-7, -1)]))
+    orig = np.repeat(np.array(origins), len(base), axis=0)
+    dirs = np.tile(base, (len(origins), 1))
+    hits = _trace_compare(ctx, hm, 30, orig, dirs)
+    assert hits > 100
+
+
 def _render_compare(ctx, hm, bits, mats, light, cam, opts, pixels=None):
     dm = rtamd.DeviceMesh(hm)
     b = rtamd.BVH(ctx, dm, bits)
