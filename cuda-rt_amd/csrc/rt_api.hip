@@ -323,6 +323,12 @@ rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, co
     if (st != RT_OK) return st;
     ws = ctx->workspace;
     shade = ctx->shade;
+  } else if (counters_dev) {
+    // the megakernel counts into 64 replicas of the counters (8 KiB)
+    rt_status st = grow(ctx, &ctx->workspace, &ctx->workspace_bytes, 64 * 16 * sizeof(uint64_t),
+                        (hipStream_t)stream, "counter replicas");
+    if (st != RT_OK) return st;
+    ws = ctx->workspace;
   }
   RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, counters_dev, ws, shade,
                                (hipStream_t)stream, nullptr, nullptr, &ctx->timer));

@@ -1155,11 +1155,13 @@ __device__ __forceinline__ void flush_counters(unsigned long long* counters, uns
     tst += __shfl_xor(tst, off);
   }
   if ((threadIdx.x & 63) == 0) {
-    if (prim) atomicAdd(&counters[0], prim);
-    if (shadow) atomicAdd(&counters[1], shadow);
-    if (bounce) atomicAdd(&counters[2], bounce);
-    if (vis) atomicAdd(&counters[3], vis);
-    if (tst) atomicAdd(&counters[4], tst);
+    // one of 64 counter replicas, 16 words apart (summed after the launch)
+    unsigned long long* r = counters + (blockIdx.x & 63) * 16;
+    if (prim) atomicAdd(&r[0], prim);
+    if (shadow) atomicAdd(&r[1], shadow);
+    if (bounce) atomicAdd(&r[2], bounce);
+    if (vis) atomicAdd(&r[3], vis);
+    if (tst) atomicAdd(&r[4], tst);
   }
 }
 
@@ -1506,6 +1508,12 @@ __global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, SceneD
   uint32_t p = W.spix[i];
   stv(W.acc, p, W.npix, ldv(W.acc, p, W.npix) + rt::hadamard(ldv(e + 12 * c, 0, c), I));
   if (W.rec) W.rec[rec_index(W, p, sample, bounce)].y = occluder_code(D, q, s);
+}
+
+// Zero n 64-bit words (graph-safe replacement for a memset).
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
 }
 
 // Sum the replicas: counters[0..2] += the ray counts, counters[3..4] += the
@@ -1887,8 +1895,12 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   const int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
   unsigned long long* cnt = (unsigned long long*)counters;
   uint32_t* ovf = b->misc + kMiscOverflow;
-  if (!workspace) {
-    k_render<<<tiles, kThreads, 0, s>>>(b->view(), M, mats, P, rgb, cnt, ovf);
+  if (!workspace || !shade_table) {
+    // megakernel; with counters, the workspace holds the counter replicas
+    unsigned long long* rep = cnt ? (unsigned long long*)workspace : nullptr;
+    if (rep) k_zero_u64<<<cdiv(kCtrReplicas * kCtrStride, 256), 256, 0, s>>>(rep, kCtrReplicas * kCtrStride);
+    k_render<<<tiles, kThreads, 0, s>>>(b->view(), M, mats, P, rgb, rep, ovf);
+    if (rep) k_wf_counters_out<<<1, 64, 0, s>>>(rep, cnt, nullptr);
     return hipGetLastError();
   }
   SceneDesc d;
