@@ -775,24 +775,57 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
             RT_CSWAP(1, 4) RT_CSWAP(3, 6) RT_CSWAP(1, 2) RT_CSWAP(3, 4) RT_CSWAP(5, 6)
           }
 #undef RT_CSWAP
+          if constexpr (NW == 4) {
+            if (S.sp - S.lo <= RING - 3) {
+              // room for three entries: push ref[m-1] .. ref[1] without a
+              // branch per entry. Slots sp .. sp+2 are free ring slots, so
+              // the writes past the m-1 kept entries are harmless.
+              const int a0 = m >= 4 ? ref[3] : (m == 3 ? ref[2] : ref[1]);
+              const int a1 = m >= 4 ? ref[2] : ref[1];
+              S.col[(S.sp & (RING - 1)) * kThreads] = a0;
+              S.col[((S.sp + 1) & (RING - 1)) * kThreads] = a1;
+              S.col[((S.sp + 2) & (RING - 1)) * kThreads] = ref[1];
+              S.sp += m > 1 ? m - 1 : 0;
+            } else {
 #pragma unroll
-          for (int k = NW - 1; k >= 1; --k)
-            if (m > k) ovf = ovf || !stack_push<RING>(S, spill, ref[k]);
+              for (int k = NW - 1; k >= 1; --k)
+                if (m > k) ovf = ovf || !stack_push<RING>(S, spill, ref[k]);
+            }
+          } else {
+#pragma unroll
+            for (int k = NW - 1; k >= 1; --k)
+              if (m > k) ovf = ovf || !stack_push<RING>(S, spill, ref[k]);
+          }
           if (m >= 1) next = ref[0];
         } else {
           // children in sorted-leaf order: the first passing one next, the
           // others pushed so that they pop in order (left-first DFS)
+          if (LAYOUT != 4 && NW == 4 && S.sp - S.lo <= RING - 3) {
+            // room for three entries: plain ring writes, no per-entry check
 #pragma unroll
-          for (int j = NW - 1; j >= 0; --j) {
-            const int ref = cref[j];
-            bool p = ref != kWideEmpty && t0[j] <= t1[j];
-            if (LAYOUT == 4 && p && ref < 0) {
-              double l0, l1;
-              p = wide_leaf_slab(r, w, j & 3, l0, l1);  // exact decision for a leaf child
+            for (int j = NW - 1; j >= 0; --j) {
+              const int ref = cref[j];
+              if ((ref != kWideEmpty) & (t0[j] <= t1[j])) {
+                if (next != kNoNode) {
+                  S.col[(S.sp & (RING - 1)) * kThreads] = next;
+                  ++S.sp;
+                }
+                next = ref;
+              }
             }
-            if (p) {
-              if (next != kNoNode) ovf = ovf || !stack_push<RING>(S, spill, next);
-              next = ref;
+          } else {
+#pragma unroll
+            for (int j = NW - 1; j >= 0; --j) {
+              const int ref = cref[j];
+              bool p = ref != kWideEmpty && t0[j] <= t1[j];
+              if (LAYOUT == 4 && p && ref < 0) {
+                double l0, l1;
+                p = wide_leaf_slab(r, w, j & 3, l0, l1);  // exact decision for a leaf child
+              }
+              if (p) {
+                if (next != kNoNode) ovf = ovf || !stack_push<RING>(S, spill, next);
+                next = ref;
+              }
             }
           }
         }
