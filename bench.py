@@ -400,6 +400,15 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kstats = ctx.kernel_stats()
+    # one more render, untimed, with every pass on one stream: each traversal
+    # launch then runs alone on the GPU (in the timed steps a sample's any-hit
+    # launch overlaps the next sample's closest-hit launch, so their spans
+    # include time shared with the other kernel)
+    iso = None
+    if args.mode == "wavefront":
+        o1 = rtamd.render_opts(opts.spp, opts.depth, opts.seed, opts.flags | rtamd.RT_RENDER_ONE_STREAM)
+        rtamd.render(ctx, bvh, dm, md, len(mats), light, cam, o1, rgb=rgb)
+        iso = ctx.kernel_stats()
     ctx.set_kernel_timing(False)
     diag = bvh.diagnostics()
     build_ms = float(np.median([e[0].elapsed_time(e[1]) for e in evs]))
@@ -438,6 +447,12 @@ def main():
     else:
         roof["traffic"] = committed_traffic("closest", args)
         roof_any["traffic"] = committed_traffic("any", args)
+        roof["note"] = ("avg_launch_ms is the launch's span in the timed steps, where it shares the GPU with the "
+                        "previous sample's any-hit launch; roofline_isolated times the same kernel alone")
+    roof_iso = None
+    if iso is not None:
+        roof_iso = {"closest": kernel_roofline("k_trace_persistent<closest hit> alone", iso["closest"]),
+                    "any": kernel_roofline("k_trace_persistent<any hit> alone", iso["any"])}
     build_bytes = BUILD_BYTES_PER_TRI_63 * args.tris
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
     out = {
@@ -459,6 +474,7 @@ def main():
         "render_mode": args.mode,
         "roofline": roof,
         "roofline_any_hit": roof_any,
+        "roofline_isolated": roof_iso,
         "render_work": {"box_tests": box_tests, "tri_tests": tests},
         "roofline_build": {
             "kernels": "k_init..k_climb_top (one rebuild)",

@@ -37,6 +37,24 @@ bool mesh_ok(const rt_mesh_view* m) {
 
 }  // namespace
 
+const rt_overlap* rt_ctx_overlap(rt_ctx* ctx) {
+  if (ctx->overlap_ready) return ctx->overlap.aux ? &ctx->overlap : nullptr;
+  ctx->overlap_ready = true;
+  rt_overlap& o = ctx->overlap;
+  bool ok = hipStreamCreateWithFlags(&o.aux, hipStreamNonBlocking) == hipSuccess;
+  for (hipEvent_t* e : {&o.start, &o.shaded, &o.flushed[0], &o.flushed[1], &o.done})
+    ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    for (hipEvent_t e : {o.start, o.shaded, o.flushed[0], o.flushed[1], o.done})
+      if (e) (void)hipEventDestroy(e);
+    if (o.aux) (void)hipStreamDestroy(o.aux);
+    o = rt_overlap{};
+    return nullptr;
+  }
+  return &o;
+}
+
 extern "C" {
 
 const char* rt_version(void) { return "rtamd 0.1 (gfx950)"; }
@@ -66,10 +84,15 @@ void rt_ctx_destroy(rt_ctx* ctx) {
   }
   for (hipEvent_t e : ctx->timer.pool) (void)hipEventDestroy(e);
   if (ctx->timer.dev_work) (void)hipFree(ctx->timer.dev_work);
+  rt_overlap& o = ctx->overlap;
+  for (hipEvent_t e : {o.start, o.shaded, o.flushed[0], o.flushed[1], o.done})
+    if (e) (void)hipEventDestroy(e);
+  if (o.aux) (void)hipStreamDestroy(o.aux);
   delete ctx;
 }
 
 const char* rt_last_error(const rt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
 
 rt_status rt_ctx_set_kernel_timing(rt_ctx* ctx, int32_t enable) {
   if (!ctx) return RT_ERR_INVALID;
@@ -311,7 +334,7 @@ rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, co
   if (cam->width <= 0 || cam->height <= 0 || opts->spp <= 0 || opts->depth < 0)
     return rt_set_error(ctx, RT_ERR_INVALID, "bad image size, spp or depth");
   if (mesh->n != bvh->n) return rt_set_error(ctx, RT_ERR_INVALID, "mesh and BVH triangle counts differ");
-  if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT))
+  if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT | RT_RENDER_ONE_STREAM))
     return rt_set_error(ctx, RT_ERR_INVALID, "unknown render flags");
   void* ws = nullptr;
   void* shade = nullptr;
@@ -331,7 +354,8 @@ rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, co
     ws = ctx->workspace;
   }
   RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, counters_dev, ws, shade,
-                               (hipStream_t)stream, nullptr, nullptr, &ctx->timer));
+                               (hipStream_t)stream, nullptr, nullptr, &ctx->timer,
+                               shade && !(opts->flags & RT_RENDER_ONE_STREAM) ? rt_ctx_overlap(ctx) : nullptr));
   return RT_OK;
 }
 
@@ -350,7 +374,7 @@ static rt_status diff_check(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* 
                         RT_DIFF_MAX_DEPTH);
   if (mesh->n != bvh->n) return rt_set_error(ctx, RT_ERR_INVALID, "mesh and BVH triangle counts differ");
   if (!(light_scale > 0)) return rt_set_error(ctx, RT_ERR_INVALID, "light_scale must be positive");
-  if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT))
+  if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT | RT_RENDER_ONE_STREAM))
     return rt_set_error(ctx, RT_ERR_INVALID, "unknown render flags");
   return RT_OK;
 }
@@ -391,7 +415,8 @@ rt_status rt_diff_forward(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* me
   rt_render_opts o = *opts;
   o.flags = (o.flags & RT_RENDER_RANDVEC_RTL) | RT_RENDER_WAVEFRONT;
   RT_HIP(ctx, rt_launch_render(bvh, &m, mats, mesh->n, &L, cam, &o, rgb, nullptr, ctx->workspace, ctx->shade, s,
-                               records_dev, image_dev));
+                               records_dev, image_dev, nullptr,
+                               (opts->flags & RT_RENDER_ONE_STREAM) ? nullptr : rt_ctx_overlap(ctx)));
   return RT_OK;
 }
 

@@ -113,6 +113,7 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->joins[k], hipEventDisableTiming);
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  (void)rt_ctx_overlap(ctx);  // created here, outside any graph capture of rt_batch_render
   if (e != hipSuccess) {
     rt_batch_destroy(b);
     return rt_set_error(ctx, RT_ERR_DEVICE, "batch setup failed: %s", hipGetErrorString(e));
@@ -144,7 +145,7 @@ rt_status rt_batch_render(rt_batch* b, uint64_t* counters_dev, void* stream) {
     RT_HIP(b->ctx, rt_ensure_layout(it.bvh, b->tuning.node_layout, (hipStream_t)stream));
   RT_HIP(b->ctx, rt_launch_wavefront(b->ws, b->npix, b->ntiles, b->nscenes, b->max_spp, b->max_depth,
                                      b->ctx->num_cus, &b->tuning, counters_dev, nullptr, (hipStream_t)stream,
-                                     nullptr, nullptr, nullptr, &b->ctx->timer));
+                                     nullptr, nullptr, nullptr, &b->ctx->timer, rt_ctx_overlap(b->ctx)));
   return RT_OK;
 }
 

@@ -23,6 +23,13 @@ struct rt_kernel_timer {
   unsigned long long* dev_work = nullptr;
 };
 
+// Second stream and events of a context's wavefront renders: a sample's
+// last any-hit pass runs there while the next sample is traced.
+struct rt_overlap {
+  hipStream_t aux = nullptr;
+  hipEvent_t start = nullptr, shaded = nullptr, flushed[2] = {nullptr, nullptr}, done = nullptr;
+};
+
 struct rt_ctx {
   int32_t device = 0;
   int32_t num_cus = 256;
@@ -34,7 +41,13 @@ struct rt_ctx {
   void* diff = nullptr;          // differentiable render: materials, identity ids, float image
   size_t diff_bytes = 0;
   rt_kernel_timer timer;
+  rt_overlap overlap;
+  bool overlap_ready = false;
 };
+
+// The context's overlap stream and events, created on first use (nullptr if
+// that fails: the passes then run on one stream).
+const rt_overlap* rt_ctx_overlap(rt_ctx* ctx);
 
 // Device-side view of a built BVH, passed by value to the trace kernels.
 // Node numbering follows the reference (inc/bvh.cuh:25-28): internal
@@ -177,7 +190,7 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
                             uint64_t* counters, void* workspace, void* shade_table, hipStream_t s,
                             int32_t* records = nullptr, double* color64 = nullptr,
-                            rt_kernel_timer* timer = nullptr);
+                            rt_kernel_timer* timer = nullptr, const rt_overlap* ov = nullptr);
 size_t rt_wavefront_bytes(int32_t width, int32_t height);
 // Per-leaf shading table of an n-triangle scene: 64 B per sorted leaf
 // (three vertex normals and the material id), refreshed by every render.
@@ -192,7 +205,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
                                uint32_t* overflow, hipStream_t s, int32_t* records = nullptr,
                                double* color64 = nullptr, const void* single_scene = nullptr,
-                               rt_kernel_timer* timer = nullptr);
+                               rt_kernel_timer* timer = nullptr, const rt_overlap* ov = nullptr);
 // Differentiable render (rt_diff.hip).
 hipError_t rt_launch_zero_f64(double* p, size_t n, hipStream_t s);
 hipError_t rt_launch_resolve(const float* big, int32_t W, int32_t H, int32_t ss, bool reference_divisor, float* out,

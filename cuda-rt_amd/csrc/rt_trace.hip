@@ -1281,20 +1281,23 @@ struct WfState {
   uint32_t ntiles;
   uint32_t* rng;        // pixels
   double* color;        // pixels*3: sum over finished samples
-  double* acc;          // pixels*3: current sample's path sum
+  double* acc;          // pixels*3: a sample's path sum (one buffer per sample parity)
   double* thr;          // pixels*3: current path throughput
   double* ray[2];       // cap*6: ray origin, direction per queue slot
   uint32_t* pix[2];     // cap: global pixel of each queue slot (~0 = unused slot)
   uint16_t* rscene[2];  // cap: scene of each queue slot
   int32_t* hpos;        // cap: closest hit leaf position or -1
   double* hit;          // cap*3: t, alpha, beta
+  // shadow queue of a sample parity: the last bounce's any-hit pass of
+  // sample k may still run while sample k+1 is traced
   double* shq;          // cap*16: q, L, I1, ds, thr, fo
   uint32_t* spix;       // cap: global pixel of each shadow entry
   uint16_t* sscene;     // cap: scene of each shadow entry
   int32_t* spos;        // cap: any-hit leaf position or -1
   double* st;           // cap: any-hit t
-  uint32_t* counts;     // [0],[1] ray queue sizes, [2] shadow entries, [8] stack overflows
-  uint32_t* fetch[2];   // kFetchWords each: closest-hit and any-hit fetch counters
+  uint32_t* scount;     // number of shadow entries (counts[2] or counts[3])
+  uint32_t* counts;     // [0],[1] ray queue sizes, [2],[3] shadow entries per parity, [8] stack overflows
+  uint32_t* fetch[2];   // kFetchWords each: closest-hit and any-hit fetch counters (the latter per parity)
   int2* rec;            // optional path record: (hit leaf, occluder code) per pixel, sample, bounce
   int rec_spp, rec_depth;
   double* color64;      // optional FP64 output (pixels*3, row-flipped like rgb, unclamped mean)
@@ -1486,7 +1489,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
       W.rng[p] = rng;
     }
   }
-  uint32_t ss = block_append(&W.counts[2], want_shadow, s_tmp);
+  uint32_t ss = block_append(W.scount, want_shadow, s_tmp);
   if (want_shadow) {
     W.spix[ss] = p;
     W.sscene[ss] = sc;
@@ -1524,7 +1527,7 @@ __device__ __forceinline__ int occluder_code(const SceneDesc& D, d3 q, const Hit
 template <bool MULTI>
 __global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, SceneDesc D1, int sample, int bounce,
                                                                unsigned long long* counters) {
-  const uint32_t n_in = W.counts[2];
+  const uint32_t n_in = *W.scount;
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   const bool live = i < n_in;
   if (counters) rep_count(W.rep, 1, live);
@@ -1736,7 +1739,7 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
 // Workspace layout of a wavefront render over `nscenes` scenes with `npix`
 // pixels and `ntiles` tiles in total.
 struct WfLayout {
-  size_t off[24];
+  size_t off[32];
   size_t total;
 };
 
@@ -1746,7 +1749,9 @@ static WfLayout wf_layout(size_t npix, size_t ntiles, size_t nscenes) {
   size_t sizes[] = {nscenes * sizeof(SceneDesc), nscenes * sizeof(BvhView), ntiles * 2, npix * 4, npix * 24,
                     npix * 24, npix * 24, cap * 48, cap * 48, cap * 4, cap * 4, cap * 2, cap * 2, cap * 4,
                     cap * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, 256, kFetchWords * 4, kFetchWords * 4,
-                    kCtrReplicas * kCtrStride * 8};
+                    kCtrReplicas * kCtrStride * 8,
+                    // parity 1: accumulator, shadow queue, any-hit fetch counters
+                    npix * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, kFetchWords * 4};
   size_t off = 0;
   for (int k = 0; k < (int)(sizeof(sizes) / sizeof(sizes[0])); ++k) {
     L.off[k] = off;
@@ -1756,7 +1761,10 @@ static WfLayout wf_layout(size_t npix, size_t ntiles, size_t nscenes) {
   return L;
 }
 
-static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes) {
+// The state as the passes of a sample of the given parity see it: the
+// accumulator, shadow queue, shadow count and any-hit fetch counters are
+// that parity's; everything else is shared.
+static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes, int parity = 0) {
   WfLayout L = wf_layout(npix, ntiles, nscenes);
   char* w = (char*)ws;
   WfState W;
@@ -1786,6 +1794,17 @@ static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes) {
   W.fetch[0] = (uint32_t*)(w + L.off[k++]);
   W.fetch[1] = (uint32_t*)(w + L.off[k++]);
   W.rep = (unsigned long long*)(w + L.off[k++]);
+  W.scount = W.counts + 2;
+  if (parity) {
+    W.acc = (double*)(w + L.off[k++]);
+    W.shq = (double*)(w + L.off[k++]);
+    W.spix = (uint32_t*)(w + L.off[k++]);
+    W.sscene = (uint16_t*)(w + L.off[k++]);
+    W.spos = (int32_t*)(w + L.off[k++]);
+    W.st = (double*)(w + L.off[k++]);
+    W.fetch[1] = (uint32_t*)(w + L.off[k++]);
+    W.scount = W.counts + 3;
+  }
   W.npix = npix;
   W.cap = ntiles * kThreads;
   return W;
@@ -1845,48 +1864,67 @@ static hipError_t timer_mark(rt_kernel_timer* T, int kind, bool start, hipStream
 hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
                                uint32_t* overflow, hipStream_t s, int32_t* records, double* color64,
-                               const void* single_scene, rt_kernel_timer* timer) {
-  WfState W = wf_state(workspace, npix, ntiles, nscenes);
+                               const void* single_scene, rt_kernel_timer* timer, const rt_overlap* ov) {
+  // Wp[k & 1] is the state sample k's passes see (its parity's accumulator
+  // and shadow queue).
+  WfState Wp[2] = {wf_state(workspace, npix, ntiles, nscenes, 0), wf_state(workspace, npix, ntiles, nscenes, 1)};
+  for (WfState& W : Wp) {
+    W.rec = (int2*)records;
+    W.rec_spp = max_spp;
+    W.rec_depth = max_depth;
+    W.color64 = color64;
+  }
+  const WfState& W0 = Wp[0];
   rt_kernel_timer* T = timer && timer->enabled ? timer : nullptr;
   const bool multi = single_scene == nullptr;
   SceneDesc D1{};
   if (!multi) D1 = *(const SceneDesc*)single_scene;
-#define RT_WF(KERNEL, ...)                                                  \
-  do {                                                                     \
-    if (multi)                                                             \
-      KERNEL<true><<<tiles, kThreads, 0, s>>>(W, D1, __VA_ARGS__);         \
-    else                                                                   \
-      KERNEL<false><<<tiles, kThreads, 0, s>>>(W, D1, __VA_ARGS__);        \
+#define RT_WF(STREAM, STATE, KERNEL, ...)                                        \
+  do {                                                                           \
+    if (multi)                                                                   \
+      KERNEL<true><<<tiles, kThreads, 0, STREAM>>>(STATE, D1, __VA_ARGS__);      \
+    else                                                                         \
+      KERNEL<false><<<tiles, kThreads, 0, STREAM>>>(STATE, D1, __VA_ARGS__);     \
   } while (0)
-  W.rec = (int2*)records;
-  W.rec_spp = max_spp;
-  W.rec_depth = max_depth;
-  W.color64 = color64;
-  rt_tuning t = tuning ? *tuning : rt_default_tuning();
-  unsigned long long* cnt = (unsigned long long*)counters;
-  const size_t cap = ntiles * kThreads;
-  const int tiles = (int)ntiles;
-  const bool work = cnt || T;  // keep the replicated counts
 #define RT_TRY(X)                          \
   do {                                     \
     hipError_t e_ = (X);                   \
     if (e_ != hipSuccess) return e_;       \
   } while (0)
+  rt_tuning t = tuning ? *tuning : rt_default_tuning();
+  unsigned long long* cnt = (unsigned long long*)counters;
+  const size_t cap = ntiles * kThreads;
+  const int tiles = (int)ntiles;
+  const bool work = cnt || T;  // keep the replicated counts
   if (multi) {
     const int gx = (int)std::max<size_t>(1, 4096 / nscenes);
-    k_shade_fill<true><<<dim3(gx, (unsigned)nscenes), 256, 0, s>>>(W.scenes, D1);
+    k_shade_fill<true><<<dim3(gx, (unsigned)nscenes), 256, 0, s>>>(W0.scenes, D1);
   } else if (D1.M.lshade) {
-    k_shade_fill<false><<<std::min(cdiv(D1.B.n, 256), 4096), 256, 0, s>>>(W.scenes, D1);
+    k_shade_fill<false><<<std::min(cdiv(D1.B.n, 256), 4096), 256, 0, s>>>(W0.scenes, D1);
   }
   const int grid = persistent_grid(num_cus, t, (uint32_t)cap);
-  uint32_t* ovf = overflow ? overflow : W.counts + 8;
+  uint32_t* ovf = overflow ? overflow : W0.counts + 8;
   if (multi)
-    k_wf_init<true><<<tiles, kThreads, 0, s>>>(W, D1);
+    k_wf_init<true><<<tiles, kThreads, 0, s>>>(W0, D1);
   else
-    k_wf_init<false><<<tiles, kThreads, 0, s>>>(W, D1);
-  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W.fetch[0], 2 * kFetchWords);
+    k_wf_init<false><<<tiles, kThreads, 0, s>>>(W0, D1);
+  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W0.fetch[0], 2 * kFetchWords);
+  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[1].fetch[1], kFetchWords);
+  // The last bounce's any-hit pass, shadow resolve and flush of sample k run
+  // on the auxiliary stream, overlapping sample k+1's camera, closest-hit
+  // and shading passes on s: they share no buffer but the replicated
+  // counters (atomics). Sample k+2 reuses sample k's parity buffers, so its
+  // camera pass waits for sample k's flush.
+  hipStream_t sa = ov ? ov->aux : s;
+  if (ov) {
+    RT_TRY(hipEventRecord(ov->start, s));
+    RT_TRY(hipStreamWaitEvent(sa, ov->start, 0));
+  }
   for (int it = 0; it < max_spp; ++it) {
-    RT_WF(k_wf_camera, it, (uint32_t)cap);
+    const int par = it & 1;
+    const WfState& W = Wp[par];
+    if (ov && it >= 2) RT_TRY(hipStreamWaitEvent(s, ov->flushed[par], 0));
+    RT_WF(s, W, k_wf_camera, it, (uint32_t)cap);
     int cur = 0;
     for (int bounce = 0; bounce <= max_depth; ++bounce) {
       RayIn R{W.ray[cur], W.ray[cur] + 3 * cap, 1, cap, &W.counts[cur], 0};
@@ -1896,24 +1934,38 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
       launch_persistent<false>(t, grid, s, W.views, nscenes > 1 ? W.rscene[cur] : nullptr, R,
                                bounce == 0 ? 0.0 : kBounceTmin, O, W.fetch[0], ovf, work ? W.rep + 3 : nullptr);
       if (T) RT_TRY(timer_mark(T, 0, false, s));
-      RT_WF(k_wf_shade, bounce, cur, it, cnt);
-      RayIn SR{W.shq, W.shq + 3 * cap, 1, cap, &W.counts[2], 0};
+      RT_WF(s, W, k_wf_shade, bounce, cur, it, cnt);
+      // the queue traced above is consumed: clear its count and the fetch counters
+      k_wf_reset<<<1, 256, 0, s>>>(W.counts, 1u << cur, W.fetch[0], kFetchWords);
+      // earlier bounces share the accumulator with the next bounce's shading
+      const bool overlap = ov && bounce == max_depth;
+      hipStream_t sb = overlap ? sa : s;
+      if (overlap) {
+        RT_TRY(hipEventRecord(ov->shaded, s));
+        RT_TRY(hipStreamWaitEvent(sa, ov->shaded, 0));
+      }
+      RayIn SR{W.shq, W.shq + 3 * cap, 1, cap, W.scount, 0};
       HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr, 0};
-      if (T) RT_TRY(timer_mark(T, 1, true, s));
-      launch_persistent<true>(t, grid, s, W.views, nscenes > 1 ? W.sscene : nullptr, SR, 0.0, SO, W.fetch[1], ovf,
-                              work ? W.rep + 5 : nullptr);
-      if (T) RT_TRY(timer_mark(T, 1, false, s));
-      RT_WF(k_wf_shadow_finish, it, bounce, cnt);
-      k_wf_reset<<<1, 256, 0, s>>>(W.counts, (1u << 2) | (1u << cur), W.fetch[0], 2 * kFetchWords);
+      if (T) RT_TRY(timer_mark(T, 1, true, sb));
+      launch_persistent<true>(t, grid, sb, W.views, nscenes > 1 ? W.sscene : nullptr, SR, 0.0, SO, W.fetch[1],
+                              ovf, work ? W.rep + 5 : nullptr);
+      if (T) RT_TRY(timer_mark(T, 1, false, sb));
+      RT_WF(sb, W, k_wf_shadow_finish, it, bounce, cnt);
+      k_wf_reset<<<1, 256, 0, sb>>>(W.counts, 1u << (2 + par), W.fetch[1], kFetchWords);
       cur ^= 1;
     }
-    RT_WF(k_wf_flush, it);
+    RT_WF(sa, W, k_wf_flush, it);
+    if (ov) RT_TRY(hipEventRecord(ov->flushed[par], sa));
+  }
+  if (ov) {
+    RT_TRY(hipEventRecord(ov->done, sa));
+    RT_TRY(hipStreamWaitEvent(s, ov->done, 0));
   }
   if (multi)
-    k_wf_resolve<true><<<tiles, kThreads, 0, s>>>(W, D1);
+    k_wf_resolve<true><<<tiles, kThreads, 0, s>>>(W0, D1);
   else
-    k_wf_resolve<false><<<tiles, kThreads, 0, s>>>(W, D1);
-  if (work) k_wf_counters_out<<<1, 64, 0, s>>>(W.rep, cnt, T ? T->dev_work : nullptr);
+    k_wf_resolve<false><<<tiles, kThreads, 0, s>>>(W0, D1);
+  if (work) k_wf_counters_out<<<1, 64, 0, s>>>(W0.rep, cnt, T ? T->dev_work : nullptr);
 #undef RT_WF
 #undef RT_TRY
   return hipGetLastError();
@@ -1922,7 +1974,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
 hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                             const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
                             uint64_t* counters, void* workspace, void* shade_table, hipStream_t s,
-                            int32_t* records, double* color64, rt_kernel_timer* timer) {
+                            int32_t* records, double* color64, rt_kernel_timer* timer, const rt_overlap* ov) {
   RenderParams P = rt_make_params(cam, light, opts, nmat);
   MeshShade M{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev, nullptr};
   const int tiles = cdiv(P.W, 16) * cdiv(P.H, 16);
@@ -1950,7 +2002,7 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
                                                   (uint16_t*)W.tile_scene, d, (uint32_t)tiles);
   return rt_launch_wavefront(workspace, (size_t)P.W * P.H, (size_t)tiles, 1, P.spp, P.depth,
                              b->ctx ? b->ctx->num_cus : 256, nullptr, counters, ovf, s, records, color64, &d,
-                             timer);
+                             timer, ov);
 }
 
 // ------------------------------------------------------------ differentiable render

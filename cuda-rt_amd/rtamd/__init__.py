@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "librtamd.so")
 RT_OK, RT_ERR_INVALID, RT_ERR_OOM, RT_ERR_DEVICE, RT_ERR_UNSUPPORTED, RT_ERR_OVERFLOW = range(6)
 RT_TRACE_CLOSEST, RT_TRACE_ANY = 0, 1
 RT_BUILD_DEFAULT, RT_BUILD_KARRAS_SEARCH = 0, 1
-RT_RENDER_RANDVEC_RTL, RT_RENDER_CLAMP, RT_RENDER_WAVEFRONT = 1, 2, 4
+RT_RENDER_RANDVEC_RTL, RT_RENDER_CLAMP, RT_RENDER_WAVEFRONT, RT_RENDER_ONE_STREAM = 1, 2, 4, 8
 
 c_f = ctypes.POINTER(ctypes.c_float)
 c_i32 = ctypes.POINTER(ctypes.c_int32)

@@ -114,7 +114,10 @@ typedef struct {
 enum {
   RT_RENDER_RANDVEC_RTL = 1, /* draw randVec3d's components z, y, x instead of x, y, z */
   RT_RENDER_CLAMP = 2,       /* clamp each channel to [0,1] (kernels/scene.cu:137) */
-  RT_RENDER_WAVEFRONT = 4    /* ray-queue passes instead of the megakernel (same image) */
+  RT_RENDER_WAVEFRONT = 4,   /* ray-queue passes instead of the megakernel (same image) */
+  RT_RENDER_ONE_STREAM = 8   /* wavefront: every pass on the caller's stream (by default the
+                                last any-hit pass of a sample overlaps the next sample on a
+                                second, context-owned stream; same image either way) */
 };
 
 typedef struct {

@@ -48,3 +48,25 @@ def test_kernel_stats_count_launches_and_work(ctx, depth):
     rtamd.render(ctx, bvh, dm, md, len(mats), light, cam, opts)
     ks2 = ctx.kernel_stats()
     assert all(v["launches"] == 0 and v["box_tests"] == 0 for v in ks2.values())
+
+
+@pytest.mark.parametrize("depth", [0, 2])
+def test_overlapped_schedule_matches_one_stream(ctx, depth):
+    """The default schedule (a sample's last any-hit pass on a second stream,
+    overlapping the next sample) gives the same image and ray counts as
+    RT_RENDER_ONE_STREAM, over enough samples to reuse each parity buffer."""
+    print(f"overlap vs one stream depth={depth}", flush=True)
+    hm = scenes.soup_mesh(30_000, seed=9)
+    dm = rtamd.DeviceMesh(hm)
+    bvh = rtamd.BVH(ctx, dm, 63)
+    mats, light, cam, opts = scenes.c2_setup(80, 48, 5)
+    md = rtamd.materials_tensor(mats)
+    out = []
+    for extra in (0, rtamd.RT_RENDER_ONE_STREAM):
+        o = rtamd.render_opts(5, depth, 3, opts.flags | rtamd.RT_RENDER_WAVEFRONT | extra)
+        cnt = torch.zeros(5, dtype=torch.int64, device="cuda")
+        img = rtamd.render(ctx, bvh, dm, md, len(mats), light, cam, o, counters=cnt)
+        torch.cuda.synchronize()
+        out.append((img.cpu().numpy(), cnt.cpu().numpy()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1][:3], out[1][1][:3])
