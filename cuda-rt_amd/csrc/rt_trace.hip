@@ -147,25 +147,26 @@ struct RingStack {
   int sp = 0, lo = 0;
 };
 
-template <int RING = kRing>
+// STRIDE = threads per workgroup (one LDS column per thread).
+template <int RING = kRing, int STRIDE = kThreads>
 __device__ __forceinline__ bool stack_push(RingStack& S, int* spill, int v) {
   if (S.sp - S.lo == RING) {
     if (S.lo == kSpill) return false;
-    spill[S.lo] = S.col[(S.lo & (RING - 1)) * kThreads];
+    spill[S.lo] = S.col[(S.lo & (RING - 1)) * STRIDE];
     ++S.lo;
   }
-  S.col[(S.sp & (RING - 1)) * kThreads] = v;
+  S.col[(S.sp & (RING - 1)) * STRIDE] = v;
   ++S.sp;
   return true;
 }
 
-template <int RING = kRing>
+template <int RING = kRing, int STRIDE = kThreads>
 __device__ __forceinline__ int stack_pop(RingStack& S, int* spill) {
   --S.sp;
-  int v = S.col[(S.sp & (RING - 1)) * kThreads];
+  int v = S.col[(S.sp & (RING - 1)) * STRIDE];
   if (S.sp == S.lo && S.lo > 0) {
     --S.lo;
-    S.col[(S.lo & (RING - 1)) * kThreads] = spill[S.lo];
+    S.col[(S.lo & (RING - 1)) * STRIDE] = spill[S.lo];
   }
   return v;
 }
@@ -540,14 +541,22 @@ __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uin
 // views[scene[i]] is the BVH of ray i (scene == nullptr: views[0] for all).
 // LAYOUT 2: binary nodes, FP64 child tests; 4: 4-wide nodes (128 B);
 // 5: quantized 4-wide nodes (64 B).
+// Workgroups of the persistent traversal: one wave each, so a wave that has
+// finished its rays frees its slot at once (for the kernel that runs
+// concurrently on the other stream) instead of waiting for three others.
+#ifndef RT_TRACE_THREADS
+#define RT_TRACE_THREADS 64
+#endif
+constexpr int kTraceThreads = RT_TRACE_THREADS;
+
 template <bool ANY, int RING, int LAYOUT, bool MULTI>
-__global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __restrict__ views,
+__global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhView* __restrict__ views,
                                                                const uint16_t* __restrict__ scene, RayIn R,
                                                                double tmin_phys, HitOut O, uint32_t* fetch_ctr,
                                                                uint32_t fetch_chunk, uint32_t* overflow,
                                                                unsigned long long* stats) {
   constexpr bool WIDE = LAYOUT != 2, QUANT = LAYOUT == 5 || LAYOUT == 8;
-  __shared__ int s_stack[RING * kThreads];
+  __shared__ int s_stack[RING * kTraceThreads];
   const uint32_t nrays = R.count_dev ? *R.count_dev : R.count;
   const uint32_t kFetchChunk = fetch_chunk;
   BvhView B = views[0];
@@ -714,7 +723,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
       }
       if (!done && node != kNoNode && node < 0 && pending == kNoNode) {
         pending = node;
-        node = S.sp == 0 ? kNoNode : stack_pop<RING>(S, spill);
+        node = S.sp == 0 ? kNoNode : stack_pop<RING, kTraceThreads>(S, spill);
       }
       if (!done && node != kNoNode && node >= 0) {
         constexpr int NW = LAYOUT == 8 ? 8 : 4;
@@ -782,19 +791,19 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
               // the writes past the m-1 kept entries are harmless.
               const int a0 = m >= 4 ? ref[3] : (m == 3 ? ref[2] : ref[1]);
               const int a1 = m >= 4 ? ref[2] : ref[1];
-              S.col[(S.sp & (RING - 1)) * kThreads] = a0;
-              S.col[((S.sp + 1) & (RING - 1)) * kThreads] = a1;
-              S.col[((S.sp + 2) & (RING - 1)) * kThreads] = ref[1];
+              S.col[(S.sp & (RING - 1)) * kTraceThreads] = a0;
+              S.col[((S.sp + 1) & (RING - 1)) * kTraceThreads] = a1;
+              S.col[((S.sp + 2) & (RING - 1)) * kTraceThreads] = ref[1];
               S.sp += m > 1 ? m - 1 : 0;
             } else {
 #pragma unroll
               for (int k = NW - 1; k >= 1; --k)
-                if (m > k) ovf = ovf || !stack_push<RING>(S, spill, ref[k]);
+                if (m > k) ovf = ovf || !stack_push<RING, kTraceThreads>(S, spill, ref[k]);
             }
           } else {
 #pragma unroll
             for (int k = NW - 1; k >= 1; --k)
-              if (m > k) ovf = ovf || !stack_push<RING>(S, spill, ref[k]);
+              if (m > k) ovf = ovf || !stack_push<RING, kTraceThreads>(S, spill, ref[k]);
           }
           if (m >= 1) next = ref[0];
         } else {
@@ -807,7 +816,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
               const int ref = cref[j];
               if ((ref != kWideEmpty) & (t0[j] <= t1[j])) {
                 if (next != kNoNode) {
-                  S.col[(S.sp & (RING - 1)) * kThreads] = next;
+                  S.col[(S.sp & (RING - 1)) * kTraceThreads] = next;
                   ++S.sp;
                 }
                 next = ref;
@@ -823,7 +832,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
                 p = wide_leaf_slab(r, w, j & 3, l0, l1);  // exact decision for a leaf child
               }
               if (p) {
-                if (next != kNoNode) ovf = ovf || !stack_push<RING>(S, spill, next);
+                if (next != kNoNode) ovf = ovf || !stack_push<RING, kTraceThreads>(S, spill, next);
                 next = ref;
               }
             }
@@ -836,7 +845,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
         } else if (next != kNoNode) {
           node = next;
         } else {
-          node = S.sp == 0 ? kNoNode : stack_pop<RING>(S, spill);
+          node = S.sp == 0 ? kNoNode : stack_pop<RING, kTraceThreads>(S, spill);
         }
       }
       if (!done && node == kNoNode && pending == kNoNode) done = true;
@@ -872,7 +881,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
       bpos = best.pos;
       if (hl && hr) {
         int nearc = (l0 <= r0) ? cl : cr, farc = (l0 <= r0) ? cr : cl;
-        if (!stack_push<RING>(S, spill, farc)) {
+        if (!stack_push<RING, kTraceThreads>(S, spill, farc)) {
           atomicAdd(overflow, 1u);
           bpos = -1;
           done = true;
@@ -885,7 +894,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
       } else if (S.sp == 0) {
         done = true;
       } else {
-        node = stack_pop<RING>(S, spill);
+        node = stack_pop<RING, kTraceThreads>(S, spill);
       }
     } else {
       // node >= 0: internal node to enter; node < 0: leaf whose box passed.
@@ -898,7 +907,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
         } else if (S.sp == 0) {
           done = true;
         } else {
-          node = stack_pop<RING>(S, spill);
+          node = stack_pop<RING, kTraceThreads>(S, spill);
         }
       } else {
         float4 q0, q1, q2, q3;
@@ -909,7 +918,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
         bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
         int cl = __float_as_int(q3.x), cr = __float_as_int(q3.y);
         if (hl) {
-          if (hr && !stack_push<RING>(S, spill, cr)) {
+          if (hr && !stack_push<RING, kTraceThreads>(S, spill, cr)) {
             atomicAdd(overflow, 1u);
             bpos = -1;
             done = true;
@@ -920,7 +929,7 @@ __global__ __launch_bounds__(kThreads) void k_trace_persistent(const BvhView* __
         } else if (S.sp == 0) {
           done = true;
         } else {
-          node = stack_pop<RING>(S, spill);
+          node = stack_pop<RING, kTraceThreads>(S, spill);
         }
       }
     }
@@ -1684,9 +1693,10 @@ bool rt_tuning_valid(const rt_tuning* t) {
           t->node_layout == 8);
 }
 
+// waves_per_simd waves on each of a CU's 4 SIMDs, kTraceThreads per workgroup
 static int persistent_grid(int num_cus, const rt_tuning& t, uint32_t max_rays) {
-  int cap = num_cus * t.waves_per_simd;  // 4 waves per workgroup, 4 SIMDs per CU
-  int need = (int)((max_rays + kThreads - 1) / kThreads);
+  int cap = num_cus * t.waves_per_simd * (4 * 64 / kTraceThreads);
+  int need = (int)((max_rays + kTraceThreads - 1) / kTraceThreads);
   return need < cap ? (need > 0 ? need : 1) : cap;
 }
 
@@ -1699,10 +1709,10 @@ static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const
   do {                                                                                                              \
     if (scene)                                                                                                      \
       k_trace_persistent<ANY, RINGV, LAYOUTV, true>                                                                 \
-          <<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                        \
+          <<<grid, kTraceThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                   \
     else                                                                                                            \
       k_trace_persistent<ANY, RINGV, LAYOUTV, false>                                                                \
-          <<<grid, kThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                        \
+          <<<grid, kTraceThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                   \
   } while (0)
   if (t.node_layout == 8) {
     if (t.stack_ring == 16) RT_LAUNCH(16, 8); else RT_LAUNCH(32, 8);
