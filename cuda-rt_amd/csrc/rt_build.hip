@@ -799,6 +799,70 @@ __global__ __launch_bounds__(256) void k_wide_parity(int n, const int32_t* __res
   flag[i] = ((d & 1) == 0 ? 1 : 0) | (d % 3 == 0 ? 2 : 0);
 }
 
+// ---- packing the 4-wide nodes: node i (even depth, flag bit 0) goes to
+// slot wcomp[i] = number of even-depth nodes before it, so the nodes the
+// traversal reads fill whole cache lines. A three-step scan over blocks of
+// 1024 flags: block counts, their exclusive scan, then in-block ranks.
+__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t* s_w, uint32_t* total) {
+  const uint32_t lane = __lane_id(), w = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t y = __shfl_up(x, off);
+    if (lane >= (uint32_t)off) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t acc = 0;
+    for (int k = 0; k < 16; ++k) {
+      uint32_t t = s_w[k];
+      s_w[k] = acc;
+      acc += t;
+    }
+    s_w[16] = acc;
+  }
+  __syncthreads();
+  if (total) *total = s_w[16];
+  return s_w[w] + x - v;
+}
+
+__global__ __launch_bounds__(1024) void k_pack_count(int m, const uint8_t* __restrict__ flag, uint32_t* sums) {
+  __shared__ uint32_t s_w[17];
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  const uint32_t v = i < m ? (flag[i] & 1u) : 0u;
+  uint32_t total;
+  block_excl_scan_1024(v, s_w, &total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_pack_offsets(int blocks, uint32_t* sums) {
+  __shared__ uint32_t s_w[17];
+  __shared__ uint32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < blocks; base += 1024) {
+    const int k = base + threadIdx.x;
+    const uint32_t v = k < blocks ? sums[k] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_excl_scan_1024(v, s_w, &total);
+    const uint32_t carry = s_carry;
+    if (k < blocks) sums[k] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry = carry + total;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_pack_index(int m, const uint8_t* __restrict__ flag,
+                                                     const uint32_t* __restrict__ offsets, int32_t* __restrict__ wcomp) {
+  __shared__ uint32_t s_w[17];
+  const int i = blockIdx.x * 1024 + threadIdx.x;
+  const uint32_t v = i < m ? (flag[i] & 1u) : 0u;
+  const uint32_t r = block_excl_scan_1024(v, s_w, nullptr);
+  if (i < m) wcomp[i] = v ? (int32_t)(offsets[blockIdx.x] + r) : -1;
+}
+
 // Fill the 4-wide node of every even-depth internal node from the binary
 // topology and the refit boxes (node numbering of the reference: internal
 // [0, n-2], leaf of sorted position p at n-1+p).
@@ -806,7 +870,7 @@ __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restr
                                                    const int32_t* __restrict__ right,
                                                    const float* __restrict__ node_box,
                                                    const uint8_t* __restrict__ flag, float4* __restrict__ wide,
-                                                   float4* __restrict__ wideq) {
+                                                   float4* __restrict__ wideq, const int32_t* __restrict__ wcomp) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n - 1 || !(flag[i] & 1)) return;
   int slot[4];
@@ -883,11 +947,15 @@ __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restr
       qhi[a] |= qh << (8 * j);
     }
   }
-  float4* q = wideq + 4 * (size_t)i;
+  // packed: this node's slot, and its internal children's slots as refs
+  float4* q = wideq + 4 * (size_t)wcomp[i];
   q[0] = make_float4(nb[0], nb[1], nb[2], __uint_as_float(ebits));
   q[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]), 0.f);
   q[2] = make_float4(__uint_as_float(qhi[0]), __uint_as_float(qhi[1]), __uint_as_float(qhi[2]), 0.f);
-  q[3] = refs;
+  int cref[4];
+  for (int j = 0; j < 4; ++j) cref[j] = ref[j] >= 0 && ref[j] != kWideEmpty ? wcomp[ref[j]] : ref[j];
+  q[3] = make_float4(__int_as_float(cref[0]), __int_as_float(cref[1]), __int_as_float(cref[2]),
+                     __int_as_float(cref[3]));
 }
 
 // Quantize up to W child boxes against the node box nb: per axis the
@@ -991,9 +1059,15 @@ inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
   const int n = b->n;
   if (n < 2) return;
-  if (topology_changed) k_wide_parity<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->parent, b->wide_flag);
+  if (topology_changed) {
+    const int m = n - 1, blocks = cdiv(m, 1024);
+    k_wide_parity<<<cdiv(m, 256), 256, 0, s>>>(n, b->parent, b->wide_flag);
+    k_pack_count<<<blocks, 1024, 0, s>>>(m, b->wide_flag, b->wscan);
+    k_pack_offsets<<<1, 1024, 0, s>>>(blocks, b->wscan);
+    k_pack_index<<<blocks, 1024, 0, s>>>(m, b->wide_flag, b->wscan, b->wcomp);
+  }
   k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag,
-                                               b->want_wide4 ? b->wide : nullptr, b->wideq);
+                                               b->want_wide4 ? b->wide : nullptr, b->wideq, b->wcomp);
   if (b->want_wide8)
     k_wide8_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
 }
@@ -1090,7 +1164,8 @@ hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s) {
   if (layout == 4 && !b->want_wide4) {
     b->want_wide4 = true;
     if (n >= 2)
-      k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide, nullptr);
+      k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide, nullptr,
+                                                   b->wcomp);
   } else if (layout == 8 && !b->want_wide8) {
     b->want_wide8 = true;
     if (n >= 2)

@@ -58,7 +58,7 @@ struct BvhView {
   const float4* tnode;     // (n-1)*4: child boxes and child refs (64 B per internal node)
   const float4* ltri;      // n*3: vertices a, b, c of the leaf's triangle + triangle id (48 B)
   const float4* wide;      // (n-1)*8: 4-wide nodes (128 B), valid at even-depth internal nodes
-  const float4* wideq;     // (n-1)*4: the same nodes with 8-bit quantized child boxes (64 B)
+  const float4* wideq;     // 8-bit quantized 4-wide nodes (64 B), packed: node k is the k-th even-depth node
   const float4* wide8;     // (n-1)*8: 8-wide quantized nodes (128 B), valid at depths divisible by 3
   const int32_t* leaf_of;  // n: sorted leaf position of each triangle (inverse of the sorted ids)
 };
@@ -130,7 +130,9 @@ struct rt_bvh {
   float4* ltri = nullptr;        // n*3
   float4* trirec = nullptr;      // n*3: the leaf records in triangle order (written with the triangle boxes)
   float4* wide = nullptr;        // (n-1)*8
-  float4* wideq = nullptr;       // (n-1)*4
+  float4* wideq = nullptr;       // (n-1)*4 reserved; the packed nodes fill the front
+  int32_t* wcomp = nullptr;      // n-1: packed index of each even-depth internal node
+  uint32_t* wscan = nullptr;     // per-1024-node block sums of the packing scan
   float4* wide8 = nullptr;       // (n-1)*8
   uint8_t* wide_flag = nullptr;  // n-1: bit 0 = even depth (4-wide node), bit 1 = depth divisible by 3 (8-wide)
   int32_t* leaf_of = nullptr;    // n: triangle id -> sorted leaf position
