@@ -80,7 +80,9 @@ struct BvhView {
 //   [0] corner.xyz, step exponents (biased as fp32 exponents) in bytes 0-2
 //   [1] qlo of children 0..3 (byte j = child j) for x, y, z; unused
 //   [2] qhi likewise
-//   [3] child refs as in the 128-B form
+//   [3] child refs as in the 128-B form, except that a wide node is named
+//       by its packed slot: the quantized nodes are stored densely, the k-th
+//       even-depth internal node (in index order) at slot k
 // 8-wide quantized node (128 B), stored at the binary internal nodes whose
 // depth is a multiple of 3; its up to eight children are the binary
 // descendants three levels down (or the leaves met before), in sorted-leaf
