@@ -367,7 +367,7 @@ __global__ __launch_bounds__(256) void k_hierarchy(const K* __restrict__ keys, c
 #pragma unroll
     for (int k = 0; k < 6; ++k) dst[k] = src[k];
     int a = idx[3 * tri + 0], b = idx[3 * tri + 1], c = idx[3 * tri + 2];
-    float4* L = ltri + 3 * (size_t)i;
+    float4* L = ltri + kLeafStride * (size_t)i;
     L[0] = make_float4(vx[a], vy[a], vz[a], vx[b]);
     L[1] = make_float4(vy[b], vz[b], vx[c], vy[c]);
     L[2] = make_float4(vz[c], __int_as_float(tri), 0.f, 0.f);
@@ -483,7 +483,7 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const float* __restrict__ 
     t[k] = bb[k];
     d[k] = bb[k];
   }
-  float4* L = ltri + 3 * (size_t)i;
+  float4* L = ltri + kLeafStride * (size_t)i;
   L[0] = make_float4(ax, ay, az, bx);
   L[1] = make_float4(by, bz, cx, cy);
   L[2] = make_float4(cz, __int_as_float(tri), 0.f, 0.f);
@@ -604,7 +604,7 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
     int tri = (int)ids[i];
     const float4* R = trirec + 3 * (size_t)tri;
     const float4 q0 = R[0], q1 = R[1], q2 = R[2];
-    float4* L = ltri + 3 * (size_t)i;
+    float4* L = ltri + kLeafStride * (size_t)i;
     L[0] = q0;
     L[1] = q1;
     L[2] = q2;

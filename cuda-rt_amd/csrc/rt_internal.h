@@ -56,7 +56,7 @@ struct BvhView {
   int32_t n;
   const float* node_box;   // (2n-1)*6: lo xyz, hi xyz
   const float4* tnode;     // (n-1)*4: child boxes and child refs (64 B per internal node)
-  const float4* ltri;      // n*3: vertices a, b, c of the leaf's triangle + triangle id (48 B)
+  const float4* ltri;      // n*kLeafStride: vertices a, b, c of the leaf's triangle + triangle id (48 B used)
   const float4* wide;      // (n-1)*8: 4-wide nodes (128 B), valid at even-depth internal nodes
   const float4* wideq;     // 8-bit quantized 4-wide nodes (64 B), packed: node k is the k-th even-depth node
   const float4* wide8;     // (n-1)*8: 8-wide quantized nodes (128 B), valid at depths divisible by 3
@@ -129,7 +129,7 @@ struct rt_bvh {
   uint32_t* flags = nullptr;     // max(n-1,1)
   float* node_box = nullptr;     // (2n-1)*6
   float4* tnode = nullptr;       // (n-1)*4
-  float4* ltri = nullptr;        // n*3
+  float4* ltri = nullptr;        // n*kLeafStride
   float4* trirec = nullptr;      // n*3: the leaf records in triangle order (written with the triangle boxes)
   float4* wide = nullptr;        // (n-1)*8
   float4* wideq = nullptr;       // (n-1)*4 reserved; the packed nodes fill the front
@@ -158,6 +158,14 @@ constexpr int kMiscFetch = 59;      // 1 word: ray fetch counter of rt_trace
 constexpr int kMiscDeferredB = 60;  // 1 word each: hand-off counts of the top-level passes
 constexpr int kMiscDeferredC = 61;  //   (kMiscDeferred, kMiscDeferredB, kMiscDeferredC in rotation)
 constexpr int kMiscWords = 64;
+
+// Leaf records are kLeafStride float4 apart: the 48-B record (three
+// vertices + triangle id) padded to 64 B, so no record straddles a
+// 128-B cache line.
+#ifndef RT_LEAF_STRIDE
+#define RT_LEAF_STRIDE 4
+#endif
+constexpr int kLeafStride = RT_LEAF_STRIDE;
 
 // Fused hierarchy + refit climb: one workgroup per chunk of sorted leaves.
 #ifndef RT_CLIMB_THREADS

@@ -124,7 +124,7 @@ __device__ __forceinline__ void consider_leaf(const BvhView& B, const RayD& r, i
                                               HitRec& best, TravStats& st) {
   double t, al, be;
   ++st.tests;
-  if (!tri_test(r, B.ltri + 3 * (size_t)pos, t, al, be)) return;
+  if (!tri_test(r, B.ltri + kLeafStride * (size_t)pos, t, al, be)) return;
   if (tmin_phys > 0 && !(t >= tmin_phys)) return;
   if (best.pos < 0 || t < best.t || (t == best.t && pos < best.pos)) {
     best.t = t;
@@ -276,7 +276,7 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
     if (hl) {
       if (cl < 0) {
         ++st.tests;
-        if (tri_test(r, B.ltri + 3 * (size_t)(~cl), t, al, be)) {
+        if (tri_test(r, B.ltri + kLeafStride * (size_t)(~cl), t, al, be)) {
           hit.t = t; hit.al = al; hit.be = be; hit.pos = ~cl;
           return true;
         }
@@ -292,7 +292,7 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
     if (!have_next && hr) {
       if (cr < 0) {
         ++st.tests;
-        if (tri_test(r, B.ltri + 3 * (size_t)(~cr), t, al, be)) {
+        if (tri_test(r, B.ltri + kLeafStride * (size_t)(~cr), t, al, be)) {
           hit.t = t; hit.al = al; hit.be = be; hit.pos = ~cr;
           return true;
         }
@@ -306,7 +306,7 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
       int ref = stack_pop(S, spill);
       if (ref < 0) {
         ++st.tests;
-        if (tri_test(r, B.ltri + 3 * (size_t)(~ref), t, al, be)) {
+        if (tri_test(r, B.ltri + kLeafStride * (size_t)(~ref), t, al, be)) {
           hit.t = t; hit.al = al; hit.be = be; hit.pos = ~ref;
           return true;
         }
@@ -322,11 +322,11 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
 }
 
 __device__ __forceinline__ int32_t leaf_tri(const BvhView& B, int pos) {
-  return __float_as_int(B.ltri[3 * (size_t)pos + 2].y);
+  return __float_as_int(B.ltri[kLeafStride * (size_t)pos + 2].y);
 }
 
 __device__ __forceinline__ int32_t leaf_tri_of(const BvhView& B, int pos) {
-  return __float_as_int(ldgf((const float*)(B.ltri + 3 * (size_t)pos + 2) + 1));
+  return __float_as_int(ldgf((const float*)(B.ltri + kLeafStride * (size_t)pos + 2) + 1));
 }
 
 // Work counts of a traversal launch: child-box tests to stats[0], triangle
@@ -677,7 +677,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
         if (has_pending) {
           const int pos = ~pending;
           pending = kNoNode;
-          const float4* L = B.ltri + 3 * (size_t)pos;
+          const float4* L = B.ltri + kLeafStride * (size_t)pos;
           bool exact = true;
           if (ANY && QUANT) {
             // quantized nodes hold no exact leaf boxes: decide it here
@@ -901,7 +901,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
       if (node < 0) {
         double t, al, be;
         ++st.tests;
-        if (tri_test(r, B.ltri + 3 * (size_t)(~node), t, al, be)) {
+        if (tri_test(r, B.ltri + kLeafStride * (size_t)(~node), t, al, be)) {
           bt = t; bal = al; bbe = be; bpos = ~node;
           done = true;
         } else if (S.sp == 0) {
@@ -2109,7 +2109,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint(BvhView B, MeshShade M, co
         b.d = d;
         b.tri = leaf_tri_of(B, r.x);
         RayD ray = make_ray(o, d);
-        tri_test(ray, B.ltri + 3 * (size_t)r.x, b.t, b.al, b.be);
+        tri_test(ray, B.ltri + kLeafStride * (size_t)r.x, b.t, b.al, b.be);
         b.lpoint = light_point(P, rng);
         const d3 N0 = shading_normal(M, b.tri, b.al, b.be);
         const LightTerms T = light_terms(P, N0, o + d * b.t, b.lpoint);
