@@ -514,6 +514,7 @@ struct RayIn {
   size_t comp;         // between components, in doubles (1: xyz interleaved; the queue capacity: SoA)
   const uint32_t* count_dev;  // number of rays (device), or nullptr
   uint32_t count;             // used when count_dev is nullptr
+  const uint32_t* perm;       // optional fetch order: the k-th ray fetched is ray perm[k]
 };
 
 struct HitOut {
@@ -624,7 +625,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
         }
       }
       if (myid != 0xffffffffu) {
-        rid = myid;
+        rid = R.perm ? R.perm[myid] : myid;
         if (MULTI) {
           B = views[scene[rid]];
           n = B.n;
@@ -1305,6 +1306,10 @@ struct WfState {
   int32_t* spos;        // cap: any-hit leaf position or -1
   double* st;           // cap: any-hit t
   uint32_t* scount;     // number of shadow entries (counts[2] or counts[3])
+  uint16_t* skey;       // cap: direction cell of each shadow ray (single-scene renders)
+  uint32_t* sperm;      // cap: shadow rays in cell order
+  uint32_t* shist;      // kShadowCells: cell counts (zero between sorts)
+  uint32_t* sbase;      // kShadowCells: cell offsets
   uint32_t* counts;     // [0],[1] ray queue sizes, [2],[3] shadow entries per parity, [8] stack overflows
   uint32_t* fetch[2];   // kFetchWords each: closest-hit and any-hit fetch counters (the latter per parity)
   int2* rec;            // optional path record: (hit leaf, occluder code) per pixel, sample, bounce
@@ -1321,6 +1326,96 @@ struct WfState {
 // triangle tests of the closest-hit traversal; 5, 6 those of the any-hit one.
 constexpr int kCtrReplicas = 64;
 constexpr int kCtrStride = 16;
+
+// Shadow rays are traced in the order of their direction from the light
+// (single-scene renders): lines through nearby cells share BVH nodes, so
+// the any-hit pass finds more of them in cache. A cell is one of
+// kShadowGrid x kShadowGrid on the square that the scene box projects onto,
+// seen from the light, numbered in Morton order. 32 x 32 measured fastest on
+// a 1M-triangle soup at 1024 x 1024: 16 x 16 sorts too coarsely, 64 x 64
+// costs more in the sort than the any-hit pass gains.
+#ifndef RT_SHADOW_SORT
+#define RT_SHADOW_SORT 1
+#endif
+constexpr bool kShadowSort = RT_SHADOW_SORT != 0;
+#ifndef RT_SHADOW_GRID
+#define RT_SHADOW_GRID 32
+#endif
+constexpr int kShadowGrid = RT_SHADOW_GRID;  // cells per side, a power of two
+constexpr int kShadowBits = __builtin_ctz(kShadowGrid);
+constexpr int kShadowCells = kShadowGrid * kShadowGrid;
+constexpr int kScanThreads = 1024;
+static_assert((kShadowGrid & (kShadowGrid - 1)) == 0 && kShadowCells >= 64, "shadow grid");
+struct ShadowCells {
+  float lx, ly, lz;           // light centre
+  float wx, wy, wz;           // unit direction from the light to the box centre
+  float ex, ey, ez, fx, fy, fz;  // two unit vectors across it
+  float a0, as, b0, bs;       // cell = ((a - a0) * as, (b - b0) * bs) of the projection (a, b)
+  bool ok;                    // false: a box corner is not in front of the light (every key 0)
+};
+__device__ __forceinline__ ShadowCells shadow_cells(const RenderParams& P, const float* root_box) {
+  ShadowCells C;
+  C.lx = (float)P.lpos[0];
+  C.ly = (float)P.lpos[1];
+  C.lz = (float)P.lpos[2];
+  float cx = 0.5f * (root_box[0] + root_box[3]) - C.lx, cy = 0.5f * (root_box[1] + root_box[4]) - C.ly,
+        cz = 0.5f * (root_box[2] + root_box[5]) - C.lz;
+  float len = sqrtf(cx * cx + cy * cy + cz * cz);
+  C.ok = len > 0.f;
+  if (!C.ok || !(len < INFINITY)) {
+    C.ok = false;
+    return C;
+  }
+  C.wx = cx / len;
+  C.wy = cy / len;
+  C.wz = cz / len;
+  // e = normalize(up x w), f = w x e, with up the axis least aligned with w
+  float ux = 0.f, uy = 0.f, uz = 1.f;
+  if (fabsf(C.wz) > 0.9f) {
+    ux = 1.f;
+    uz = 0.f;
+  }
+  float ex = uy * C.wz - uz * C.wy, ey = uz * C.wx - ux * C.wz, ez = ux * C.wy - uy * C.wx;
+  float el = sqrtf(ex * ex + ey * ey + ez * ez);
+  C.ex = ex / el;
+  C.ey = ey / el;
+  C.ez = ez / el;
+  C.fx = C.wy * C.ez - C.wz * C.ey;
+  C.fy = C.wz * C.ex - C.wx * C.ez;
+  C.fz = C.wx * C.ey - C.wy * C.ex;
+  float amin = INFINITY, amax = -INFINITY, bmin = INFINITY, bmax = -INFINITY;
+  for (int k = 0; k < 8; ++k) {
+    const float vx = root_box[(k & 1) ? 3 : 0] - C.lx, vy = root_box[(k & 2) ? 4 : 1] - C.ly,
+                vz = root_box[(k & 4) ? 5 : 2] - C.lz;
+    const float z = vx * C.wx + vy * C.wy + vz * C.wz;
+    if (!(z > 1e-3f * len)) C.ok = false;
+    const float a = (vx * C.ex + vy * C.ey + vz * C.ez) / z, b = (vx * C.fx + vy * C.fy + vz * C.fz) / z;
+    amin = fminf(amin, a);
+    amax = fmaxf(amax, a);
+    bmin = fminf(bmin, b);
+    bmax = fmaxf(bmax, b);
+  }
+  C.a0 = amin;
+  C.b0 = bmin;
+  C.as = amax > amin ? (float)kShadowGrid / (amax - amin) : 0.f;
+  C.bs = bmax > bmin ? (float)kShadowGrid / (bmax - bmin) : 0.f;
+  return C;
+}
+__device__ __forceinline__ uint16_t shadow_key(const ShadowCells& C, d3 q) {
+  if (!C.ok) return 0;
+  const float vx = (float)q.x - C.lx, vy = (float)q.y - C.ly, vz = (float)q.z - C.lz;
+  const float z = vx * C.wx + vy * C.wy + vz * C.wz;
+  if (!(z > 0.f)) return 0;
+  const float a = (vx * C.ex + vy * C.ey + vz * C.ez) / z, b = (vx * C.fx + vy * C.fy + vz * C.fz) / z;
+  const float top = (float)(kShadowGrid - 1);
+  const uint32_t ia = (uint32_t)fminf(top, fmaxf(0.f, (a - C.a0) * C.as)),
+                 ib = (uint32_t)fminf(top, fmaxf(0.f, (b - C.b0) * C.bs));
+  uint32_t k = 0;
+#pragma unroll
+  for (int bit = 0; bit < kShadowBits; ++bit)
+    k |= ((ia >> bit) & 1u) << (2 * bit) | ((ib >> bit) & 1u) << (2 * bit + 1);
+  return (uint16_t)k;
+}
 __device__ __forceinline__ void rep_count(unsigned long long* rep, int k, bool pred) {
   const uint64_t m = __ballot(pred);
   if ((threadIdx.x & 63) == 0 && m)
@@ -1395,6 +1490,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_init(WfState W, SceneDesc D1) {
   if (blockIdx.x == 0) {
     if (threadIdx.x < 16) W.counts[threadIdx.x] = 0u;
     for (int k = threadIdx.x; k < kCtrReplicas * kCtrStride; k += kThreads) W.rep[k] = 0ull;
+    for (int k = threadIdx.x; k < kShadowCells; k += kThreads) W.shist[k] = 0u;
   }
   int s, px, py;
   uint32_t gp;
@@ -1469,6 +1565,9 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
   ShadePrep S;
   d3 thr = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
   uint16_t sc = 0;
+  ShadowCells cells;
+  cells.ok = false;
+  if (!MULTI && kShadowSort && D1.B.n > 0) cells = shadow_cells(D1.P, D1.B.node_box);
   if (live) {
     sc = MULTI ? W.rscene[cur][i] : 0;
     const SceneDesc& D = scene_of<MULTI>(W, D1, sc);
@@ -1502,6 +1601,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
   if (want_shadow) {
     W.spix[ss] = p;
     W.sscene[ss] = sc;
+    if (!MULTI && kShadowSort) W.skey[ss] = shadow_key(cells, S.q);
     // shadow entry, component-major: q, L, I1, ds, thr, fo
     double* e = W.shq + ss;
     const size_t c = W.cap;
@@ -1521,6 +1621,72 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
   }
   // rays traced by the closest-hit pass of this bounce
   if (counters) rep_count(W.rep, bounce == 0 ? 0 : 2, live);
+}
+
+// Counting sort of a sample's shadow queue by cell key, in three passes:
+// per-cell counts, their exclusive scan, and the scatter of slot indices.
+// The order within a cell depends on atomic timing; the any-hit pass writes
+// every result to the ray's own slot, so the image does not depend on it.
+__global__ __launch_bounds__(kThreads) void k_sort_hist(WfState W) {
+  __shared__ uint32_t h[kShadowCells];
+  for (int k = threadIdx.x; k < kShadowCells; k += kThreads) h[k] = 0u;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t n = *W.scount;
+  if (i < n) atomicAdd(&h[W.skey[i]], 1u);
+  __syncthreads();
+  if (blockIdx.x * kThreads >= n) return;
+  for (int k = threadIdx.x; k < kShadowCells; k += kThreads)
+    if (h[k]) atomicAdd(&W.shist[k], h[k]);
+}
+
+// One workgroup; thread k scans cells [k*per, (k+1)*per).
+__global__ __launch_bounds__(kScanThreads) void k_sort_scan(WfState W) {
+  constexpr int per = kShadowCells >= kScanThreads ? kShadowCells / kScanThreads : 1;
+  __shared__ uint32_t part[kScanThreads / 64];
+  const int k = threadIdx.x, lane = k & 63, wv = k >> 6;
+  const int c0 = k * per;
+  uint32_t v[per], sum = 0;
+#pragma unroll
+  for (int j = 0; j < per; ++j) {
+    v[j] = c0 + j < kShadowCells ? W.shist[c0 + j] : 0u;
+    sum += v[j];
+  }
+  uint32_t x = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) part[wv] = x;
+  __syncthreads();
+  uint32_t run = x - sum;
+  for (int j = 0; j < wv; ++j) run += part[j];
+#pragma unroll
+  for (int j = 0; j < per; ++j) {
+    if (c0 + j < kShadowCells) {
+      W.sbase[c0 + j] = run;
+      W.shist[c0 + j] = 0u;  // ready for the next sort of this parity
+    }
+    run += v[j];
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_sort_scatter(WfState W) {
+  __shared__ uint32_t h[kShadowCells];
+  for (int k = threadIdx.x; k < kShadowCells; k += kThreads) h[k] = 0u;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
+  const uint32_t n = *W.scount;
+  if (blockIdx.x * kThreads >= n) return;
+  const uint32_t key = i < n ? W.skey[i] : 0u;
+  const uint32_t rank = i < n ? atomicAdd(&h[key], 1u) : 0u;
+  __syncthreads();
+  // one global reservation per non-empty cell of the block
+  for (int k = threadIdx.x; k < kShadowCells; k += kThreads)
+    if (h[k]) h[k] = atomicAdd(&W.sbase[k], h[k]);
+  __syncthreads();
+  if (i < n) W.sperm[h[key] + rank] = i;
 }
 
 // Occluder code of a shadow query: -1 = unshadowed (factor 1), -2 = opaque
@@ -1749,7 +1915,7 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
 // Workspace layout of a wavefront render over `nscenes` scenes with `npix`
 // pixels and `ntiles` tiles in total.
 struct WfLayout {
-  size_t off[32];
+  size_t off[40];
   size_t total;
 };
 
@@ -1761,7 +1927,10 @@ static WfLayout wf_layout(size_t npix, size_t ntiles, size_t nscenes) {
                     cap * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, 256, kFetchWords * 4, kFetchWords * 4,
                     kCtrReplicas * kCtrStride * 8,
                     // parity 1: accumulator, shadow queue, any-hit fetch counters
-                    npix * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, kFetchWords * 4};
+                    npix * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, kFetchWords * 4,
+                    // shadow-ray sort, per parity: keys, order, cell counts, cell offsets
+                    cap * 2, cap * 4, kShadowCells * 4, kShadowCells * 4,
+                    cap * 2, cap * 4, kShadowCells * 4, kShadowCells * 4};
   size_t off = 0;
   for (int k = 0; k < (int)(sizeof(sizes) / sizeof(sizes[0])); ++k) {
     L.off[k] = off;
@@ -1814,7 +1983,14 @@ static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes, in
     W.st = (double*)(w + L.off[k++]);
     W.fetch[1] = (uint32_t*)(w + L.off[k++]);
     W.scount = W.counts + 3;
+  } else {
+    k += 7;
   }
+  k += parity ? 4 : 0;
+  W.skey = (uint16_t*)(w + L.off[k++]);
+  W.sperm = (uint32_t*)(w + L.off[k++]);
+  W.shist = (uint32_t*)(w + L.off[k++]);
+  W.sbase = (uint32_t*)(w + L.off[k++]);
   W.npix = npix;
   W.cap = ntiles * kThreads;
   return W;
@@ -1920,6 +2096,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     k_wf_init<false><<<tiles, kThreads, 0, s>>>(W0, D1);
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W0.fetch[0], 2 * kFetchWords);
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[1].fetch[1], kFetchWords);
+  if (!multi) k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[1].shist, kShadowCells);
   // The last bounce's any-hit pass, shadow resolve and flush of sample k run
   // on the auxiliary stream, overlapping sample k+1's camera, closest-hit
   // and shading passes on s: they share no buffer but the replicated
@@ -1954,7 +2131,13 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
         RT_TRY(hipEventRecord(ov->shaded, s));
         RT_TRY(hipStreamWaitEvent(sa, ov->shaded, 0));
       }
-      RayIn SR{W.shq, W.shq + 3 * cap, 1, cap, W.scount, 0};
+      const bool ssort = !multi && kShadowSort;
+      RayIn SR{W.shq, W.shq + 3 * cap, 1, cap, W.scount, 0, ssort ? W.sperm : nullptr};
+      if (ssort) {
+        k_sort_hist<<<tiles, kThreads, 0, sb>>>(W);
+        k_sort_scan<<<1, kScanThreads, 0, sb>>>(W);
+        k_sort_scatter<<<tiles, kThreads, 0, sb>>>(W);
+      }
       HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr, 0};
       if (T) RT_TRY(timer_mark(T, 1, true, sb));
       launch_persistent<true>(t, grid, sb, W.views, nscenes > 1 ? W.sscene : nullptr, SR, 0.0, SO, W.fetch[1],
