@@ -949,9 +949,12 @@ __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restr
   }
   // packed: this node's slot, and its internal children's slots as refs
   float4* q = wideq + 4 * (size_t)wcomp[i];
-  q[0] = make_float4(nb[0], nb[1], nb[2], __uint_as_float(ebits));
-  q[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]), 0.f);
-  q[2] = make_float4(__uint_as_float(qhi[0]), __uint_as_float(qhi[1]), __uint_as_float(qhi[2]), 0.f);
+  // the steps as floats (2^(e-127), or 0), one in each spare word
+  q[0] = make_float4(nb[0], nb[1], nb[2], __uint_as_float((ebits & 0xffu) << 23));
+  q[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]),
+                     __uint_as_float(((ebits >> 8) & 0xffu) << 23));
+  q[2] = make_float4(__uint_as_float(qhi[0]), __uint_as_float(qhi[1]), __uint_as_float(qhi[2]),
+                     __uint_as_float(((ebits >> 16) & 0xffu) << 23));
   int cref[4];
   for (int j = 0; j < 4; ++j) cref[j] = ref[j] >= 0 && ref[j] != kWideEmpty ? wcomp[ref[j]] : ref[j];
   q[3] = make_float4(__int_as_float(cref[0]), __int_as_float(cref[1]), __int_as_float(cref[2]),

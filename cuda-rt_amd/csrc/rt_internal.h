@@ -77,9 +77,9 @@ struct BvhView {
 // per axis, a power-of-two step; child planes are lo = corner + qlo*step
 // (rounded down) and hi = corner + qhi*step (rounded up), so the decoded
 // child boxes contain the exact ones:
-//   [0] corner.xyz, step exponents (biased as fp32 exponents) in bytes 0-2
-//   [1] qlo of children 0..3 (byte j = child j) for x, y, z; unused
-//   [2] qhi likewise
+//   [0] corner.xyz, step.x (a float: a power of two, or 0 for a flat axis)
+//   [1] qlo of children 0..3 (byte j = child j) for x, y, z; step.y
+//   [2] qhi likewise; step.z
 //   [3] child refs as in the 128-B form, except that a wide node is named
 //       by its packed slot: the quantized nodes are stored densely, the k-th
 //       even-depth internal node (in index order) at slot k

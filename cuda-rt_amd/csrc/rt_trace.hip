@@ -412,10 +412,7 @@ __device__ __forceinline__ void wide_slabs(const RayD& r, const RayF& f, const f
 // folded into t = q * (step / d) + (corner / d - offset).
 __device__ __forceinline__ void wideq_slabs(const RayD& r, const RayF& f, const float4 (&q)[4], float (&t0)[4],
                                             float (&t1)[4]) {
-  const uint32_t eb = __float_as_uint(q[0].w);
-  const float sx = __uint_as_float((eb & 0xffu) << 23), sy = __uint_as_float(((eb >> 8) & 0xffu) << 23),
-              sz = __uint_as_float(((eb >> 16) & 0xffu) << 23);
-  const float ax = sx * f.ix, ay = sy * f.iy, az = sz * f.iz;
+  const float ax = q[0].w * f.ix, ay = q[1].w * f.iy, az = q[2].w * f.iz;  // step / d
   const float cnx = fmaf(q[0].x, f.ix, -f.nx), cfx = fmaf(q[0].x, f.ix, -f.fx);
   const float cny = fmaf(q[0].y, f.iy, -f.ny), cfy = fmaf(q[0].y, f.iy, -f.fy);
   const float cnz = fmaf(q[0].z, f.iz, -f.nz), cfz = fmaf(q[0].z, f.iz, -f.fz);
@@ -540,6 +537,24 @@ __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uin
 }
 
 // views[scene[i]] is the BVH of ray i (scene == nullptr: views[0] for all).
+// Closest-hit child keys: key = entry distance of a child the ray enters
+// within [low, lim], +inf otherwise; returns the number of such children.
+// LOW = false leaves out the test against low (for low = -inf).
+template <bool LOW, int NW>
+__device__ __forceinline__ int child_keys(const float (&t0)[NW], const float (&t1)[NW], const int (&cref)[NW],
+                                          float lim, float low, float (&key)[NW], int (&ref)[NW]) {
+  int m = 0;
+#pragma unroll
+  for (int j = 0; j < NW; ++j) {
+    ref[j] = cref[j];
+    bool h = (ref[j] != kWideEmpty) & (t0[j] <= t1[j]) & (t0[j] <= lim);
+    if (LOW) h = h & (t1[j] >= low);
+    key[j] = h ? t0[j] : __int_as_float(0x7f800000);
+    m += h ? 1 : 0;
+  }
+  return m;
+}
+
 // LAYOUT 2: binary nodes, FP64 child tests; 4: 4-wide nodes (128 B);
 // 5: quantized 4-wide nodes (64 B).
 // Workgroups of the persistent traversal: one wave each, so a wave that has
@@ -550,7 +565,9 @@ __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uin
 #endif
 constexpr int kTraceThreads = RT_TRACE_THREADS;
 
-template <bool ANY, int RING, int LAYOUT, bool MULTI>
+// LOW = false: a closest-hit launch with tmin 0, whose child test can leave
+// out the lower bound (t1 >= -inf is implied by t0 <= t1).
+template <bool ANY, int RING, int LAYOUT, bool MULTI, bool LOW = true>
 __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhView* __restrict__ views,
                                                                const uint16_t* __restrict__ scene, RayIn R,
                                                                double tmin_phys, HitOut O, uint32_t* fetch_ctr,
@@ -762,14 +779,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
               tmin_phys > 0 ? f_down(tmin_phys - prune_margin(tmin_phys)) : __int_as_float(0xff800000);
           float key[NW];
           int ref[NW];
-          int m = 0;
-#pragma unroll
-          for (int j = 0; j < NW; ++j) {
-            ref[j] = cref[j];
-            const bool h = (ref[j] != kWideEmpty) & (t0[j] <= t1[j]) & (t0[j] <= lim) & (t1[j] >= low);
-            key[j] = h ? t0[j] : __int_as_float(0x7f800000);
-            m += h ? 1 : 0;
-          }
+          const int m = child_keys<LOW>(t0, t1, cref, lim, low, key, ref);
           // nearest child next, the others pushed far to near (sorting
           // networks of 5 and 19 comparators)
 #define RT_CSWAP(a, b)                                  \
@@ -1871,14 +1881,25 @@ static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const
                               const uint16_t* scene, RayIn R, double tmin, HitOut O, uint32_t* fetch,
                               uint32_t* overflow, unsigned long long* stats) {
   const uint32_t fc = t.fetch_chunk;
+  // the default layout has closest-hit kernels without the lower bound
+  // for launches with tmin 0
 #define RT_LAUNCH(RINGV, LAYOUTV)                                                                                  \
   do {                                                                                                              \
-    if (scene)                                                                                                      \
+    constexpr bool kNoLow = !ANY && LAYOUTV == 5;                                                                   \
+    if (kNoLow && !(tmin > 0)) {                                                                                    \
+      if (scene)                                                                                                    \
+        k_trace_persistent<ANY, RINGV, LAYOUTV, true, !kNoLow>                                                      \
+            <<<grid, kTraceThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                 \
+      else                                                                                                          \
+        k_trace_persistent<ANY, RINGV, LAYOUTV, false, !kNoLow>                                                     \
+            <<<grid, kTraceThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                 \
+    } else if (scene) {                                                                                             \
       k_trace_persistent<ANY, RINGV, LAYOUTV, true>                                                                 \
           <<<grid, kTraceThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                   \
-    else                                                                                                            \
+    } else {                                                                                                        \
       k_trace_persistent<ANY, RINGV, LAYOUTV, false>                                                                \
           <<<grid, kTraceThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                   \
+    }                                                                                                               \
   } while (0)
   if (t.node_layout == 8) {
     if (t.stack_ring == 16) RT_LAUNCH(16, 8); else RT_LAUNCH(32, 8);
