@@ -134,41 +134,83 @@ __device__ __forceinline__ void consider_leaf(const BvhView& B, const RayD& r, i
   }
 }
 
-// Traversal stack: a per-lane ring of kRing entries in LDS (one column per
-// thread, conflict-free) backed by a private spill area that is only touched
-// when a path is more than kRing levels deep. Depth beyond kRing + kSpill is
-// reported as an overflow and the ray as a miss.
+// Traversal stack: per lane, the top RING entries in LDS (one column per
+// thread: entry k of a lane at col[k * STRIDE], conflict-free) over a
+// private spill area; depth beyond RING + kSpill is reported as an overflow
+// and the ray as a miss. Two ways to keep the LDS part:
+// - LIN (linear): when the LDS part is full, its lower half moves to the
+//   spill area, and it comes back when the LDS part runs empty, so a push or
+//   pop addresses col[d * STRIDE] with no wrap-around;
+// - ring: entry k at col[(k % RING) * STRIDE]; past RING entries, each push
+//   moves the oldest LDS entry to the spill area and each pop below it
+//   brings one back.
+// The persistent closest-hit traversal (stacks mostly shallower than RING)
+// gains from the cheaper addressing; the any-hit traversal, whose stacks
+// along whole lines often pass RING entries, from the single-entry spills.
+// The per-pixel megakernel (binary nodes) keeps the ring.
 constexpr int kThreads = 256;
 constexpr int kRing = 32;
 constexpr int kSpill = 64;
 
 struct RingStack {
-  int* col;  // this lane's column of the workgroup's LDS ring
-  int sp = 0, lo = 0;
+  int* col;  // this lane's column of the workgroup's LDS stack
+  int d = 0;   // LIN: entries in LDS; ring: entries in all
+  int lo = 0;  // entries in the spill area (LIN: a multiple of RING / 2)
 };
 
+template <bool LIN = true>
+__device__ __forceinline__ bool stack_empty(const RingStack& S) {
+  return LIN ? (S.d | S.lo) == 0 : S.d == 0;
+}
+
 // STRIDE = threads per workgroup (one LDS column per thread).
-template <int RING = kRing, int STRIDE = kThreads>
+template <int RING = kRing, int STRIDE = kThreads, bool LIN = true>
 __device__ __forceinline__ bool stack_push(RingStack& S, int* spill, int v) {
-  if (S.sp - S.lo == RING) {
-    if (S.lo == kSpill) return false;
-    spill[S.lo] = S.col[(S.lo & (RING - 1)) * STRIDE];
-    ++S.lo;
+  static_assert(kSpill % (RING / 2) == 0, "the spill area holds whole halves");
+  if (!LIN) {
+    if (S.d - S.lo == RING) {
+      if (S.lo == kSpill) return false;
+      spill[S.lo] = S.col[(S.lo & (RING - 1)) * STRIDE];
+      ++S.lo;
+    }
+    S.col[(S.d & (RING - 1)) * STRIDE] = v;
+    ++S.d;
+    return true;
   }
-  S.col[(S.sp & (RING - 1)) * STRIDE] = v;
-  ++S.sp;
+  if (S.d == RING) {
+    if (S.lo == kSpill) return false;
+#pragma unroll
+    for (int k = 0; k < RING / 2; ++k) spill[S.lo + k] = S.col[k * STRIDE];
+#pragma unroll
+    for (int k = 0; k < RING / 2; ++k) S.col[k * STRIDE] = S.col[(k + RING / 2) * STRIDE];
+    S.lo += RING / 2;
+    S.d = RING / 2;
+  }
+  S.col[S.d * STRIDE] = v;
+  ++S.d;
   return true;
 }
 
-template <int RING = kRing, int STRIDE = kThreads>
+// Requires !stack_empty(S).
+template <int RING = kRing, int STRIDE = kThreads, bool LIN = true>
 __device__ __forceinline__ int stack_pop(RingStack& S, int* spill) {
-  --S.sp;
-  int v = S.col[(S.sp & (RING - 1)) * STRIDE];
-  if (S.sp == S.lo && S.lo > 0) {
-    --S.lo;
-    S.col[(S.lo & (RING - 1)) * STRIDE] = spill[S.lo];
+  if (!LIN) {
+    --S.d;
+    const int v = S.col[(S.d & (RING - 1)) * STRIDE];
+    if (S.d == S.lo && S.lo > 0) {
+      --S.lo;
+      S.col[(S.lo & (RING - 1)) * STRIDE] = spill[S.lo];
+    }
+    return v;
   }
-  return v;
+  if (S.d == 0) {
+    S.lo -= RING / 2;
+#pragma unroll
+    for (int k = 0; k < RING / 2; ++k) S.col[k * STRIDE] = spill[S.lo + k];
+    S.d = RING / 2;
+  }
+  --S.d;
+  return S.col[S.d * STRIDE];
 }
 
 // Closest hit (kernels/bvh.cu:382-414): minimum (t, leaf position) over all
@@ -221,7 +263,7 @@ __device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, H
     }
     if (hl && hr) {
       int nearc = (l0 <= r0) ? cl : cr, farc = (l0 <= r0) ? cr : cl;
-      if (!stack_push(S, spill, farc)) {
+      if (!stack_push<kRing, kThreads, false>(S, spill, farc)) {
         atomicAdd(overflow, 1u);
         best.pos = -1;
         return false;
@@ -232,8 +274,8 @@ __device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, H
     } else if (hr) {
       node = cr;
     } else {
-      if (S.sp == 0) return true;
-      node = stack_pop(S, spill);
+      if (stack_empty<false>(S)) return true;
+      node = stack_pop<kRing, kThreads, false>(S, spill);
     }
   }
   atomicAdd(overflow, 1u);
@@ -281,7 +323,7 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
           return true;
         }
       } else {
-        if (hr && !stack_push(S, spill, cr)) {
+        if (hr && !stack_push<kRing, kThreads, false>(S, spill, cr)) {
           atomicAdd(overflow, 1u);
           return false;
         }
@@ -302,8 +344,8 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
       }
     }
     while (!have_next) {
-      if (S.sp == 0) return false;
-      int ref = stack_pop(S, spill);
+      if (stack_empty<false>(S)) return false;
+      int ref = stack_pop<kRing, kThreads, false>(S, spill);
       if (ref < 0) {
         ++st.tests;
         if (tri_test(r, B.ltri + kLeafStride * (size_t)(~ref), t, al, be)) {
@@ -669,7 +711,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
           active = true;
           node = 0;
           pending = kNoNode;
-          S.sp = 0;
+          S.d = 0;
           S.lo = 0;
         }
       }
@@ -741,7 +783,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
       }
       if (!done && node != kNoNode && node < 0 && pending == kNoNode) {
         pending = node;
-        node = S.sp == 0 ? kNoNode : stack_pop<RING, kTraceThreads>(S, spill);
+        node = stack_empty<!ANY>(S) ? kNoNode : stack_pop<RING, kTraceThreads, !ANY>(S, spill);
       }
       if (!done && node != kNoNode && node >= 0) {
         constexpr int NW = LAYOUT == 8 ? 8 : 4;
@@ -796,39 +838,40 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
           }
 #undef RT_CSWAP
           if constexpr (NW == 4) {
-            if (S.sp - S.lo <= RING - 3) {
+            if (S.d <= RING - 3) {
               // room for three entries: push ref[m-1] .. ref[1] without a
-              // branch per entry. Slots sp .. sp+2 are free ring slots, so
-              // the writes past the m-1 kept entries are harmless.
+              // branch per entry. Slots d .. d+2 are free, so the writes
+              // past the m-1 kept entries are harmless.
               const int a0 = m >= 4 ? ref[3] : (m == 3 ? ref[2] : ref[1]);
               const int a1 = m >= 4 ? ref[2] : ref[1];
-              S.col[(S.sp & (RING - 1)) * kTraceThreads] = a0;
-              S.col[((S.sp + 1) & (RING - 1)) * kTraceThreads] = a1;
-              S.col[((S.sp + 2) & (RING - 1)) * kTraceThreads] = ref[1];
-              S.sp += m > 1 ? m - 1 : 0;
+              int* top = S.col + S.d * kTraceThreads;
+              top[0] = a0;
+              top[kTraceThreads] = a1;
+              top[2 * kTraceThreads] = ref[1];
+              S.d += m > 1 ? m - 1 : 0;
             } else {
 #pragma unroll
               for (int k = NW - 1; k >= 1; --k)
-                if (m > k) ovf = ovf || !stack_push<RING, kTraceThreads>(S, spill, ref[k]);
+                if (m > k) ovf = ovf || !stack_push<RING, kTraceThreads, !ANY>(S, spill, ref[k]);
             }
           } else {
 #pragma unroll
             for (int k = NW - 1; k >= 1; --k)
-              if (m > k) ovf = ovf || !stack_push<RING, kTraceThreads>(S, spill, ref[k]);
+              if (m > k) ovf = ovf || !stack_push<RING, kTraceThreads, !ANY>(S, spill, ref[k]);
           }
           if (m >= 1) next = ref[0];
         } else {
           // children in sorted-leaf order: the first passing one next, the
           // others pushed so that they pop in order (left-first DFS)
-          if (LAYOUT != 4 && NW == 4 && S.sp - S.lo <= RING - 3) {
+          if (LAYOUT != 4 && NW == 4 && S.d - S.lo <= RING - 3) {
             // room for three entries: plain ring writes, no per-entry check
 #pragma unroll
             for (int j = NW - 1; j >= 0; --j) {
               const int ref = cref[j];
               if ((ref != kWideEmpty) & (t0[j] <= t1[j])) {
                 if (next != kNoNode) {
-                  S.col[(S.sp & (RING - 1)) * kTraceThreads] = next;
-                  ++S.sp;
+                  S.col[(S.d & (RING - 1)) * kTraceThreads] = next;  // ring stack
+                  ++S.d;
                 }
                 next = ref;
               }
@@ -843,7 +886,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
                 p = wide_leaf_slab(r, w, j & 3, l0, l1);  // exact decision for a leaf child
               }
               if (p) {
-                if (next != kNoNode) ovf = ovf || !stack_push<RING, kTraceThreads>(S, spill, next);
+                if (next != kNoNode) ovf = ovf || !stack_push<RING, kTraceThreads, !ANY>(S, spill, next);
                 next = ref;
               }
             }
@@ -856,7 +899,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
         } else if (next != kNoNode) {
           node = next;
         } else {
-          node = S.sp == 0 ? kNoNode : stack_pop<RING, kTraceThreads>(S, spill);
+          node = stack_empty<!ANY>(S) ? kNoNode : stack_pop<RING, kTraceThreads, !ANY>(S, spill);
         }
       }
       if (!done && node == kNoNode && pending == kNoNode) done = true;
@@ -892,7 +935,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
       bpos = best.pos;
       if (hl && hr) {
         int nearc = (l0 <= r0) ? cl : cr, farc = (l0 <= r0) ? cr : cl;
-        if (!stack_push<RING, kTraceThreads>(S, spill, farc)) {
+        if (!stack_push<RING, kTraceThreads, !ANY>(S, spill, farc)) {
           atomicAdd(overflow, 1u);
           bpos = -1;
           done = true;
@@ -902,10 +945,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
         node = cl;
       } else if (hr) {
         node = cr;
-      } else if (S.sp == 0) {
+      } else if (stack_empty<!ANY>(S)) {
         done = true;
       } else {
-        node = stack_pop<RING, kTraceThreads>(S, spill);
+        node = stack_pop<RING, kTraceThreads, !ANY>(S, spill);
       }
     } else {
       // node >= 0: internal node to enter; node < 0: leaf whose box passed.
@@ -915,10 +958,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
         if (tri_test(r, B.ltri + kLeafStride * (size_t)(~node), t, al, be)) {
           bt = t; bal = al; bbe = be; bpos = ~node;
           done = true;
-        } else if (S.sp == 0) {
+        } else if (stack_empty<!ANY>(S)) {
           done = true;
         } else {
-          node = stack_pop<RING, kTraceThreads>(S, spill);
+          node = stack_pop<RING, kTraceThreads, !ANY>(S, spill);
         }
       } else {
         float4 q0, q1, q2, q3;
@@ -929,7 +972,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
         bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
         int cl = __float_as_int(q3.x), cr = __float_as_int(q3.y);
         if (hl) {
-          if (hr && !stack_push<RING, kTraceThreads>(S, spill, cr)) {
+          if (hr && !stack_push<RING, kTraceThreads, !ANY>(S, spill, cr)) {
             atomicAdd(overflow, 1u);
             bpos = -1;
             done = true;
@@ -937,10 +980,10 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
           node = cl;
         } else if (hr) {
           node = cr;
-        } else if (S.sp == 0) {
+        } else if (stack_empty<!ANY>(S)) {
           done = true;
         } else {
-          node = stack_pop<RING, kTraceThreads>(S, spill);
+          node = stack_pop<RING, kTraceThreads, !ANY>(S, spill);
         }
       }
     }

@@ -42,6 +42,36 @@ def random_rays(n, seed, origin=(0.0, 0.0, 0.0), spread=0.6):
     return o, d
 
 
+def deep_chain_mesh(hittable=(), bits=63):
+    """Flat triangles whose Morton keys are C + (2^i - 1), i = 0..54 (C: the
+    top bit of each axis), plus degenerate triangles at the scene corners
+    (keys 0 and all ones). Consecutive keys then first differ at ever higher
+    bits, so the radix tree is a chain about 57 levels deep. Every box
+    contains the line x = y = 0.499; a triangle covers x + y >= cx + cy >= 1
+    (so the line misses it) unless its index is in `hittable`, where it
+    covers x + y <= cx + cy. Returns (mesh, index of each chain triangle)."""
+    assert bits == 63
+    h = 0.249
+    tris = [np.zeros((3, 3)), np.ones((3, 3))]
+    for i in range(55):
+        # bit 3k of a key is z's bit k, 3k+1 y's, 3k+2 x's (kernels/bvh.cu:31)
+        j = [i // 3, i // 3 + (1 if i % 3 >= 2 else 0), i // 3 + (1 if i % 3 >= 1 else 0)]  # x, y, z
+        c = [((1 << 20) + (1 << jj) - 1 + 0.5) / float(1 << 21) for jj in j]
+        cx, cy, cz = c
+        if i in hittable:
+            v = [(cx - h, cy - h, cz), (cx + h, cy - h, cz), (cx - h, cy + h, cz)]
+        else:
+            v = [(cx + h, cy + h, cz), (cx - h, cy + h, cz), (cx + h, cy - h, cz)]
+        tris.append(np.array(v))
+    t = np.array(tris, np.float64)
+    n = len(t)
+    p = t.reshape(-1, 3).astype(np.float32)
+    nrm = np.tile(np.array([0.0, 0.0, 1.0], np.float32), (3 * n, 1))
+    idx = np.arange(3 * n, dtype=np.int32)
+    hm = rtamd.HostMesh(p[:, 0], p[:, 1], p[:, 2], nrm[:, 0], nrm[:, 1], nrm[:, 2], idx)
+    return hm, np.arange(2, n)
+
+
 def duplicate_key_mesh(n, seed=77):
     """Most triangles packed into one tiny cluster, a few spread over the unit
     cube: the cluster's centroids share Morton cells, so keys repeat and the

@@ -210,6 +210,31 @@ def test_trace_axis_aligned_and_tiny_directions(ctx):
     assert hits > 100
 
 
+@pytest.mark.parametrize("hittable", [(54,), (20, 54)])
+def test_trace_deep_chain_spills_stack(ctx, hittable):
+    """A chain-shaped tree 56 levels deep whose every box the rays cross, with
+    the hittable triangles at the far end of the traversal order: the stack
+    grows past the 32 entries a lane keeps in LDS, moves halves to the spill
+    area and brings them back, and the hits must still equal the oracle's."""
+    _say(f"trace deep chain hittable={hittable}")
+    hm, _ = scenes.deep_chain_mesh(hittable)
+    parent = O.build(hm, 63)["parent"]
+    depth = 0
+    for p in range(hm.n):
+        x, d = hm.n - 1 + p, 0
+        while x != 0:
+            x, d = parent[x], d + 1
+        depth = max(depth, d)
+    assert depth == hm.n - 1  # the keys make a chain
+    rng = np.random.default_rng(31)
+    xy = rng.uniform(0.45, 0.499, size=(64, 2))
+    for z, dz in ((-1.0, 1.0), (2.0, -1.0)):
+        orig = np.column_stack([xy, np.full(len(xy), z)])
+        dirs = np.tile([1e-4, 1e-4, dz], (len(xy), 1))
+        dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+        assert _trace_compare(ctx, hm, 63, orig, dirs) == len(xy)
+
+
 def _render_compare(ctx, hm, bits, mats, light, cam, opts, pixels=None):
     dm = rtamd.DeviceMesh(hm)
     b = rtamd.BVH(ctx, dm, bits)
