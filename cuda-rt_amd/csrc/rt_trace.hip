@@ -738,46 +738,36 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
           const int pos = ~pending;
           pending = kNoNode;
           const float4* L = B.ltri + kLeafStride * (size_t)pos;
-          bool exact = true;
-          if (ANY && QUANT) {
-            // quantized nodes hold no exact leaf boxes: decide it here
-            const float4 q0 = ldg4(L), q1 = ldg4(L + 1), q2 = ldg4(L + 2);
-            const float ax = q0.x, ay = q0.y, az = q0.z, bx = q0.w, by = q1.x, bz = q1.y, cx = q1.z, cy = q1.w,
-                        cz = q2.x;
-            double l0, l1;
-            exact = slab(r, fminf(cx, fminf(ax, bx)), fminf(cy, fminf(ay, by)), fminf(cz, fminf(az, bz)),
-                         fmaxf(cx, fmaxf(ax, bx)), fmaxf(cy, fmaxf(ay, by)), fmaxf(cz, fmaxf(az, bz)), l0, l1);
+          // The reference tests a triangle only if the line enters its box
+          // (the leaf's box test, kernels/bvh.cu:382-449). The quantized
+          // nodes hold no exact leaf boxes, so that test is made here, after
+          // the triangle test and only for a triangle that would be taken:
+          // a hit needs both, and the box test is then rarely run at all.
+          double t, al, be;
+          ++st.tests;
+          bool take = tri_test(r, L, t, al, be);
+          if (!ANY) {
+            if (take && tmin_phys > 0 && !(t >= tmin_phys)) take = false;
+            // closest hit: the minimum (t, leaf position), as consider_leaf
+            if (take && bpos >= 0 && !(t < bt || (t == bt && pos < bpos))) take = false;
           }
-          if (ANY) {
-            // otherwise the leaf's box was decided exactly when its node was entered
-            double t, al, be;
-            if (exact) {
-              ++st.tests;
-              if (tri_test(r, L, t, al, be)) {
-                bt = t; bal = al; bbe = be; bpos = pos;
-                done = true;
-              }
-            }
-          } else {
+          if (take && (QUANT || !ANY)) {
             // leaf record: q0 = a.xyz, b.x; q1 = b.yz, c.xy; q2 = c.z, id. The
             // box is formed exactly as the build forms the triangle box.
             const float4 q0 = ldg4(L), q1 = ldg4(L + 1), q2 = ldg4(L + 2);
             const float ax = q0.x, ay = q0.y, az = q0.z, bx = q0.w, by = q1.x, bz = q1.y, cx = q1.z, cy = q1.w,
                         cz = q2.x;
             double l0, l1;
-            bool ok = slab(r, fminf(cx, fminf(ax, bx)), fminf(cy, fminf(ay, by)), fminf(cz, fminf(az, bz)),
-                           fmaxf(cx, fmaxf(ax, bx)), fmaxf(cy, fmaxf(ay, by)), fmaxf(cz, fmaxf(az, bz)), l0, l1);
-            if (ok && tmin_phys > 0) ok = !(l1 < tmin_phys - prune_margin(tmin_phys));
-            if (ok && bpos >= 0) ok = !(l0 > bt + prune_margin(bt));
-            if (ok) {
-              HitRec best{bt, bal, bbe, bpos};
-              consider_leaf(B, r, pos, tmin_phys, best, st);
-              bt = best.t;
-              bal = best.al;
-              bbe = best.be;
-              bpos = best.pos;
-              if (bpos >= 0) flim = f_up(bt + prune_margin(bt));
-            }
+            take = slab(r, fminf(cx, fminf(ax, bx)), fminf(cy, fminf(ay, by)), fminf(cz, fminf(az, bz)),
+                        fmaxf(cx, fmaxf(ax, bx)), fmaxf(cy, fmaxf(ay, by)), fmaxf(cz, fmaxf(az, bz)), l0, l1);
+          }
+          if (take) {
+            bt = t;
+            bal = al;
+            bbe = be;
+            bpos = pos;
+            if (ANY) done = true;
+            else flim = f_up(bt + prune_margin(bt));
           }
         }
       }
