@@ -1670,15 +1670,23 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
 // per-cell counts, their exclusive scan, and the scatter of slot indices.
 // The order within a cell depends on atomic timing; the any-hit pass writes
 // every result to the ray's own slot, so the image does not depend on it.
+// A workgroup sorts kSortPer * kThreads slots (kSortPer per thread, strided),
+// so that few workgroups add their counts to the global ones.
+constexpr int kSortPer = 16;
+constexpr int kSortSpan = kSortPer * kThreads;
+
 __global__ __launch_bounds__(kThreads) void k_sort_hist(WfState W) {
   __shared__ uint32_t h[kShadowCells];
+  const uint32_t n = *W.scount, i0 = blockIdx.x * kSortSpan + threadIdx.x;
+  if (blockIdx.x * kSortSpan >= n) return;
   for (int k = threadIdx.x; k < kShadowCells; k += kThreads) h[k] = 0u;
   __syncthreads();
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t n = *W.scount;
-  if (i < n) atomicAdd(&h[W.skey[i]], 1u);
+#pragma unroll
+  for (int j = 0; j < kSortPer; ++j) {
+    const uint32_t i = i0 + j * kThreads;
+    if (i < n) atomicAdd(&h[W.skey[i]], 1u);
+  }
   __syncthreads();
-  if (blockIdx.x * kThreads >= n) return;
   for (int k = threadIdx.x; k < kShadowCells; k += kThreads)
     if (h[k]) atomicAdd(&W.shist[k], h[k]);
 }
@@ -1717,19 +1725,27 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_scan(WfState W) {
 
 __global__ __launch_bounds__(kThreads) void k_sort_scatter(WfState W) {
   __shared__ uint32_t h[kShadowCells];
+  const uint32_t n = *W.scount, i0 = blockIdx.x * kSortSpan + threadIdx.x;
+  if (blockIdx.x * kSortSpan >= n) return;
   for (int k = threadIdx.x; k < kShadowCells; k += kThreads) h[k] = 0u;
   __syncthreads();
-  const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-  const uint32_t n = *W.scount;
-  if (blockIdx.x * kThreads >= n) return;
-  const uint32_t key = i < n ? W.skey[i] : 0u;
-  const uint32_t rank = i < n ? atomicAdd(&h[key], 1u) : 0u;
+  uint32_t key[kSortPer], rank[kSortPer];
+#pragma unroll
+  for (int j = 0; j < kSortPer; ++j) {
+    const uint32_t i = i0 + j * kThreads;
+    key[j] = i < n ? W.skey[i] : 0u;
+    rank[j] = i < n ? atomicAdd(&h[key[j]], 1u) : 0u;
+  }
   __syncthreads();
-  // one global reservation per non-empty cell of the block
+  // one global reservation per non-empty cell of the workgroup
   for (int k = threadIdx.x; k < kShadowCells; k += kThreads)
     if (h[k]) h[k] = atomicAdd(&W.sbase[k], h[k]);
   __syncthreads();
-  if (i < n) W.sperm[h[key] + rank] = i;
+#pragma unroll
+  for (int j = 0; j < kSortPer; ++j) {
+    const uint32_t i = i0 + j * kThreads;
+    if (i < n) W.sperm[h[key[j]] + rank[j]] = i;
+  }
 }
 
 // Occluder code of a shadow query: -1 = unshadowed (factor 1), -2 = opaque
@@ -2188,9 +2204,10 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
       const bool ssort = !multi && kShadowSort;
       RayIn SR{W.shq, W.shq + 3 * cap, 1, cap, W.scount, 0, ssort ? W.sperm : nullptr};
       if (ssort) {
-        k_sort_hist<<<tiles, kThreads, 0, sb>>>(W);
+        const int sort_blocks = cdiv((long)cap, kSortSpan);
+        k_sort_hist<<<sort_blocks, kThreads, 0, sb>>>(W);
         k_sort_scan<<<1, kScanThreads, 0, sb>>>(W);
-        k_sort_scatter<<<tiles, kThreads, 0, sb>>>(W);
+        k_sort_scatter<<<sort_blocks, kThreads, 0, sb>>>(W);
       }
       HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr, 0};
       if (T) RT_TRY(timer_mark(T, 1, true, sb));
