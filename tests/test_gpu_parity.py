@@ -275,6 +275,24 @@ def test_render_parity_thinlens_soup(ctx):
     _render_compare(ctx, hm, 63, mats, light, cam, opts)
 
 
+@pytest.mark.parametrize("where", ["centre", "inside", "near"])
+def test_render_light_inside_scene_box(ctx, where):
+    """Shadow rays are ordered by their direction from the light. With the
+    light at the scene box's centre or inside it the direction cells are
+    undefined (every key 0); just outside a face they span a wide angle.
+    The order never changes a result, so the image must still equal the
+    oracle's."""
+    _say(f"render light {where}")
+    hm = scenes.soup_mesh(20_000)
+    mats, light, cam, opts = scenes.c2_setup(64, 48, 2)
+    lo = np.array([hm.vx.min(), hm.vy.min(), hm.vz.min()], np.float64)
+    hi = np.array([hm.vx.max(), hm.vy.max(), hm.vz.max()], np.float64)
+    c = 0.5 * (lo + hi)
+    pos = {"centre": c, "inside": lo + 0.3 * (hi - lo), "near": c + np.array([0.6 * (hi[0] - lo[0]), 0.0, 0.0])}
+    light.position[:] = [float(x) for x in pos[where]]
+    _render_compare(ctx, hm, 63, mats, light, cam, opts)
+
+
 def test_render_parity_bounces(ctx):
     _say("render parity bounces depth 3")
     hm = scenes.displaced_mesh(20)
