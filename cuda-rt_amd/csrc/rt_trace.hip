@@ -152,20 +152,20 @@ constexpr int kThreads = 256;
 constexpr int kRing = 32;
 constexpr int kSpill = 64;
 
-struct RingStack {
+struct LaneStack {
   int* col;  // this lane's column of the workgroup's LDS stack
   int d = 0;   // LIN: entries in LDS; ring: entries in all
   int lo = 0;  // entries in the spill area (LIN: a multiple of RING / 2)
 };
 
 template <bool LIN = true>
-__device__ __forceinline__ bool stack_empty(const RingStack& S) {
+__device__ __forceinline__ bool stack_empty(const LaneStack& S) {
   return LIN ? (S.d | S.lo) == 0 : S.d == 0;
 }
 
 // STRIDE = threads per workgroup (one LDS column per thread).
 template <int RING = kRing, int STRIDE = kThreads, bool LIN = true>
-__device__ __forceinline__ bool stack_push(RingStack& S, int* spill, int v) {
+__device__ __forceinline__ bool stack_push(LaneStack& S, int* spill, int v) {
   static_assert(kSpill % (RING / 2) == 0, "the spill area holds whole halves");
   if (!LIN) {
     if (S.d - S.lo == RING) {
@@ -193,7 +193,7 @@ __device__ __forceinline__ bool stack_push(RingStack& S, int* spill, int v) {
 
 // Requires !stack_empty(S).
 template <int RING = kRing, int STRIDE = kThreads, bool LIN = true>
-__device__ __forceinline__ int stack_pop(RingStack& S, int* spill) {
+__device__ __forceinline__ int stack_pop(LaneStack& S, int* spill) {
   if (!LIN) {
     --S.d;
     const int v = S.col[(S.d & (RING - 1)) * STRIDE];
@@ -230,7 +230,7 @@ __device__ bool closest_hit(const BvhView& B, const RayD& r, double tmin_phys, H
     consider_leaf(B, r, 0, tmin_phys, best, st);
     return true;
   }
-  RingStack S;
+  LaneStack S;
   S.col = lds_col;
   int spill[kSpill];
   int node = 0;
@@ -301,7 +301,7 @@ __device__ bool any_hit(const BvhView& B, const RayD& r, HitRec& hit, uint32_t* 
     }
     return false;
   }
-  RingStack S;
+  LaneStack S;
   S.col = lds_col;
   int spill[kSpill];
   int node = 0;
@@ -624,7 +624,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
   const uint32_t lane = __lane_id();
   const uint64_t mlt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
   TravStats st;
-  RingStack S;
+  LaneStack S;
   S.col = s_stack + threadIdx.x;
   int spill[kSpill];
   RayD r;
