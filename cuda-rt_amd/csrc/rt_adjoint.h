@@ -56,6 +56,7 @@ struct TriAdj {
 };
 RTA_HD TriAdj hit_adj(v3 A, v3 B, v3 C, v3 o, v3 d, double t, double al, double be, double tbar, double abar,
                       double bbar) {
+  (void)o;  // the tangent does not depend on the origin itself
   v3 c0 = -d, c1 = B - A, c2 = C - A;
   v3 k0 = cross(c1, c2), k1 = cross(c2, c0), k2 = cross(c0, c1);
   double det = dot(c0, k0);

@@ -532,7 +532,6 @@ __device__ __forceinline__ float f_down(double x) {
 // its longest ray. Closest-hit and any-hit use the same decisions as
 // closest_hit()/any_hit() above, one node (or, for any-hit, one popped leaf)
 // per step.
-constexpr uint32_t kDefaultFetchChunk = 256;
 #ifndef RT_REFILL_MIN
 #define RT_REFILL_MIN 16
 #endif
@@ -983,29 +982,6 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
     }
   }
   flush_stats(stats, st);
-}
-
-// ------------------------------------------------------------ trace kernel
-__global__ __launch_bounds__(kThreads) void k_trace(BvhView B, const double* __restrict__ orig,
-                                                    const double* __restrict__ dir, long nrays, int mode, double tmin,
-                                                    int32_t* __restrict__ tri, double* __restrict__ tout,
-                                                    double* __restrict__ aout, double* __restrict__ bout,
-                                                    uint32_t* overflow) {
-  __shared__ int s_stack[kRing * kThreads];
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= nrays) return;
-  RayD r = make_ray(mk(orig[3 * i], orig[3 * i + 1], orig[3 * i + 2]), mk(dir[3 * i], dir[3 * i + 1], dir[3 * i + 2]));
-  HitRec h;
-  TravStats st;
-  if (mode == RT_TRACE_ANY)
-    any_hit(B, r, h, overflow, st, s_stack + threadIdx.x);
-  else
-    closest_hit(B, r, tmin, h, overflow, st, s_stack + threadIdx.x);
-  bool ok = h.pos >= 0;
-  tri[i] = ok ? leaf_tri(B, h.pos) : -1;
-  if (tout) tout[i] = ok ? h.t : 0.0;
-  if (aout) aout[i] = ok ? h.al : 0.0;
-  if (bout) bout[i] = ok ? h.be : 0.0;
 }
 
 // ------------------------------------------------------------ shading
@@ -1470,7 +1446,6 @@ __device__ __forceinline__ size_t rec_index(const WfState& W, uint32_t p, int sa
   return ((size_t)p * W.rec_spp + sample) * (W.rec_depth + 1) + bounce;
 }
 
-__device__ __forceinline__ d3 ld3(const double* p) { return mk(p[0], p[1], p[2]); }
 // Component-major (SoA) vectors: x at a[i], y at a[n + i], z at a[2n + i].
 __device__ __forceinline__ d3 ldv(const double* a, size_t i, size_t n) { return mk(a[i], a[n + i], a[2 * n + i]); }
 __device__ __forceinline__ void stv(double* a, size_t i, size_t n, d3 v) {
@@ -1971,7 +1946,7 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
   BvhView* view = (BvhView*)(fetch + 512);
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, fetch, kFetchWords);
   k_put_view<<<1, 1, 0, s>>>(view, b->view());
-  RayIn R{rays->orig_dev, rays->dir_dev, 3, 1, nullptr, (uint32_t)rays->n};
+  RayIn R{rays->orig_dev, rays->dir_dev, 3, 1, nullptr, (uint32_t)rays->n, nullptr};
   HitOut O{nullptr, hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev, nullptr, 0};
   rt_tuning t = rt_default_tuning();
   int grid = persistent_grid(b->ctx ? b->ctx->num_cus : 256, t, (uint32_t)rays->n);
@@ -2184,7 +2159,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     RT_WF(s, W, k_wf_camera, it, (uint32_t)cap);
     int cur = 0;
     for (int bounce = 0; bounce <= max_depth; ++bounce) {
-      RayIn R{W.ray[cur], W.ray[cur] + 3 * cap, 1, cap, &W.counts[cur], 0};
+      RayIn R{W.ray[cur], W.ray[cur] + 3 * cap, 1, cap, &W.counts[cur], 0, nullptr};
       HitOut O{W.hpos, nullptr, nullptr, nullptr, nullptr, W.hit, cap};
       // one scene: every ray uses views[0] (no per-ray scene lookup)
       if (T) RT_TRY(timer_mark(T, 0, true, s));
