@@ -52,7 +52,7 @@ def parse():
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront",
                     help="render kernel structure (identical images)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-pixel-stride", type=int, default=5,
+    ap.add_argument("--cpu-pixel-stride", type=int, default=1,
                     help="CPU baseline renders every k-th pixel (launch order)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="only run device copies of this many MiB (HBM counter calibration) and exit")
@@ -107,7 +107,8 @@ def cpu_baseline(hm, mats, light, cam, opts, stride, tris):
         "cores": threads,
         "kind": "port",
         "sample": f"oracle FP64 build of all {tris} tris ({t_build:.2f} s, "
-                  f"{tris / t_build / 1e6:.3f} Mtris/s) + render of every {stride}th pixel "
+                  f"{tris / t_build / 1e6:.3f} Mtris/s) + render of "
+                  f"{'every pixel' if stride == 1 else f'every {stride}th pixel'} "
                   f"({pixels.size} px x {opts.spp} spp = {int(counts[0])} primary, {int(counts[1])} shadow rays, "
                   f"{t_render:.2f} s)",
         "build_mtris_s": round(tris / t_build / 1e6, 4),

@@ -406,8 +406,10 @@ struct RayF {
   float fx, fy, fz;  // o/d - E: far-plane offsets
 };
 
-__device__ __forceinline__ void rayf_axis(double o, double d, double bmax, float& iv, float& nn, float& ff) {
-  double inv = 1.0 / d;
+// a = 1/d of the axis (RayD's ax, ay or az)
+__device__ __forceinline__ void rayf_axis(double o, double d, double a, double bmax, float& iv, float& nn,
+                                          float& ff) {
+  double inv = a;
   if (!(fabs(inv) <= 1e30)) inv = copysign(1e30, d);
   const double oi = o * inv;
   const double E = 6.0 * 5.9604644775390625e-08 * (bmax + fabs(o)) * fabs(inv);
@@ -421,9 +423,9 @@ __device__ __forceinline__ RayF make_rayf(const RayD& r, const float* root_box) 
 #pragma unroll
   for (int k = 0; k < 6; ++k) bmax = fmax(bmax, fabs((double)ldgf(root_box + k)));
   RayF f;
-  rayf_axis(r.o.x, r.d.x, bmax, f.ix, f.nx, f.fx);
-  rayf_axis(r.o.y, r.d.y, bmax, f.iy, f.ny, f.fy);
-  rayf_axis(r.o.z, r.d.z, bmax, f.iz, f.nz, f.fz);
+  rayf_axis(r.o.x, r.d.x, r.ax, bmax, f.ix, f.nx, f.fx);
+  rayf_axis(r.o.y, r.d.y, r.ay, bmax, f.iy, f.ny, f.fy);
+  rayf_axis(r.o.z, r.d.z, r.az, bmax, f.iz, f.nz, f.fz);
   return f;
 }
 
