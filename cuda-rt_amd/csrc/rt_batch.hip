@@ -31,6 +31,7 @@ struct rt_batch {
   hipStream_t streams[kBuildStreams] = {};
   hipEvent_t fork = nullptr;
   hipEvent_t joins[kBuildStreams] = {};
+  rt_seg_build* seg = nullptr;  // one launch per build stage for all scenes (nullptr: per-scene builds)
 };
 
 extern "C" {
@@ -42,6 +43,7 @@ void rt_batch_destroy(rt_batch* b) {
     if (b->streams[k]) (void)hipStreamDestroy(b->streams[k]);
   }
   if (b->fork) (void)hipEventDestroy(b->fork);
+  rt_seg_build_destroy(b->seg);
   if (b->ws) (void)hipFree(b->ws);
   if (b->shade) (void)hipFree(b->shade);
   delete b;
@@ -113,6 +115,7 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->joins[k], hipEventDisableTiming);
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
+  if (e == hipSuccess) e = rt_seg_build_create(items, count, &b->seg);
   (void)rt_ctx_overlap(ctx);  // created here, outside any graph capture of rt_batch_render
   if (e != hipSuccess) {
     rt_batch_destroy(b);
@@ -125,6 +128,14 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
 rt_status rt_batch_rebuild(rt_batch* b, void* stream) {
   if (!b) return RT_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
+  // Every stage in one launch over all scenes, unless a scene also keeps
+  // the nodes of layout 4 or 8 (the segmented build fills layout 5's).
+  bool seg = b->seg != nullptr;
+  for (const rt_batch_item& it : b->items) seg = seg && !it.bvh->want_wide4 && !it.bvh->want_wide8;
+  if (seg) {
+    RT_HIP(b->ctx, rt_seg_build_launch(b->seg, s));
+    return RT_OK;
+  }
   RT_HIP(b->ctx, hipEventRecord(b->fork, s));
   int used = (int)(b->nscenes < (size_t)kBuildStreams ? b->nscenes : (size_t)kBuildStreams);
   for (int k = 0; k < used; ++k) RT_HIP(b->ctx, hipStreamWaitEvent(b->streams[k], b->fork, 0));
@@ -153,6 +164,7 @@ rt_status rt_batch_set_tuning(rt_batch* b, const rt_tuning* t) {
   if (!b) return RT_ERR_INVALID;
   if (!rt_tuning_valid(t)) return rt_set_error(b->ctx, RT_ERR_INVALID, "invalid tuning");
   b->tuning = *t;
+  for (const rt_batch_item& it : b->items) rt_keep_only_layout(it.bvh, t->node_layout);
   return RT_OK;
 }
 
@@ -198,6 +210,8 @@ done:
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->tuning = st == RT_OK ? win : keep;
+  // the trials filled every layout: rebuilds keep only the one in use
+  for (const rt_batch_item& it : b->items) rt_keep_only_layout(it.bvh, b->tuning.node_layout);
   if (st == RT_OK) {
     if (best) *best = win;
     if (best_ms) *best_ms = win_ms;

@@ -8,6 +8,9 @@
 // :71-207 (Karras hierarchy), :209-258 (refit).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <new>
+
 #include "rt_internal.h"
 #include "rt_math.h"
 
@@ -28,9 +31,9 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 // contiguous range) and the per-build counters; if init_bounds, set the
 // scene-bound accumulators to the reference's initial values (+1e9 for
 // the minimum, -1e9 for the maximum, kernels/scene.cu:78,83).
-__global__ void k_init(uint32_t* __restrict__ misc, uint32_t* __restrict__ zero, size_t words, int init_bounds) {
-  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
-  size_t stride = (size_t)gridDim.x * blockDim.x;
+__device__ __forceinline__ void init_body(uint32_t bid, uint32_t nblk, uint32_t* __restrict__ misc, uint32_t* __restrict__ zero, size_t words, int init_bounds) {
+  size_t i = bid * (size_t)blockDim.x + threadIdx.x;
+  size_t stride = (size_t)nblk * blockDim.x;
   for (size_t k = i; k < words; k += stride) zero[k] = 0u;
   if (i < kMiscWords) {
     if (i < kMiscBounds + 6 * kBoundsReplicas) {
@@ -39,6 +42,9 @@ __global__ void k_init(uint32_t* __restrict__ misc, uint32_t* __restrict__ zero,
       misc[i] = 0u;
     }
   }
+}
+__global__ void k_init(uint32_t* __restrict__ misc, uint32_t* __restrict__ zero, size_t words, int init_bounds) {
+  init_body(blockIdx.x, gridDim.x, misc, zero, words, init_bounds);
 }
 
 // ------------------------------------------------------------ AABB + bounds
@@ -49,12 +55,12 @@ __global__ void k_init(uint32_t* __restrict__ misc, uint32_t* __restrict__ zero,
 // coordinates are ignored, as fminf/fmaxf ignore them. trirec receives each
 // triangle's leaf record in triangle order (the climb then copies one
 // record per sorted leaf instead of gathering index and vertices).
-__global__ __launch_bounds__(256) void k_aabb_bounds(const float* __restrict__ vx, const float* __restrict__ vy,
+__device__ __forceinline__ void aabb_bounds_body(uint32_t bid, uint32_t nblk, const float* __restrict__ vx, const float* __restrict__ vy,
                                                      const float* __restrict__ vz, const int32_t* __restrict__ idx,
                                                      int n, float* __restrict__ tri_box, float4* __restrict__ trirec,
                                                      uint32_t* __restrict__ misc, int with_bounds) {
   float acc[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+  for (int i = bid * 256 + threadIdx.x; i < n; i += nblk * 256) {
     int a = idx[3 * i + 0], b = idx[3 * i + 1], c = idx[3 * i + 2];
     float ax = vx[a], ay = vy[a], az = vz[a];
     float bx = vx[b], by = vy[b], bz = vz[b];
@@ -91,13 +97,19 @@ __global__ __launch_bounds__(256) void k_aabb_bounds(const float* __restrict__ v
     int k = threadIdx.x;
     float v = s_acc[0][k];
     for (int j = 1; j < 4; ++j) v = (k < 3) ? fminf(v, s_acc[j][k]) : fmaxf(v, s_acc[j][k]);
-    uint32_t* dst = misc + kMiscBounds + 6 * (blockIdx.x % kBoundsReplicas) + k;
+    uint32_t* dst = misc + kMiscBounds + 6 * (bid % kBoundsReplicas) + k;
     if (k < 3) {
       if (v < INFINITY) atomicMin(dst, f2ord(v));
     } else {
       if (v > -INFINITY) atomicMax(dst, f2ord(v));
     }
   }
+}
+__global__ __launch_bounds__(256) void k_aabb_bounds(const float* __restrict__ vx, const float* __restrict__ vy,
+                                                     const float* __restrict__ vz, const int32_t* __restrict__ idx,
+                                                     int n, float* __restrict__ tri_box, float4* __restrict__ trirec,
+                                                     uint32_t* __restrict__ misc, int with_bounds) {
+  aabb_bounds_body(blockIdx.x, gridDim.x, vx, vy, vz, idx, n, tri_box, trirec, misc, with_bounds);
 }
 
 // Scene bounds from the accumulator replicas.
@@ -163,7 +175,7 @@ struct KeyTraits<uint64_t> {
 // Each workgroup also builds the digit histograms of every sort pass in
 // LDS and adds them to the global histogram once.
 template <typename K>
-__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ tri_box, int n,
+__device__ __forceinline__ void morton_body(uint32_t bid, uint32_t nblk, const float* __restrict__ tri_box, int n,
                                                 const uint32_t* __restrict__ misc, K* __restrict__ keys,
                                                 uint32_t* __restrict__ hist, int per_block) {
   constexpr int P = KeyTraits<K>::kPasses;
@@ -174,7 +186,7 @@ __global__ __launch_bounds__(256) void k_morton(const float* __restrict__ tri_bo
 #pragma unroll
   for (int k = 0; k < 3; ++k) ext[k] = mx[k] - mn[k];
   __syncthreads();
-  int begin = blockIdx.x * per_block;
+  int begin = bid * per_block;
   int end = min(n, begin + per_block);
   for (int i = begin + threadIdx.x; i < end; i += 256) {
     const float* bx = tri_box + 6 * (size_t)i;
@@ -196,6 +208,12 @@ __global__ __launch_bounds__(256) void k_morton(const float* __restrict__ tri_bo
     uint32_t v = s_hist[k];
     if (v) atomicAdd(&hist[k], v);
   }
+}
+template <typename K>
+__global__ __launch_bounds__(256) void k_morton(const float* __restrict__ tri_box, int n,
+                                                const uint32_t* __restrict__ misc, K* __restrict__ keys,
+                                                uint32_t* __restrict__ hist, int per_block) {
+  morton_body(blockIdx.x, gridDim.x, tri_box, n, misc, keys, hist, per_block);
 }
 
 // ------------------------------------------------------------ onesweep sort
@@ -228,7 +246,7 @@ __device__ __forceinline__ uint32_t block_excl_scan256(uint32_t v, uint32_t* s_w
 }
 
 template <typename K>
-__global__ __launch_bounds__(kSortThreads) void k_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ iin,
+__device__ __forceinline__ void onesweep_body(uint32_t bid, uint32_t nblk, const K* __restrict__ kin, const uint32_t* __restrict__ iin,
                                                            K* __restrict__ kout, uint32_t* __restrict__ iout, int n,
                                                            int shift, const uint32_t* __restrict__ hist,
                                                            uint32_t* status, uint32_t* tile_ctr) {
@@ -331,6 +349,13 @@ __global__ __launch_bounds__(kSortThreads) void k_onesweep(const K* __restrict__
     kout[g] = kk;
     iout[g] = s_ids[j];
   }
+}
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_onesweep(const K* __restrict__ kin, const uint32_t* __restrict__ iin,
+                                                           K* __restrict__ kout, uint32_t* __restrict__ iout, int n,
+                                                           int shift, const uint32_t* __restrict__ hist,
+                                                           uint32_t* status, uint32_t* tile_ctr) {
+  onesweep_body(blockIdx.x, gridDim.x, kin, iin, kout, iout, n, shift, hist, status, tile_ctr);
 }
 
 // ------------------------------------------------------------ hierarchy
@@ -588,14 +613,14 @@ __device__ __forceinline__ int climb_step(ClimbNode& c, const K* __restrict__ ke
 }
 
 template <typename K>
-__global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
+__device__ __forceinline__ void climb_chunks_body(uint32_t bid, uint32_t nblk, 
     const K* __restrict__ keys, const uint32_t* __restrict__ ids, int n, const float4* __restrict__ trirec,
     int32_t* __restrict__ parent, int32_t* __restrict__ left,
     int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode, float4* __restrict__ ltri,
     float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, uint32_t* __restrict__ deferred,
     uint32_t* __restrict__ misc, int32_t* __restrict__ leaf_of) {
   __shared__ uint32_t s_arr[kClimbChunk];
-  const int s0 = blockIdx.x * kClimbChunk;
+  const int s0 = bid * kClimbChunk;
   const int e0 = min(n, s0 + kClimbChunk);
   for (int k = threadIdx.x; k < kClimbChunk; k += kClimbThreads) s_arr[k] = 0u;
   // Leaves: node n-1+i = the triangle's box, plus its vertex record (the
@@ -658,6 +683,15 @@ __global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
     }
   }
 }
+template <typename K>
+__global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
+    const K* __restrict__ keys, const uint32_t* __restrict__ ids, int n, const float4* __restrict__ trirec,
+    int32_t* __restrict__ parent, int32_t* __restrict__ left,
+    int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode, float4* __restrict__ ltri,
+    float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, uint32_t* __restrict__ deferred,
+    uint32_t* __restrict__ misc, int32_t* __restrict__ leaf_of) {
+  climb_chunks_body(blockIdx.x, gridDim.x, keys, ids, n, trirec, parent, left, right, node_box, tnode, ltri, deposits, arrivals, deferred, misc, leaf_of);
+}
 
 // The climber of hand-off `code` (split * 2 + side) from its deposit.
 __device__ __forceinline__ ClimbNode climber_of(uint32_t code, const float4* __restrict__ deposits) {
@@ -682,14 +716,14 @@ __device__ __forceinline__ ClimbNode climber_of(uint32_t code, const float4* __r
 // balanced top halves the hand-offs per launch; what a bounded number of
 // launches leaves (a deep, unbalanced top) k_climb_top finishes.
 template <typename K>
-__global__ __launch_bounds__(256) void k_climb_pass(
+__device__ __forceinline__ void climb_pass_body(uint32_t bid, uint32_t nblk, 
     const K* __restrict__ keys, int n, int32_t* __restrict__ parent, int32_t* __restrict__ left,
     int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode,
     float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, const uint32_t* __restrict__ in,
     uint32_t* __restrict__ out, uint32_t* __restrict__ misc, int in_slot, int out_slot, int clear_slot) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) misc[clear_slot] = 0u;
+  if (bid == 0 && threadIdx.x == 0) misc[clear_slot] = 0u;
   const uint32_t count = min(misc[in_slot], (uint32_t)n);
-  for (uint32_t e = blockIdx.x * 256 + threadIdx.x; e < count; e += gridDim.x * 256) {
+  for (uint32_t e = bid * 256 + threadIdx.x; e < count; e += nblk * 256) {
     const uint32_t code = in[e];
     const int split = (int)(code >> 1), side = (int)(code & 1u);
     if (split >= n - 1) {
@@ -716,9 +750,17 @@ __global__ __launch_bounds__(256) void k_climb_pass(
       atomicAdd(&misc[kMiscBuildError], 1u);  // cannot happen: hand-offs have disjoint leaf ranges
   }
 }
+template <typename K>
+__global__ __launch_bounds__(256) void k_climb_pass(
+    const K* __restrict__ keys, int n, int32_t* __restrict__ parent, int32_t* __restrict__ left,
+    int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode,
+    float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, const uint32_t* __restrict__ in,
+    uint32_t* __restrict__ out, uint32_t* __restrict__ misc, int in_slot, int out_slot, int clear_slot) {
+  climb_pass_body(blockIdx.x, gridDim.x, keys, n, parent, left, right, node_box, tnode, deposits, arrivals, in, out, misc, in_slot, out_slot, clear_slot);
+}
 
 template <typename K>
-__global__ __launch_bounds__(kClimbTopThreads) void k_climb_top(
+__device__ __forceinline__ void climb_top_body(uint32_t bid, uint32_t nblk, 
     const K* __restrict__ keys, int n, int32_t* __restrict__ parent, int32_t* __restrict__ left,
     int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode,
     float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, const uint32_t* __restrict__ deferred,
@@ -745,6 +787,14 @@ __global__ __launch_bounds__(kClimbTopThreads) void k_climb_top(
       split = climb_step<K>(c, keys, n, &side, parent, left, right, node_box, tnode, deposits);
     }
   }
+}
+template <typename K>
+__global__ __launch_bounds__(kClimbTopThreads) void k_climb_top(
+    const K* __restrict__ keys, int n, int32_t* __restrict__ parent, int32_t* __restrict__ left,
+    int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode,
+    float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, const uint32_t* __restrict__ deferred,
+    uint32_t* __restrict__ misc, int in_slot) {
+  climb_top_body(blockIdx.x, gridDim.x, keys, n, parent, left, right, node_box, tnode, deposits, arrivals, deferred, misc, in_slot);
 }
 
 // ------------------------------------------------------------ export
@@ -787,9 +837,9 @@ __global__ __launch_bounds__(256) void k_export(int n, const K* __restrict__ key
 // ---- 4-wide traversal nodes
 // Depth parity of each internal node (walk to the root through the parent
 // links): the even-depth internal nodes become 4-wide nodes.
-__global__ __launch_bounds__(256) void k_wide_parity(int n, const int32_t* __restrict__ parent,
+__device__ __forceinline__ void wide_parity_body(uint32_t bid, uint32_t nblk, int n, const int32_t* __restrict__ parent,
                                                      uint8_t* __restrict__ flag) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = bid * blockDim.x + threadIdx.x;
   if (i >= n - 1) return;
   int x = i, d = 0;
   for (int guard = 0; x != 0 && guard < n; ++guard) {
@@ -797,6 +847,10 @@ __global__ __launch_bounds__(256) void k_wide_parity(int n, const int32_t* __res
     ++d;
   }
   flag[i] = ((d & 1) == 0 ? 1 : 0) | (d % 3 == 0 ? 2 : 0);
+}
+__global__ __launch_bounds__(256) void k_wide_parity(int n, const int32_t* __restrict__ parent,
+                                                     uint8_t* __restrict__ flag) {
+  wide_parity_body(blockIdx.x, gridDim.x, n, parent, flag);
 }
 
 // ---- packing the 4-wide nodes: node i (even depth, flag bit 0) goes to
@@ -827,16 +881,19 @@ __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t* s
   return s_w[w] + x - v;
 }
 
-__global__ __launch_bounds__(1024) void k_pack_count(int m, const uint8_t* __restrict__ flag, uint32_t* sums) {
+__device__ __forceinline__ void pack_count_body(uint32_t bid, uint32_t nblk, int m, const uint8_t* __restrict__ flag, uint32_t* sums) {
   __shared__ uint32_t s_w[17];
-  const int i = blockIdx.x * 1024 + threadIdx.x;
+  const int i = bid * 1024 + threadIdx.x;
   const uint32_t v = i < m ? (flag[i] & 1u) : 0u;
   uint32_t total;
   block_excl_scan_1024(v, s_w, &total);
-  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+  if (threadIdx.x == 0) sums[bid] = total;
+}
+__global__ __launch_bounds__(1024) void k_pack_count(int m, const uint8_t* __restrict__ flag, uint32_t* sums) {
+  pack_count_body(blockIdx.x, gridDim.x, m, flag, sums);
 }
 
-__global__ __launch_bounds__(1024) void k_pack_offsets(int blocks, uint32_t* sums) {
+__device__ __forceinline__ void pack_offsets_body(uint32_t bid, uint32_t nblk, int blocks, uint32_t* sums) {
   __shared__ uint32_t s_w[17];
   __shared__ uint32_t s_carry;
   if (threadIdx.x == 0) s_carry = 0;
@@ -853,25 +910,32 @@ __global__ __launch_bounds__(1024) void k_pack_offsets(int blocks, uint32_t* sum
     __syncthreads();
   }
 }
+__global__ __launch_bounds__(1024) void k_pack_offsets(int blocks, uint32_t* sums) {
+  pack_offsets_body(blockIdx.x, gridDim.x, blocks, sums);
+}
 
-__global__ __launch_bounds__(1024) void k_pack_index(int m, const uint8_t* __restrict__ flag,
+__device__ __forceinline__ void pack_index_body(uint32_t bid, uint32_t nblk, int m, const uint8_t* __restrict__ flag,
                                                      const uint32_t* __restrict__ offsets, int32_t* __restrict__ wcomp) {
   __shared__ uint32_t s_w[17];
-  const int i = blockIdx.x * 1024 + threadIdx.x;
+  const int i = bid * 1024 + threadIdx.x;
   const uint32_t v = i < m ? (flag[i] & 1u) : 0u;
   const uint32_t r = block_excl_scan_1024(v, s_w, nullptr);
-  if (i < m) wcomp[i] = v ? (int32_t)(offsets[blockIdx.x] + r) : -1;
+  if (i < m) wcomp[i] = v ? (int32_t)(offsets[bid] + r) : -1;
+}
+__global__ __launch_bounds__(1024) void k_pack_index(int m, const uint8_t* __restrict__ flag,
+                                                     const uint32_t* __restrict__ offsets, int32_t* __restrict__ wcomp) {
+  pack_index_body(blockIdx.x, gridDim.x, m, flag, offsets, wcomp);
 }
 
 // Fill the 4-wide node of every even-depth internal node from the binary
 // topology and the refit boxes (node numbering of the reference: internal
 // [0, n-2], leaf of sorted position p at n-1+p).
-__global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restrict__ left,
+__device__ __forceinline__ void wide_fill_body(uint32_t bid, uint32_t nblk, int n, const int32_t* __restrict__ left,
                                                    const int32_t* __restrict__ right,
                                                    const float* __restrict__ node_box,
                                                    const uint8_t* __restrict__ flag, float4* __restrict__ wide,
                                                    float4* __restrict__ wideq, const int32_t* __restrict__ wcomp) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = bid * blockDim.x + threadIdx.x;
   if (i >= n - 1 || !(flag[i] & 1)) return;
   int slot[4];
   int k = 0;
@@ -959,6 +1023,13 @@ __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restr
   for (int j = 0; j < 4; ++j) cref[j] = ref[j] >= 0 && ref[j] != kWideEmpty ? wcomp[ref[j]] : ref[j];
   q[3] = make_float4(__int_as_float(cref[0]), __int_as_float(cref[1]), __int_as_float(cref[2]),
                      __int_as_float(cref[3]));
+}
+__global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restrict__ left,
+                                                   const int32_t* __restrict__ right,
+                                                   const float* __restrict__ node_box,
+                                                   const uint8_t* __restrict__ flag, float4* __restrict__ wide,
+                                                   float4* __restrict__ wideq, const int32_t* __restrict__ wcomp) {
+  wide_fill_body(blockIdx.x, gridDim.x, n, left, right, node_box, flag, wide, wideq, wcomp);
 }
 
 // Quantize up to W child boxes against the node box nb: per axis the
@@ -1057,6 +1128,141 @@ __global__ __launch_bounds__(256) void k_wide8_fill(int n, const int32_t* __rest
   w[7] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+
+// ---- segmented build: the BVHs of a batch share every launch
+// A launch's workgroups are split into consecutive ranges, one per BVH
+// (off[s] .. off[s+1]-1; empty for a BVH with no work in that stage), and
+// each workgroup runs the stage's body for its BVH with its block index
+// inside the range. Every stage runs once over all BVHs, so a batch build
+// is about 30 launches that each fill the GPU, instead of 30 small ones per
+// BVH. The onesweep tiles of a BVH are still taken in launch order from its
+// own tile counter, so a tile waits only on tiles of workgroups that have
+// already started.
+struct SegBvh {
+  int n, ntiles;
+  size_t zero_words;
+  const float *vx, *vy, *vz;
+  const int32_t* idx;
+  float* tri_box;
+  float4* trirec;
+  uint32_t *misc, *hist, *status, *arrivals, *deferred, *deferred2, *wscan;
+  void* keys[2];
+  uint32_t* ids[2];
+  int32_t *parent, *left, *right, *leaf_of, *wcomp;
+  float* node_box;
+  float4 *tnode, *ltri, *deposits, *wideq;
+  uint8_t* wide_flag;
+};
+
+struct SegBlock {
+  const SegBvh* g;
+  uint32_t bid, nblk;
+};
+
+// The BVH of this workgroup: the last s with off[s] <= blockIdx.x.
+__device__ __forceinline__ SegBlock seg_block(const SegBvh* __restrict__ segs, const uint32_t* __restrict__ off,
+                                              int ns) {
+  int lo = 0, hi = ns - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return SegBlock{segs + lo, blockIdx.x - off[lo], off[lo + 1] - off[lo]};
+}
+
+constexpr int kMortonPerBlock = 4096;
+// Hand-off count slot k % 3 of the top-level passes (pass p reads slot p,
+// writes slot p + 1 and clears slot p + 2).
+__host__ __device__ constexpr int hand_off_slot(int k) {
+  return k % 3 == 0 ? kMiscDeferred : (k % 3 == 1 ? kMiscDeferredB : kMiscDeferredC);
+}
+
+__global__ __launch_bounds__(256) void k_seg_init(const SegBvh* segs, const uint32_t* off, int ns) {
+  const SegBlock b = seg_block(segs, off, ns);
+  init_body(b.bid, b.nblk, b.g->misc, b.g->hist, b.g->zero_words, 1);
+}
+
+__global__ __launch_bounds__(256) void k_seg_aabb_bounds(const SegBvh* segs, const uint32_t* off, int ns) {
+  const SegBlock b = seg_block(segs, off, ns);
+  const SegBvh& g = *b.g;
+  aabb_bounds_body(b.bid, b.nblk, g.vx, g.vy, g.vz, g.idx, g.n, g.tri_box, g.trirec, g.misc, 1);
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_seg_morton(const SegBvh* segs, const uint32_t* off, int ns) {
+  const SegBlock b = seg_block(segs, off, ns);
+  const SegBvh& g = *b.g;
+  morton_body<K>(b.bid, b.nblk, g.tri_box, g.n, g.misc, (K*)g.keys[0], g.hist, kMortonPerBlock);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kSortThreads) void k_seg_onesweep(const SegBvh* segs, const uint32_t* off, int ns,
+                                                               int p) {
+  const SegBlock b = seg_block(segs, off, ns);
+  const SegBvh& g = *b.g;
+  const int src = p & 1, dst = src ^ 1;
+  onesweep_body<K>(b.bid, b.nblk, (const K*)g.keys[src], p == 0 ? nullptr : g.ids[src], (K*)g.keys[dst],
+                   g.ids[dst], g.n, 8 * p, g.hist + 256 * p, g.status + (size_t)p * g.ntiles * 256,
+                   g.misc + kMiscTileCtr + p);
+}
+
+template <typename K>
+__global__ __launch_bounds__(kClimbThreads) void k_seg_climb_chunks(const SegBvh* segs, const uint32_t* off,
+                                                                    int ns) {
+  const SegBlock b = seg_block(segs, off, ns);
+  const SegBvh& g = *b.g;
+  climb_chunks_body<K>(b.bid, b.nblk, (const K*)g.keys[0], g.ids[0], g.n, g.trirec, g.parent, g.left, g.right,
+                       g.node_box, g.tnode, g.ltri, g.deposits, g.arrivals, g.deferred, g.misc, g.leaf_of);
+}
+
+template <typename K>
+__global__ __launch_bounds__(256) void k_seg_climb_pass(const SegBvh* segs, const uint32_t* off, int ns, int p) {
+  const SegBlock b = seg_block(segs, off, ns);
+  const SegBvh& g = *b.g;
+  uint32_t* lists[2] = {g.deferred, g.deferred2};
+  climb_pass_body<K>(b.bid, b.nblk, (const K*)g.keys[0], g.n, g.parent, g.left, g.right, g.node_box, g.tnode,
+                     g.deposits, g.arrivals, lists[p & 1], lists[(p + 1) & 1], g.misc, hand_off_slot(p),
+                     hand_off_slot(p + 1), hand_off_slot(p + 2));
+}
+
+template <typename K>
+__global__ __launch_bounds__(kClimbTopThreads) void k_seg_climb_top(const SegBvh* segs, const uint32_t* off, int ns,
+                                                                    int passes) {
+  const SegBlock b = seg_block(segs, off, ns);
+  const SegBvh& g = *b.g;
+  climb_top_body<K>(b.bid, b.nblk, (const K*)g.keys[0], g.n, g.parent, g.left, g.right, g.node_box, g.tnode,
+                    g.deposits, g.arrivals, (passes & 1) ? g.deferred2 : g.deferred, g.misc, hand_off_slot(passes));
+}
+
+__global__ __launch_bounds__(256) void k_seg_wide_parity(const SegBvh* segs, const uint32_t* off, int ns) {
+  const SegBlock b = seg_block(segs, off, ns);
+  wide_parity_body(b.bid, b.nblk, b.g->n, b.g->parent, b.g->wide_flag);
+}
+
+__global__ __launch_bounds__(1024) void k_seg_pack_count(const SegBvh* segs, const uint32_t* off, int ns) {
+  const SegBlock b = seg_block(segs, off, ns);
+  pack_count_body(b.bid, b.nblk, b.g->n - 1, b.g->wide_flag, b.g->wscan);
+}
+
+__global__ __launch_bounds__(1024) void k_seg_pack_offsets(const SegBvh* segs, const uint32_t* off, int ns) {
+  const SegBlock b = seg_block(segs, off, ns);
+  pack_offsets_body(b.bid, b.nblk, (b.g->n - 1 + 1023) / 1024, b.g->wscan);
+}
+
+__global__ __launch_bounds__(1024) void k_seg_pack_index(const SegBvh* segs, const uint32_t* off, int ns) {
+  const SegBlock b = seg_block(segs, off, ns);
+  pack_index_body(b.bid, b.nblk, b.g->n - 1, b.g->wide_flag, b.g->wscan, b.g->wcomp);
+}
+
+__global__ __launch_bounds__(256) void k_seg_wide_fill(const SegBvh* segs, const uint32_t* off, int ns) {
+  const SegBlock b = seg_block(segs, off, ns);
+  const SegBvh& g = *b.g;
+  wide_fill_body(b.bid, b.nblk, g.n, g.left, g.right, g.node_box, g.wide_flag, nullptr, g.wideq, g.wcomp);
+}
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
@@ -1075,28 +1281,36 @@ void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
     k_wide8_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
 }
 
-// The top of the tree above the chunks: about log2(chunks) + 2 one-level
-// launches over the whole GPU, then one workgroup for whatever remains.
-template <typename K>
-void launch_climb_top(rt_bvh* b, const K* keys, hipStream_t s) {
-  const int n = b->n;
-  if (n <= 1) return;
+// Number of top-level passes for n leaves (0 when one chunk holds them all)
+// and their grid. A pass over an empty hand-off list changes nothing, so a
+// segmented build runs the largest count for every BVH.
+int top_passes(int n) {
   const int chunks = cdiv(n, kClimbChunk);
   int passes = 0;
   if (chunks > 1) {
     passes = 2;
     while ((1 << (passes - 2)) < chunks && passes < 26) ++passes;
   }
-  const int slots[3] = {kMiscDeferred, kMiscDeferredB, kMiscDeferredC};
+  return passes;
+}
+int top_pass_grid(int n) { return std::max(1, std::min(cdiv(3L * cdiv(n, kClimbChunk), 256), 1024)); }
+
+// The top of the tree above the chunks: about log2(chunks) + 2 one-level
+// launches over the whole GPU, then one workgroup for whatever remains.
+template <typename K>
+void launch_climb_top(rt_bvh* b, const K* keys, hipStream_t s) {
+  const int n = b->n;
+  if (n <= 1) return;
+  const int passes = top_passes(n);
   uint32_t* lists[2] = {b->deferred, b->deferred2};
-  const int grid = std::max(1, std::min(cdiv(3 * chunks, 256), 1024));
+  const int grid = top_pass_grid(n);
   for (int p = 0; p < passes; ++p)
     k_climb_pass<K><<<grid, 256, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode, b->deposits,
-                                         b->arrivals, lists[p & 1], lists[(p + 1) & 1], b->misc, slots[p % 3],
-                                         slots[(p + 1) % 3], slots[(p + 2) % 3]);
+                                         b->arrivals, lists[p & 1], lists[(p + 1) & 1], b->misc, hand_off_slot(p),
+                                         hand_off_slot(p + 1), hand_off_slot(p + 2));
   k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
                                                 b->deposits, b->arrivals, lists[passes & 1], b->misc,
-                                                slots[passes % 3]);
+                                                hand_off_slot(passes));
 }
 
 template <typename K>
@@ -1108,7 +1322,7 @@ hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
   k_init<<<min(cdiv((long)zero_words, 256), 2048), 256, 0, s>>>(b->misc, b->hist, zero_words, 1);
   k_aabb_bounds<<<min(cdiv(n, 256), 1024), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box,
                                                         b->trirec, b->misc, 1);
-  int per_block = 4096;
+  const int per_block = kMortonPerBlock;
   k_morton<K><<<cdiv(n, per_block), 256, 0, s>>>(b->tri_box, n, b->misc, (K*)b->keys[0], b->hist, per_block);
   for (int p = 0; p < P; ++p) {
     int src = p & 1, dst = src ^ 1;
@@ -1174,6 +1388,158 @@ hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s) {
     if (n >= 2)
       k_wide8_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
   }
+  return hipGetLastError();
+}
+
+void rt_keep_only_layout(const rt_bvh* b, int layout) {
+  b->want_wide4 = b->want_wide4 && layout == 4;
+  b->want_wide8 = b->want_wide8 && layout == 8;
+}
+
+// ---- segmented batch build (host side)
+namespace {
+enum SegStage {
+  kStInit, kStAabb, kStMorton, kStSort, kStChunks, kStPass, kStTop, kStParity, kStPackCount, kStPackOffsets,
+  kStPackIndex, kStWideFill, kSegStages
+};
+}  // namespace
+
+struct rt_seg_build {
+  int ns = 0, bits = 0, passes = 0;
+  void* dev = nullptr;  // SegBvh[ns], then kSegStages tables of ns + 1 block offsets
+  const SegBvh* segs = nullptr;
+  const uint32_t* off[kSegStages] = {};
+  uint32_t total[kSegStages] = {};
+};
+
+void rt_seg_build_destroy(rt_seg_build* p) {
+  if (!p) return;
+  if (p->dev) (void)hipFree(p->dev);
+  delete p;
+}
+
+hipError_t rt_seg_build_create(const rt_batch_item* items, int count, rt_seg_build** out) {
+  *out = nullptr;
+  if (count < 1) return hipSuccess;
+  const int bits = items[0].bvh->bits;
+  for (int i = 0; i < count; ++i) {
+    const rt_bvh* b = items[i].bvh;
+    if (b->bits != bits || b->n < 1 || (b->build_flags & RT_BUILD_KARRAS_SEARCH)) return hipSuccess;
+  }
+  rt_seg_build* p = new (std::nothrow) rt_seg_build();
+  if (!p) return hipErrorOutOfMemory;
+  p->ns = count;
+  p->bits = bits;
+  std::vector<SegBvh> segs(count);
+  std::vector<uint32_t> off((size_t)kSegStages * (count + 1), 0u);
+  for (int i = 0; i < count; ++i) {
+    const rt_bvh* b = items[i].bvh;
+    const rt_mesh_view& m = items[i].mesh;
+    const int n = b->n;
+    SegBvh& g = segs[i];
+    g.n = n;
+    g.ntiles = b->ntiles;
+    g.zero_words = (size_t)b->passes * 256 + (size_t)b->passes * b->ntiles * 256 + (size_t)(n > 1 ? n - 1 : 1);
+    g.vx = m.vx_dev;
+    g.vy = m.vy_dev;
+    g.vz = m.vz_dev;
+    g.idx = m.idx_dev;
+    g.tri_box = b->tri_box;
+    g.trirec = b->trirec;
+    g.misc = b->misc;
+    g.hist = b->hist;
+    g.status = b->status;
+    g.arrivals = b->arrivals;
+    g.deferred = b->deferred;
+    g.deferred2 = b->deferred2;
+    g.wscan = b->wscan;
+    g.keys[0] = b->keys[0];
+    g.keys[1] = b->keys[1];
+    g.ids[0] = b->ids[0];
+    g.ids[1] = b->ids[1];
+    g.parent = b->parent;
+    g.left = b->left;
+    g.right = b->right;
+    g.leaf_of = b->leaf_of;
+    g.wcomp = b->wcomp;
+    g.node_box = b->node_box;
+    g.tnode = b->tnode;
+    g.ltri = b->ltri;
+    g.deposits = b->deposits;
+    g.wideq = b->wideq;
+    g.wide_flag = b->wide_flag;
+    // workgroups of each stage, as the single-BVH launches use them
+    const int wide = n >= 2;
+    const uint32_t blocks[kSegStages] = {
+        (uint32_t)std::min(cdiv((long)g.zero_words, 256), 2048),
+        (uint32_t)std::min(cdiv(n, 256), 1024),
+        (uint32_t)cdiv(n, kMortonPerBlock),
+        (uint32_t)b->ntiles,
+        (uint32_t)cdiv(n, kClimbChunk),
+        n > 1 ? (uint32_t)top_pass_grid(n) : 0u,
+        n > 1 ? 1u : 0u,
+        wide ? (uint32_t)cdiv(n - 1, 256) : 0u,
+        wide ? (uint32_t)cdiv(n - 1, 1024) : 0u,
+        wide ? 1u : 0u,
+        wide ? (uint32_t)cdiv(n - 1, 1024) : 0u,
+        wide ? (uint32_t)cdiv(n - 1, 256) : 0u};
+    for (int st = 0; st < kSegStages; ++st) {
+      uint32_t* o = off.data() + (size_t)st * (count + 1);
+      if ((uint64_t)o[i] + blocks[st] >= (1ull << 31) / 1024) {  // grid too large for one launch
+        delete p;
+        return hipSuccess;
+      }
+      o[i + 1] = o[i] + blocks[st];
+    }
+    p->passes = std::max(p->passes, top_passes(n));
+  }
+  const size_t seg_bytes = rt_align256(sizeof(SegBvh) * count);
+  hipError_t e = hipMalloc(&p->dev, seg_bytes + off.size() * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemcpy(p->dev, segs.data(), sizeof(SegBvh) * count, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy((char*)p->dev + seg_bytes, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    rt_seg_build_destroy(p);
+    return e;
+  }
+  p->segs = (const SegBvh*)p->dev;
+  for (int st = 0; st < kSegStages; ++st) {
+    p->off[st] = (const uint32_t*)((char*)p->dev + seg_bytes) + (size_t)st * (count + 1);
+    p->total[st] = off[(size_t)st * (count + 1) + count];
+  }
+  *out = p;
+  return hipSuccess;
+}
+
+namespace {
+template <typename K>
+void seg_build_typed(const rt_seg_build* p, hipStream_t s) {
+  constexpr int P = KeyTraits<K>::kPasses;
+  const SegBvh* g = p->segs;
+  const int ns = p->ns;
+#define RT_SEG(ST, KERNEL, THREADS, ...) \
+  if (p->total[ST]) KERNEL<<<p->total[ST], THREADS, 0, s>>>(g, p->off[ST], ns, ##__VA_ARGS__)
+  RT_SEG(kStInit, k_seg_init, 256);
+  RT_SEG(kStAabb, k_seg_aabb_bounds, 256);
+  RT_SEG(kStMorton, k_seg_morton<K>, 256);
+  for (int q = 0; q < P; ++q) RT_SEG(kStSort, k_seg_onesweep<K>, kSortThreads, q);
+  RT_SEG(kStChunks, k_seg_climb_chunks<K>, kClimbThreads);
+  for (int q = 0; q < p->passes; ++q) RT_SEG(kStPass, k_seg_climb_pass<K>, 256, q);
+  RT_SEG(kStTop, k_seg_climb_top<K>, kClimbTopThreads, p->passes);
+  RT_SEG(kStParity, k_seg_wide_parity, 256);
+  RT_SEG(kStPackCount, k_seg_pack_count, 1024);
+  RT_SEG(kStPackOffsets, k_seg_pack_offsets, 1024);
+  RT_SEG(kStPackIndex, k_seg_pack_index, 1024);
+  RT_SEG(kStWideFill, k_seg_wide_fill, 256);
+#undef RT_SEG
+}
+}  // namespace
+
+hipError_t rt_seg_build_launch(const rt_seg_build* p, hipStream_t s) {
+  if (p->bits == 30)
+    seg_build_typed<uint32_t>(p, s);
+  else
+    seg_build_typed<uint64_t>(p, s);
   return hipGetLastError();
 }
 

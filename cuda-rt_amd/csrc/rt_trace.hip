@@ -1327,10 +1327,12 @@ struct WfState {
   int32_t* spos;        // cap: any-hit leaf position or -1
   double* st;           // cap: any-hit t
   uint32_t* scount;     // number of shadow entries (counts[2] or counts[3])
-  uint16_t* skey;       // cap: direction cell of each shadow ray (single-scene renders)
-  uint32_t* sperm;      // cap: shadow rays in cell order
-  uint32_t* shist;      // kShadowCells: cell counts (zero between sorts)
-  uint32_t* sbase;      // kShadowCells: cell offsets
+  uint16_t* skey;       // cap: sort key of each shadow ray: (scene, direction cell)
+  uint32_t* sperm;      // cap: shadow rays in key order
+  uint32_t* shist;      // sbins: key counts (zero between sorts)
+  uint32_t* sbase;      // sbins: key offsets
+  uint32_t sbins;       // number of keys: kShadowCells for one scene, nscenes << scell_bits for a batch
+  int scell_bits;       // bits of the cell index in a batch key
   uint32_t* counts;     // [0],[1] ray queue sizes, [2],[3] shadow entries per parity, [8] stack overflows
   uint32_t* fetch[2];   // kFetchWords each: closest-hit and any-hit fetch counters (the latter per parity)
   int2* rec;            // optional path record: (hit leaf, occluder code) per pixel, sample, bounce
@@ -1367,6 +1369,16 @@ constexpr int kShadowBits = __builtin_ctz(kShadowGrid);
 constexpr int kShadowCells = kShadowGrid * kShadowGrid;
 constexpr int kScanThreads = 1024;
 static_assert((kShadowGrid & (kShadowGrid - 1)) == 0 && kShadowCells >= 64, "shadow grid");
+// A batch sorts by (scene, cell) in a 16-bit key: the scene index in the
+// high bits, the cell's Morton index below it. Past 64 scenes the cell
+// index loses low bit pairs, i.e. the grid halves per side each time.
+inline int batch_cell_bits(size_t nscenes) {
+  int sb = 0;
+  while (sb < 16 && (size_t(1) << sb) < nscenes) ++sb;
+  const int cb = (16 - sb) & ~1;
+  return cb < 2 * kShadowBits ? cb : 2 * kShadowBits;
+}
+inline uint32_t sort_bins(size_t nscenes) { return (uint32_t)(nscenes << batch_cell_bits(nscenes)); }
 struct ShadowCells {
   float lx, ly, lz;           // light centre
   float wx, wy, wz;           // unit direction from the light to the box centre
@@ -1510,7 +1522,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_init(WfState W, SceneDesc D1) {
   if (blockIdx.x == 0) {
     if (threadIdx.x < 16) W.counts[threadIdx.x] = 0u;
     for (int k = threadIdx.x; k < kCtrReplicas * kCtrStride; k += kThreads) W.rep[k] = 0ull;
-    for (int k = threadIdx.x; k < kShadowCells; k += kThreads) W.shist[k] = 0u;
+    for (uint32_t k = threadIdx.x; k < W.sbins; k += kThreads) W.shist[k] = 0u;
   }
   int s, px, py;
   uint32_t gp;
@@ -1584,7 +1596,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
   bool want_shadow = false, want_next = false;
   ShadePrep S;
   d3 thr = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
-  uint16_t sc = 0;
+  uint16_t sc = 0, skey = 0;
   ShadowCells cells;
   cells.ok = false;
   if (!MULTI && kShadowSort && D1.B.n > 0) cells = shadow_cells(D1.P, D1.B.node_box);
@@ -1604,6 +1616,9 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
       const rt_material& m = *LS.m;
       S = shade_prep(D.P, m, blend_normal(LS.na, LS.nb, LS.nc, hh[1], hh[2]), o, d, hh[0], rng);
       want_shadow = true;
+      if (MULTI && kShadowSort)
+        skey = (uint16_t)(((uint32_t)sc << W.scell_bits) |
+                          (shadow_key(shadow_cells(D.P, D.B.node_box), S.q) >> (2 * kShadowBits - W.scell_bits)));
       if (bounce < D.P.depth) {
         double u1 = rt::rand_unit(rng);
         double u2 = rt::rand_unit(rng);
@@ -1621,7 +1636,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
   if (want_shadow) {
     W.spix[ss] = p;
     W.sscene[ss] = sc;
-    if (!MULTI && kShadowSort) W.skey[ss] = shadow_key(cells, S.q);
+    if (kShadowSort) W.skey[ss] = MULTI ? skey : shadow_key(cells, S.q);
     // shadow entry, component-major: q, L, I1, ds, thr, fo
     double* e = W.shq + ss;
     const size_t c = W.cap;
@@ -1722,6 +1737,78 @@ __global__ __launch_bounds__(kThreads) void k_sort_scatter(WfState W) {
   for (int j = 0; j < kSortPer; ++j) {
     const uint32_t i = i0 + j * kThreads;
     if (i < n) W.sperm[h[key[j]] + rank[j]] = i;
+  }
+}
+
+// The batch form of the three passes, for up to 65536 keys: global counts,
+// with one atomic per run of equal keys among a wave's lanes (neighbouring
+// queue entries mostly share a key).
+struct KeyRun {
+  bool head;      // first lane of its run (and lane valid)
+  uint32_t first; // first lane of this lane's run
+  uint32_t len;   // run length (heads)
+};
+__device__ __forceinline__ KeyRun key_run(uint32_t key, bool valid) {
+  const uint32_t lane = __lane_id();
+  const uint32_t pk = __shfl_up(key, 1, 64);
+  const bool pv = __shfl_up((int)valid, 1, 64) != 0;
+  const bool start = lane == 0 || !valid || !pv || pk != key;
+  const uint64_t heads = __ballot(start);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  KeyRun r;
+  r.first = 63u - (uint32_t)__clzll(heads & upto);
+  const uint64_t above = heads & ~upto;
+  r.len = (above ? (uint32_t)__ffsll((long long)above) - 1u : 64u) - lane;
+  r.head = start && valid;
+  return r;
+}
+
+__global__ __launch_bounds__(kThreads) void k_msort_hist(WfState W) {
+  const uint32_t n = *W.scount;
+  for (uint32_t b = blockIdx.x * kThreads; b < n; b += gridDim.x * kThreads) {
+    const uint32_t i = b + threadIdx.x;
+    const uint32_t key = i < n ? W.skey[i] : 0u;
+    const KeyRun r = key_run(key, i < n);
+    if (r.head) atomicAdd(&W.shist[key], r.len);
+  }
+}
+
+// One workgroup; thread k scans keys [k*per, (k+1)*per).
+__global__ __launch_bounds__(kScanThreads) void k_msort_scan(WfState W) {
+  __shared__ uint32_t part[kScanThreads / 64];
+  const uint32_t nb = W.sbins, per = (nb + kScanThreads - 1) / kScanThreads;
+  const int k = threadIdx.x, lane = k & 63, wv = k >> 6;
+  const uint32_t c0 = min(nb, k * per), c1 = min(nb, c0 + per);
+  uint32_t sum = 0;
+  for (uint32_t c = c0; c < c1; ++c) sum += W.shist[c];
+  uint32_t x = sum;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) part[wv] = x;
+  __syncthreads();
+  uint32_t run = x - sum;
+  for (int j = 0; j < wv; ++j) run += part[j];
+  for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t v = W.shist[c];
+    W.sbase[c] = run;
+    W.shist[c] = 0u;  // ready for the next sort of this parity
+    run += v;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_msort_scatter(WfState W) {
+  const uint32_t n = *W.scount;
+  for (uint32_t b = blockIdx.x * kThreads; b < n; b += gridDim.x * kThreads) {
+    const uint32_t i = b + threadIdx.x;
+    const uint32_t key = i < n ? W.skey[i] : 0u;
+    const KeyRun r = key_run(key, i < n);
+    uint32_t base = 0;
+    if (r.head) base = atomicAdd(&W.sbase[key], r.len);
+    base = __shfl(base, (int)r.first, 64);
+    if (i < n) W.sperm[base + (__lane_id() - r.first)] = i;
   }
 }
 
@@ -1969,6 +2056,7 @@ struct WfLayout {
 static WfLayout wf_layout(size_t npix, size_t ntiles, size_t nscenes) {
   WfLayout L;
   size_t cap = ntiles * kThreads;
+  const size_t bins = sort_bins(nscenes);
   size_t sizes[] = {nscenes * sizeof(SceneDesc), nscenes * sizeof(BvhView), ntiles * 2, npix * 4, npix * 24,
                     npix * 24, npix * 24, cap * 48, cap * 48, cap * 4, cap * 4, cap * 2, cap * 2, cap * 4,
                     cap * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, 256, kFetchWords * 4, kFetchWords * 4,
@@ -1976,8 +2064,8 @@ static WfLayout wf_layout(size_t npix, size_t ntiles, size_t nscenes) {
                     // parity 1: accumulator, shadow queue, any-hit fetch counters
                     npix * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, kFetchWords * 4,
                     // shadow-ray sort, per parity: keys, order, cell counts, cell offsets
-                    cap * 2, cap * 4, kShadowCells * 4, kShadowCells * 4,
-                    cap * 2, cap * 4, kShadowCells * 4, kShadowCells * 4};
+                    cap * 2, cap * 4, bins * 4, bins * 4,
+                    cap * 2, cap * 4, bins * 4, bins * 4};
   size_t off = 0;
   for (int k = 0; k < (int)(sizeof(sizes) / sizeof(sizes[0])); ++k) {
     L.off[k] = off;
@@ -2038,6 +2126,8 @@ static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes, in
   W.sperm = (uint32_t*)(w + L.off[k++]);
   W.shist = (uint32_t*)(w + L.off[k++]);
   W.sbase = (uint32_t*)(w + L.off[k++]);
+  W.sbins = sort_bins(nscenes);
+  W.scell_bits = batch_cell_bits(nscenes);
   W.npix = npix;
   W.cap = ntiles * kThreads;
   return W;
@@ -2143,7 +2233,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     k_wf_init<false><<<tiles, kThreads, 0, s>>>(W0, D1);
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W0.fetch[0], 2 * kFetchWords);
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[1].fetch[1], kFetchWords);
-  if (!multi) k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[1].shist, kShadowCells);
+  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[1].shist, Wp[1].sbins);
   // The last bounce's any-hit pass, shadow resolve and flush of sample k run
   // on the auxiliary stream, overlapping sample k+1's camera, closest-hit
   // and shading passes on s: they share no buffer but the replicated
@@ -2178,13 +2268,17 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
         RT_TRY(hipEventRecord(ov->shaded, s));
         RT_TRY(hipStreamWaitEvent(sa, ov->shaded, 0));
       }
-      const bool ssort = !multi && kShadowSort;
-      RayIn SR{W.shq, W.shq + 3 * cap, 1, cap, W.scount, 0, ssort ? W.sperm : nullptr};
-      if (ssort) {
+      RayIn SR{W.shq, W.shq + 3 * cap, 1, cap, W.scount, 0, kShadowSort ? W.sperm : nullptr};
+      if (kShadowSort && !multi) {
         const int sort_blocks = cdiv((long)cap, kSortSpan);
         k_sort_hist<<<sort_blocks, kThreads, 0, sb>>>(W);
         k_sort_scan<<<1, kScanThreads, 0, sb>>>(W);
         k_sort_scatter<<<sort_blocks, kThreads, 0, sb>>>(W);
+      } else if (kShadowSort) {
+        const int sort_blocks = std::min(cdiv((long)cap, kThreads), 4096);
+        k_msort_hist<<<sort_blocks, kThreads, 0, sb>>>(W);
+        k_msort_scan<<<1, kScanThreads, 0, sb>>>(W);
+        k_msort_scatter<<<sort_blocks, kThreads, 0, sb>>>(W);
       }
       HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr, 0};
       if (T) RT_TRY(timer_mark(T, 1, true, sb));

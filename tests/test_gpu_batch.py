@@ -63,6 +63,68 @@ def test_batch_matches_single_and_oracle(ctx):
     assert batch.diagnostics() == 0
 
 
+_EXPORT_KEYS = ("codes", "ids", "parent", "left", "right", "lo", "hi", "tri_lo", "tri_hi")
+
+
+@pytest.mark.parametrize("bits", [30, 63])
+def test_batch_rebuild_matches_single_builds(ctx, bits):
+    """rt_batch_rebuild builds every scene in shared launches (one per build
+    stage): each BVH equals a single-scene build of the moved positions, bit
+    for bit, and its image equals a single-scene render. Sizes cover one
+    triangle, a few, chunk edges (1024 leaves per climb chunk) and scenes
+    that need several top-level passes."""
+    sizes = [1, 2, 3, 1023, 1025, 4097, 70_000, 250_000]
+    print(f"segmented batch rebuild bits={bits} sizes={sizes}", flush=True)
+    specs = []
+    for k, n in enumerate(sizes):
+        sp = scenes.batch_scenes(1, 20 + k, tri_range=(n, n + 1), resolutions=(32,), spp_range=(1, 1))[0]
+        specs.append(sp)
+    sc = _make(ctx, specs, bits)
+    batch = rtamd.Batch(ctx, sc)
+    moved = [rtamd.gen_soup(s["mesh"].n, 0xBEEF + k) for k, s in enumerate(sc)]
+    for s, hm in zip(sc, moved):
+        s["mesh"].set_positions(*(torch.from_numpy(a).cuda() for a in (hm.vx, hm.vy, hm.vz)))
+    batch.rebuild()
+    batch.render()
+    torch.cuda.synchronize()
+    for k, (s, hm) in enumerate(zip(sc, moved)):
+        got = s["bvh"].export()
+        ref_bvh = rtamd.BVH(ctx, rtamd.DeviceMesh(hm), bits)
+        ref = ref_bvh.export()
+        for key in _EXPORT_KEYS:
+            a, r = got[key], ref[key]
+            if a.dtype == np.float32:
+                a, r = a.view(np.uint32), r.view(np.uint32)
+            assert np.array_equal(a, r), f"scene {k} (n={sizes[k]}): {key} differs"
+        assert s["bvh"].diagnostics() == (0, 0)
+        single = _single(ctx, s)
+        torch.cuda.synchronize()
+        assert np.array_equal(s["rgb"].cpu().numpy(), single.cpu().numpy()), f"scene {k} image differs"
+    b0 = sc[5]["bvh"].export()
+    ref = O.build(moved[5], bits)
+    assert np.array_equal(b0["parent"][1:], ref["parent"][1:]), "parent links differ from the oracle"
+    for key in ("codes", "ids", "left", "right"):
+        assert np.array_equal(b0[key], ref[key]), f"{key} differs from the oracle"
+    assert batch.diagnostics() == 0
+
+
+def test_batch_of_many_scenes_matches_single(ctx):
+    """80 scenes: the shadow-ray sort key keeps 7 bits for the scene and a
+    16 x 16 light-cell grid (64 scenes or fewer keep the 32 x 32 grid)."""
+    print("batch of 80 scenes vs single", flush=True)
+    specs = scenes.batch_scenes(80, 14, tri_range=(300, 4_000), resolutions=(16, 24, 32), spp_range=(1, 2))
+    specs[7]["opts"].depth = 1
+    sc = _make(ctx, specs)
+    batch = rtamd.Batch(ctx, sc)
+    batch.render()
+    torch.cuda.synchronize()
+    for k, s in enumerate(sc):
+        single = _single(ctx, s)
+        torch.cuda.synchronize()
+        assert np.array_equal(s["rgb"].cpu().numpy(), single.cpu().numpy()), f"scene {k} differs from single render"
+    assert batch.diagnostics() == 0
+
+
 def test_batch_outputs_do_not_depend_on_tuning(ctx):
     print("batch tuning invariance", flush=True)
     specs = scenes.batch_scenes(5, 12, tri_range=(1000, 60_000), resolutions=(64, 96), spp_range=(1, 2))
