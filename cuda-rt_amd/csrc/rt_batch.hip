@@ -66,10 +66,12 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
   b->items.assign(items, items + count);
   b->nscenes = (size_t)count;
   b->tuning = rt_default_tuning();
-  std::vector<size_t> pix_off(count), tile_off(count), shade_off(count);
-  size_t shade_bytes = 0;
+  std::vector<size_t> pix_off(count), tile_off(count), shade_off(count), tri_off(count);
+  size_t shade_bytes = 0, ntris = 0;
   for (int i = 0; i < count; ++i) {
     const rt_camera& c = items[i].cam;
+    tri_off[i] = ntris;
+    ntris += (size_t)items[i].mesh.n;
     shade_off[i] = shade_bytes;
     shade_bytes += rt_align256(rt_shade_table_bytes((size_t)items[i].mesh.n));
     pix_off[i] = b->npix;
@@ -79,9 +81,9 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
     b->max_spp = items[i].opts.spp > b->max_spp ? items[i].opts.spp : b->max_spp;
     b->max_depth = items[i].opts.depth > b->max_depth ? items[i].opts.depth : b->max_depth;
   }
-  if (b->npix >= (1ull << 32) || b->ntiles * 256 >= (1ull << 32)) {
+  if (b->npix >= (1ull << 32) || b->ntiles * 256 >= (1ull << 32) || ntris >= (1ull << 32)) {
     rt_batch_destroy(b);
-    return rt_set_error(ctx, RT_ERR_INVALID, "batch has too many pixels");
+    return rt_set_error(ctx, RT_ERR_INVALID, "batch has too many pixels or triangles");
   }
   size_t bytes = rt_wavefront_bytes_batch(b->npix, b->ntiles, b->nscenes);
   if (hipMalloc(&b->ws, bytes) != hipSuccess || hipMalloc(&b->shade, shade_bytes) != hipSuccess) {
@@ -96,7 +98,8 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
   for (int i = 0; i < count; ++i) {
     const rt_batch_item& it = items[i];
     rt_fill_scene(table.data() + sd * i, it.bvh, &it.mesh, it.mats_dev, it.nmat, &it.light, &it.cam, &it.opts,
-                  it.rgb_dev, (uint32_t)pix_off[i], (uint32_t)tile_off[i], (char*)b->shade + shade_off[i]);
+                  it.rgb_dev, (uint32_t)pix_off[i], (uint32_t)tile_off[i], (uint32_t)tri_off[i],
+                  (char*)b->shade + shade_off[i]);
     BvhView v = it.bvh->view();
     ::memcpy(views.data() + vb * i, &v, sizeof(v));
     size_t nt = (size_t)((it.cam.width + 15) / 16) * ((it.cam.height + 15) / 16);

@@ -226,7 +226,7 @@ size_t rt_scene_desc_bytes();
 size_t rt_bvh_view_bytes();
 void rt_fill_scene(void* table_entry, const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                    const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
-                   uint32_t pix_off, uint32_t tile_off, void* shade_table);
+                   uint32_t pix_off, uint32_t tile_off, uint32_t tri_off, void* shade_table);
 hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
                                uint32_t* overflow, hipStream_t s, int32_t* records = nullptr,

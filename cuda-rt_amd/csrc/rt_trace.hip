@@ -1303,6 +1303,7 @@ struct SceneDesc {
   float* rgb;
   uint32_t pix_off;   // first pixel of the scene in the per-pixel arrays
   uint32_t tile_off;  // first 16x16 tile of the scene
+  uint32_t tri_off;   // triangles of the scenes before this one (batches)
 };
 
 struct WfState {
@@ -1567,21 +1568,40 @@ __global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, SceneDesc D1,
 // Shading table of each scene (blockIdx.y), in triangle order so the
 // index, normal and material reads are as coherent as the mesh; each
 // triangle's record goes to its sorted leaf position.
-template <bool MULTI>
-__global__ __launch_bounds__(256) void k_shade_fill(const SceneDesc* scenes, SceneDesc D1) {
-  const SceneDesc& D = MULTI ? scenes[blockIdx.y] : D1;
+__device__ __forceinline__ void shade_fill_one(const SceneDesc& D, int t) {
   const MeshShade& M = D.M;
   float4* out = const_cast<float4*>(M.lshade);
   if (!out) return;
-  const int n = D.B.n;
-  for (int t = blockIdx.x * 256 + threadIdx.x; t < n; t += gridDim.x * 256) {
-    const int a = M.idx[3 * t], b = M.idx[3 * t + 1], c = M.idx[3 * t + 2];
-    const int mat = M.mat_id ? M.mat_id[t] : 0;
-    float4* r = out + 4 * (size_t)D.B.leaf_of[t];
-    r[0] = make_float4(M.nx[a], M.ny[a], M.nz[a], __int_as_float(mat));
-    r[1] = make_float4(M.nx[b], M.ny[b], M.nz[b], 0.f);
-    r[2] = make_float4(M.nx[c], M.ny[c], M.nz[c], 0.f);
-    r[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+  const int a = M.idx[3 * t], b = M.idx[3 * t + 1], c = M.idx[3 * t + 2];
+  const int mat = M.mat_id ? M.mat_id[t] : 0;
+  float4* r = out + 4 * (size_t)D.B.leaf_of[t];
+  r[0] = make_float4(M.nx[a], M.ny[a], M.nz[a], __int_as_float(mat));
+  r[1] = make_float4(M.nx[b], M.ny[b], M.nz[b], 0.f);
+  r[2] = make_float4(M.nx[c], M.ny[c], M.nz[c], 0.f);
+  r[3] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// A batch runs one thread per triangle of all its scenes in turn (the
+// scene of triangle g is the last s with tri_off <= g), so every scene gets
+// threads in proportion to its size.
+template <bool MULTI>
+__global__ __launch_bounds__(256) void k_shade_fill(const SceneDesc* scenes, int nscenes, SceneDesc D1) {
+  if constexpr (MULTI) {
+    const SceneDesc& last = scenes[nscenes - 1];
+    const uint32_t total = last.tri_off + (uint32_t)last.B.n;
+    for (uint32_t g = blockIdx.x * 256 + threadIdx.x; g < total; g += gridDim.x * 256) {
+      int lo = 0, hi = nscenes - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (scenes[mid].tri_off <= g)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      shade_fill_one(scenes[lo], (int)(g - scenes[lo].tri_off));
+    }
+  } else {
+    for (int t = blockIdx.x * 256 + threadIdx.x; t < D1.B.n; t += gridDim.x * 256) shade_fill_one(D1, t);
   }
 }
 
@@ -2149,7 +2169,7 @@ size_t rt_wavefront_bytes(int32_t width, int32_t height) {
 // Fill the scene table of a batch (host side; the caller copies it to the device).
 void rt_fill_scene(void* table_entry, const rt_bvh* b, const rt_mesh_view* m, const rt_material* mats, int32_t nmat,
                    const rt_light* light, const rt_camera* cam, const rt_render_opts* opts, float* rgb,
-                   uint32_t pix_off, uint32_t tile_off, void* shade_table) {
+                   uint32_t pix_off, uint32_t tile_off, uint32_t tri_off, void* shade_table) {
   SceneDesc d;
   d.B = b->view();
   d.M = MeshShade{m->nx_dev, m->ny_dev, m->nz_dev, m->idx_dev, m->mat_id_dev, (const float4*)shade_table};
@@ -2158,6 +2178,7 @@ void rt_fill_scene(void* table_entry, const rt_bvh* b, const rt_mesh_view* m, co
   d.rgb = rgb;
   d.pix_off = pix_off;
   d.tile_off = tile_off;
+  d.tri_off = tri_off;
   *(SceneDesc*)table_entry = d;
 }
 size_t rt_scene_desc_bytes() { return sizeof(SceneDesc); }
@@ -2220,10 +2241,9 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   const int tiles = (int)ntiles;
   const bool work = cnt || T;  // keep the replicated counts
   if (multi) {
-    const int gx = (int)std::max<size_t>(1, 4096 / nscenes);
-    k_shade_fill<true><<<dim3(gx, (unsigned)nscenes), 256, 0, s>>>(W0.scenes, D1);
+    k_shade_fill<true><<<4096, 256, 0, s>>>(W0.scenes, (int)nscenes, D1);
   } else if (D1.M.lshade) {
-    k_shade_fill<false><<<std::min(cdiv(D1.B.n, 256), 4096), 256, 0, s>>>(W0.scenes, D1);
+    k_shade_fill<false><<<std::min(cdiv(D1.B.n, 256), 4096), 256, 0, s>>>(W0.scenes, 1, D1);
   }
   const int grid = persistent_grid(num_cus, t, (uint32_t)cap);
   uint32_t* ovf = overflow ? overflow : W0.counts + 8;
@@ -2332,6 +2352,7 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   d.rgb = rgb;
   d.pix_off = 0;
   d.tile_off = 0;
+  d.tri_off = 0;
   WfState W = wf_state(workspace, (size_t)P.W * P.H, (size_t)tiles, 1);
   k_wf_put_scene<<<cdiv(tiles, 256), 256, 0, s>>>((SceneDesc*)W.scenes, (BvhView*)W.views,
                                                   (uint16_t*)W.tile_scene, d, (uint32_t)tiles);
