@@ -132,9 +132,10 @@ rt_status rt_batch_rebuild(rt_batch* b, void* stream) {
   if (!b) return RT_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
   // Every stage in one launch over all scenes, unless a scene also keeps
-  // the nodes of layout 4 or 8 (the segmented build fills layout 5's).
+  // the nodes of layout 2, 4 or 8 (the segmented build fills layout 5's).
   bool seg = b->seg != nullptr;
-  for (const rt_batch_item& it : b->items) seg = seg && !it.bvh->want_wide4 && !it.bvh->want_wide8;
+  for (const rt_batch_item& it : b->items)
+    seg = seg && !it.bvh->want_tnode && !it.bvh->want_wide4 && !it.bvh->want_wide8;
   if (seg) {
     RT_HIP(b->ctx, rt_seg_build_launch(b->seg, s));
     return RT_OK;

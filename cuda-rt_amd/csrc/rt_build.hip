@@ -553,11 +553,13 @@ __device__ __forceinline__ void write_node(const ClimbNode& c, int idx, int n, i
   float* dst = node_box + 6 * (size_t)idx;
 #pragma unroll
   for (int k = 0; k < 6; ++k) dst[k] = c.b[k];
-  float4* T = tnode + 4 * (size_t)idx;
-  T[0] = make_float4(c.lb[0], c.lb[1], c.lb[2], c.lb[3]);
-  T[1] = make_float4(c.lb[4], c.lb[5], c.rb[0], c.rb[1]);
-  T[2] = make_float4(c.rb[2], c.rb[3], c.rb[4], c.rb[5]);
-  T[3] = make_float4(__int_as_float(child_ref(c.lidx, n)), __int_as_float(child_ref(c.ridx, n)), 0.f, 0.f);
+  if (tnode) {
+    float4* T = tnode + 4 * (size_t)idx;
+    T[0] = make_float4(c.lb[0], c.lb[1], c.lb[2], c.lb[3]);
+    T[1] = make_float4(c.lb[4], c.lb[5], c.rb[0], c.rb[1]);
+    T[2] = make_float4(c.rb[2], c.rb[3], c.rb[4], c.rb[5]);
+    T[3] = make_float4(__int_as_float(child_ref(c.lidx, n)), __int_as_float(child_ref(c.ridx, n)), 0.f, 0.f);
+  }
   left[idx] = c.lidx;
   right[idx] = c.ridx;
   parent[c.lidx] = idx;
@@ -1263,6 +1265,27 @@ __global__ __launch_bounds__(256) void k_seg_wide_fill(const SegBvh* segs, const
   wide_fill_body(b.bid, b.nblk, g.n, g.left, g.right, g.node_box, g.wide_flag, nullptr, g.wideq, g.wcomp);
 }
 
+// Binary traversal records of every internal node from the topology and
+// the refit boxes (the layout-2 traversal and the megakernel read them; the
+// climb writes them itself while a BVH wants them).
+__global__ __launch_bounds__(256) void k_tnode_fill(int n, const int32_t* __restrict__ left,
+                                                    const int32_t* __restrict__ right,
+                                                    const float* __restrict__ node_box, float4* __restrict__ tnode) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n - 1) return;
+  const int l = left[i], r = right[i];
+  const float* lb = node_box + 6 * (size_t)l;
+  const float* rb = node_box + 6 * (size_t)r;
+  float4* T = tnode + 4 * (size_t)i;
+  T[0] = make_float4(lb[0], lb[1], lb[2], lb[3]);
+  T[1] = make_float4(lb[4], lb[5], rb[0], rb[1]);
+  T[2] = make_float4(rb[2], rb[3], rb[4], rb[5]);
+  T[3] = make_float4(__int_as_float(child_ref(l, n)), __int_as_float(child_ref(r, n)), 0.f, 0.f);
+}
+
+// The climb writes binary records only while a traversal wants them.
+inline float4* tnode_out(const rt_bvh* b) { return b->want_tnode ? b->tnode : nullptr; }
+
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
 void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
@@ -1305,10 +1328,10 @@ void launch_climb_top(rt_bvh* b, const K* keys, hipStream_t s) {
   uint32_t* lists[2] = {b->deferred, b->deferred2};
   const int grid = top_pass_grid(n);
   for (int p = 0; p < passes; ++p)
-    k_climb_pass<K><<<grid, 256, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode, b->deposits,
+    k_climb_pass<K><<<grid, 256, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, tnode_out(b), b->deposits,
                                          b->arrivals, lists[p & 1], lists[(p + 1) & 1], b->misc, hand_off_slot(p),
                                          hand_off_slot(p + 1), hand_off_slot(p + 2));
-  k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, b->tnode,
+  k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, tnode_out(b),
                                                 b->deposits, b->arrivals, lists[passes & 1], b->misc,
                                                 hand_off_slot(passes));
 }
@@ -1340,7 +1363,7 @@ hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
       k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
   } else {
     k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
-        keys, b->ids[0], n, b->trirec, b->parent, b->left, b->right, b->node_box, b->tnode, b->ltri, b->deposits,
+        keys, b->ids[0], n, b->trirec, b->parent, b->left, b->right, b->node_box, tnode_out(b), b->ltri, b->deposits,
         b->arrivals, b->deferred, b->misc, b->leaf_of);
     launch_climb_top<K>(b, keys, s);
   }
@@ -1367,7 +1390,7 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
                                                         b->trirec, b->misc, 0);
   const K* keys = (const K*)b->keys[0];
   k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
-      keys, b->ids[0], n, b->trirec, b->parent, b->left, b->right, b->node_box, b->tnode, b->ltri, b->deposits,
+      keys, b->ids[0], n, b->trirec, b->parent, b->left, b->right, b->node_box, tnode_out(b), b->ltri, b->deposits,
       b->arrivals, b->deferred, b->misc, nullptr);
   launch_climb_top<K>(b, keys, s);
   launch_wide(b, false, s);
@@ -1378,7 +1401,10 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
 
 hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s) {
   const int n = b->n;
-  if (layout == 4 && !b->want_wide4) {
+  if (layout == 2 && !b->want_tnode) {
+    b->want_tnode = true;
+    if (n >= 2) k_tnode_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->tnode);
+  } else if (layout == 4 && !b->want_wide4) {
     b->want_wide4 = true;
     if (n >= 2)
       k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide, nullptr,
@@ -1392,6 +1418,7 @@ hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s) {
 }
 
 void rt_keep_only_layout(const rt_bvh* b, int layout) {
+  b->want_tnode = b->want_tnode && layout == 2;
   b->want_wide4 = b->want_wide4 && layout == 4;
   b->want_wide8 = b->want_wide8 && layout == 8;
 }
@@ -1463,7 +1490,7 @@ hipError_t rt_seg_build_create(const rt_batch_item* items, int count, rt_seg_bui
     g.leaf_of = b->leaf_of;
     g.wcomp = b->wcomp;
     g.node_box = b->node_box;
-    g.tnode = b->tnode;
+    g.tnode = nullptr;  // the segmented build runs only while no BVH wants binary records
     g.ltri = b->ltri;
     g.deposits = b->deposits;
     g.wideq = b->wideq;

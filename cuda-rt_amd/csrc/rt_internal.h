@@ -138,9 +138,10 @@ struct rt_bvh {
   float4* wide8 = nullptr;       // (n-1)*8
   uint8_t* wide_flag = nullptr;  // n-1: bit 0 = even depth (4-wide node), bit 1 = depth divisible by 3 (8-wide)
   int32_t* leaf_of = nullptr;    // n: triangle id -> sorted leaf position
-  // The unquantized 4-wide nodes and the 8-wide nodes are only kept up to
-  // date once a traversal has asked for them (rt_ensure_layout); from then on
-  // every build and refit fills them.
+  // The binary records, the unquantized 4-wide nodes and the 8-wide nodes
+  // are only kept up to date once a traversal has asked for them
+  // (rt_ensure_layout); from then on every build and refit fills them.
+  mutable bool want_tnode = false;  // binary records (layout 2, the megakernel); the search build always writes them
   mutable bool want_wide4 = false;
   mutable bool want_wide8 = false;
 
@@ -189,16 +190,17 @@ hipError_t rt_launch_refit(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
 // Before the first traversal of a BVH with node layout 4 or 8: fill those
 // nodes now and keep them filled by every later build and refit.
 hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s);
-// Builds and refits stop filling the unquantized 4-wide and the 8-wide nodes
-// unless `layout` traverses them; rt_ensure_layout fills them again on demand.
+// Builds and refits stop filling the binary records and the unquantized
+// 4-wide and 8-wide nodes unless `layout` traverses them; rt_ensure_layout
+// fills them again on demand.
 void rt_keep_only_layout(const rt_bvh* b, int layout);
 
 // Segmented build of a batch's BVHs: each build stage is one launch over
 // all of them. create returns *out = nullptr (and hipSuccess) when the BVHs
 // cannot share launches (mixed key widths, the search hierarchy, n < 1).
 // The launch builds as rt_launch_build does for every BVH, except the
-// unquantized 4-wide and the 8-wide nodes: use it only while no BVH wants
-// them (want_wide4, want_wide8).
+// binary records and the unquantized 4-wide and 8-wide nodes: use it only
+// while no BVH wants them (want_tnode, want_wide4, want_wide8).
 struct rt_seg_build;
 hipError_t rt_seg_build_create(const rt_batch_item* items, int count, rt_seg_build** out);
 hipError_t rt_seg_build_launch(const rt_seg_build* p, hipStream_t s);

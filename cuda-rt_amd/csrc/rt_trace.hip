@@ -2058,6 +2058,8 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
   RayIn R{rays->orig_dev, rays->dir_dev, 3, 1, nullptr, (uint32_t)rays->n, nullptr};
   HitOut O{nullptr, hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev, nullptr, 0};
   rt_tuning t = rt_default_tuning();
+  hipError_t e = rt_ensure_layout(b, t.node_layout, s);
+  if (e != hipSuccess) return e;
   int grid = persistent_grid(b->ctx ? b->ctx->num_cus : 256, t, (uint32_t)rays->n);
   if (mode == RT_TRACE_ANY)
     launch_persistent<true>(t, grid, s, view, nullptr, R, 0.0, O, fetch, b->misc + kMiscOverflow, nullptr);
@@ -2336,7 +2338,9 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   unsigned long long* cnt = (unsigned long long*)counters;
   uint32_t* ovf = b->misc + kMiscOverflow;
   if (!workspace || !shade_table) {
-    // megakernel; with counters, the workspace holds the counter replicas
+    // megakernel (binary records); with counters, the workspace holds the counter replicas
+    hipError_t e = rt_ensure_layout(b, 2, s);
+    if (e != hipSuccess) return e;
     unsigned long long* rep = cnt ? (unsigned long long*)workspace : nullptr;
     if (rep) k_zero_u64<<<cdiv(kCtrReplicas * kCtrStride, 256), 256, 0, s>>>(rep, kCtrReplicas * kCtrStride);
     k_render<<<tiles, kThreads, 0, s>>>(b->view(), M, mats, P, rgb, rep, ovf);
@@ -2353,6 +2357,8 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   d.pix_off = 0;
   d.tile_off = 0;
   d.tri_off = 0;
+  hipError_t e = rt_ensure_layout(b, rt_default_tuning().node_layout, s);
+  if (e != hipSuccess) return e;
   WfState W = wf_state(workspace, (size_t)P.W * P.H, (size_t)tiles, 1);
   k_wf_put_scene<<<cdiv(tiles, 256), 256, 0, s>>>((SceneDesc*)W.scenes, (BvhView*)W.views,
                                                   (uint16_t*)W.tile_scene, d, (uint32_t)tiles);

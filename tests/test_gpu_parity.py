@@ -103,6 +103,35 @@ def test_refit_parity(ctx, flags):
         assert b.diagnostics() == (0, 0)
 
 
+def test_megakernel_after_refit_and_rebuild(ctx):
+    """The megakernel's binary node records are filled on demand: the first
+    megakernel render after a refit fills them from the refit boxes, and from
+    then on every refit and rebuild keeps them current. Stale records would
+    make its image differ from the wavefront's, which reads the quantized
+    4-wide nodes."""
+    _say("megakernel after refit / rebuild")
+    hm = scenes.soup_mesh(30_000)
+    mats, light, cam, opts = scenes.c2_setup(64, 48, 2)
+    dm = rtamd.DeviceMesh(hm)
+    b = rtamd.BVH(ctx, dm, 63)
+    md = rtamd.materials_tensor(mats)
+    for frame in range(4):
+        moved = rtamd.gen_jitter(hm, 1e-3, 2000 + frame)
+        dm.set_positions(*[torch.from_numpy(a).cuda() for a in (moved.vx, moved.vy, moved.vz)])
+        if frame == 2:
+            b.rebuild(dm)
+        else:
+            b.refit(dm)
+        imgs = []
+        for wf in (False, True):
+            o = rtamd.render_opts(opts.spp, opts.depth, opts.seed,
+                                  opts.flags | (rtamd.RT_RENDER_WAVEFRONT if wf else 0))
+            imgs.append(rtamd.render(ctx, b, dm, md, len(mats), light, cam, o).cpu().numpy())
+        assert np.array_equal(imgs[0], imgs[1]), f"frame {frame}: megakernel and wavefront images differ"
+        assert imgs[0].max() > 0.1
+    assert b.diagnostics() == (0, 0)
+
+
 def test_build_duplicate_keys(ctx):
     """Every triangle in a tiny region: 30-bit codes collide, the index fallback orders the tree."""
     _say("build duplicate keys")
