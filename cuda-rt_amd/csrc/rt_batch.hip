@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <new>
+#include <unordered_set>
 #include <vector>
 
 #include "rt_internal.h"
@@ -23,6 +24,7 @@ constexpr int kBuildStreams = 8;
 struct rt_batch {
   rt_ctx* ctx = nullptr;
   std::vector<rt_batch_item> items;
+  std::vector<int> builds;  // the first item of each distinct BVH: rebuilds build each BVH once
   void* ws = nullptr;
   void* shade = nullptr;  // the scenes' shading tables, back to back
   size_t npix = 0, ntiles = 0, nscenes = 0;
@@ -118,7 +120,12 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
     if (e == hipSuccess) e = hipEventCreateWithFlags(&b->joins[k], hipEventDisableTiming);
   }
   if (e == hipSuccess) e = hipEventCreateWithFlags(&b->fork, hipEventDisableTiming);
-  if (e == hipSuccess) e = rt_seg_build_create(items, count, &b->seg);
+  std::unordered_set<const rt_bvh*> seen;
+  for (int i = 0; i < count; ++i)
+    if (seen.insert(items[i].bvh).second) b->builds.push_back(i);
+  std::vector<rt_batch_item> uniq;
+  for (int k : b->builds) uniq.push_back(items[k]);
+  if (e == hipSuccess) e = rt_seg_build_create(uniq.data(), (int)uniq.size(), &b->seg);
   (void)rt_ctx_overlap(ctx);  // created here, outside any graph capture of rt_batch_render
   if (e != hipSuccess) {
     rt_batch_destroy(b);
@@ -141,11 +148,12 @@ rt_status rt_batch_rebuild(rt_batch* b, void* stream) {
     return RT_OK;
   }
   RT_HIP(b->ctx, hipEventRecord(b->fork, s));
-  int used = (int)(b->nscenes < (size_t)kBuildStreams ? b->nscenes : (size_t)kBuildStreams);
+  const size_t nb = b->builds.size();
+  int used = (int)(nb < (size_t)kBuildStreams ? nb : (size_t)kBuildStreams);
   for (int k = 0; k < used; ++k) RT_HIP(b->ctx, hipStreamWaitEvent(b->streams[k], b->fork, 0));
-  for (size_t i = 0; i < b->nscenes; ++i) {
-    rt_batch_item& it = b->items[i];
-    RT_HIP(b->ctx, rt_launch_build(const_cast<rt_bvh*>(it.bvh), &it.mesh, b->streams[i % kBuildStreams]));
+  for (size_t j = 0; j < nb; ++j) {
+    rt_batch_item& it = b->items[b->builds[j]];
+    RT_HIP(b->ctx, rt_launch_build(const_cast<rt_bvh*>(it.bvh), &it.mesh, b->streams[j % kBuildStreams]));
   }
   for (int k = 0; k < used; ++k) {
     RT_HIP(b->ctx, hipEventRecord(b->joins[k], b->streams[k]));

@@ -108,6 +108,44 @@ def test_batch_rebuild_matches_single_builds(ctx, bits):
     assert batch.diagnostics() == 0
 
 
+def test_batch_items_sharing_a_bvh(ctx):
+    """Two cameras on one scene: items 0 and 2 share a BVH and mesh. A batch
+    rebuild builds that BVH once (two concurrent builds of one BVH would
+    race), and every item's image equals its single-scene render."""
+    print("batch items sharing a BVH", flush=True)
+    specs = scenes.batch_scenes(2, 30, tri_range=(20_000, 40_000), resolutions=(48,), spp_range=(1, 2))
+    sc = _make(ctx, specs)
+    third = dict(sc[0])
+    third["cam"] = scenes.c2_setup(64, 40, 1)[2]
+    third["opts"] = rtamd.render_opts(2, 1, 7, sc[0]["opts"].flags)
+    third["rgb"] = torch.zeros(64 * 40 * 3, dtype=torch.float32, device="cuda")
+    sc.append(third)
+    batch = rtamd.Batch(ctx, sc)
+    for path in ("segmented", "per-scene"):
+        if path == "per-scene":
+            # a batch whose BVHs keep layout 8 nodes rebuilds them in per-scene launch chains
+            batch.set_tuning(rtamd.Tuning(32, 64, 4, 8))
+            batch.render()
+        moved = rtamd.gen_soup(sc[0]["mesh"].n, 0xF00D if path == "segmented" else 0xF00E)
+        sc[0]["mesh"].set_positions(*(torch.from_numpy(a).cuda() for a in (moved.vx, moved.vy, moved.vz)))
+        batch.rebuild()
+        batch.render()
+        torch.cuda.synchronize()
+        got = sc[0]["bvh"].export()
+        ref = rtamd.BVH(ctx, rtamd.DeviceMesh(moved), 63).export()
+        for key in _EXPORT_KEYS:
+            a, r = got[key], ref[key]
+            if a.dtype == np.float32:
+                a, r = a.view(np.uint32), r.view(np.uint32)
+            assert np.array_equal(a, r), f"{path}: shared BVH {key} differs"
+        assert sc[0]["bvh"].diagnostics() == (0, 0)
+        for k, s in enumerate(sc):
+            single = _single(ctx, s)
+            torch.cuda.synchronize()
+            assert np.array_equal(s["rgb"].cpu().numpy(), single.cpu().numpy()), f"{path}: item {k} differs"
+    assert batch.diagnostics() == 0
+
+
 def test_batch_of_many_scenes_matches_single(ctx):
     """80 scenes: the shadow-ray sort key keeps 7 bits for the scene and a
     16 x 16 light-cell grid (64 scenes or fewer keep the 32 x 32 grid)."""
