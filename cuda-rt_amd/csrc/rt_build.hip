@@ -560,10 +560,12 @@ __device__ __forceinline__ void write_node(const ClimbNode& c, int idx, int n, i
     T[2] = make_float4(c.rb[2], c.rb[3], c.rb[4], c.rb[5]);
     T[3] = make_float4(__int_as_float(child_ref(c.lidx, n)), __int_as_float(child_ref(c.ridx, n)), 0.f, 0.f);
   }
-  left[idx] = c.lidx;
-  right[idx] = c.ridx;
-  parent[c.lidx] = idx;
-  parent[c.ridx] = idx;
+  if (left) {  // nullptr: a refit, whose topology is already in place
+    left[idx] = c.lidx;
+    right[idx] = c.ridx;
+    parent[c.lidx] = idx;
+    parent[c.ridx] = idx;
+  }
 }
 
 // Merge the arriving child c with the sibling's deposit; c becomes the parent.
@@ -645,7 +647,7 @@ __device__ __forceinline__ void climb_chunks_body(uint32_t bid, uint32_t nblk,
     dst[5] = fmaxf(cz, fmaxf(az, bz));
     if (leaf_of) leaf_of[tri] = i;  // only a build reorders the leaves
   }
-  if (s0 == 0 && threadIdx.x == 0) parent[0] = -1;
+  if (s0 == 0 && threadIdx.x == 0 && parent) parent[0] = -1;
   __syncthreads();
   if (n == 1) return;
   for (int i = s0 + threadIdx.x; i < e0; i += kClimbThreads) {
@@ -1321,17 +1323,19 @@ int top_pass_grid(int n) { return std::max(1, std::min(cdiv(3L * cdiv(n, kClimbC
 // The top of the tree above the chunks: about log2(chunks) + 2 one-level
 // launches over the whole GPU, then one workgroup for whatever remains.
 template <typename K>
-void launch_climb_top(rt_bvh* b, const K* keys, hipStream_t s) {
+void launch_climb_top(rt_bvh* b, const K* keys, hipStream_t s, bool topology = true) {
   const int n = b->n;
   if (n <= 1) return;
+  int32_t *par = topology ? b->parent : nullptr, *lft = topology ? b->left : nullptr,
+          *rgt = topology ? b->right : nullptr;
   const int passes = top_passes(n);
   uint32_t* lists[2] = {b->deferred, b->deferred2};
   const int grid = top_pass_grid(n);
   for (int p = 0; p < passes; ++p)
-    k_climb_pass<K><<<grid, 256, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, tnode_out(b), b->deposits,
+    k_climb_pass<K><<<grid, 256, 0, s>>>(keys, n, par, lft, rgt, b->node_box, tnode_out(b), b->deposits,
                                          b->arrivals, lists[p & 1], lists[(p + 1) & 1], b->misc, hand_off_slot(p),
                                          hand_off_slot(p + 1), hand_off_slot(p + 2));
-  k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, b->parent, b->left, b->right, b->node_box, tnode_out(b),
+  k_climb_top<K><<<1, kClimbTopThreads, 0, s>>>(keys, n, par, lft, rgt, b->node_box, tnode_out(b),
                                                 b->deposits, b->arrivals, lists[passes & 1], b->misc,
                                                 hand_off_slot(passes));
 }
@@ -1382,17 +1386,18 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
     launch_wide(b, false, s);
     return hipGetLastError();
   }
-  // The keys are unchanged, so the climb re-derives the same topology and
-  // recomputes every box from the moved vertices.
+  // The keys are unchanged, so the climb re-derives the same topology (and
+  // leaves the parent and child links in place) and recomputes every box
+  // from the moved vertices.
   size_t zero_words = (size_t)(n > 1 ? n - 1 : 1);
   k_init<<<min(cdiv((long)zero_words, 256), 2048), 256, 0, s>>>(b->misc, b->arrivals, zero_words, 0);
   k_aabb_bounds<<<min(cdiv(n, 256), 1024), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box,
                                                         b->trirec, b->misc, 0);
   const K* keys = (const K*)b->keys[0];
   k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
-      keys, b->ids[0], n, b->trirec, b->parent, b->left, b->right, b->node_box, tnode_out(b), b->ltri, b->deposits,
+      keys, b->ids[0], n, b->trirec, nullptr, nullptr, nullptr, b->node_box, tnode_out(b), b->ltri, b->deposits,
       b->arrivals, b->deferred, b->misc, nullptr);
-  launch_climb_top<K>(b, keys, s);
+  launch_climb_top<K>(b, keys, s, false);
   launch_wide(b, false, s);
   return hipGetLastError();
 }
