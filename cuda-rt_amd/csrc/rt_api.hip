@@ -42,11 +42,11 @@ const rt_overlap* rt_ctx_overlap(rt_ctx* ctx) {
   ctx->overlap_ready = true;
   rt_overlap& o = ctx->overlap;
   bool ok = hipStreamCreateWithFlags(&o.aux, hipStreamNonBlocking) == hipSuccess;
-  for (hipEvent_t* e : {&o.start, &o.shaded, &o.flushed[0], &o.flushed[1], &o.done})
+  for (hipEvent_t* e : {&o.start, &o.filled, &o.shaded, &o.flushed[0], &o.flushed[1], &o.done})
     ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
-    for (hipEvent_t e : {o.start, o.shaded, o.flushed[0], o.flushed[1], o.done})
+    for (hipEvent_t e : {o.start, o.filled, o.shaded, o.flushed[0], o.flushed[1], o.done})
       if (e) (void)hipEventDestroy(e);
     if (o.aux) (void)hipStreamDestroy(o.aux);
     o = rt_overlap{};
@@ -85,7 +85,7 @@ void rt_ctx_destroy(rt_ctx* ctx) {
   for (hipEvent_t e : ctx->timer.pool) (void)hipEventDestroy(e);
   if (ctx->timer.dev_work) (void)hipFree(ctx->timer.dev_work);
   rt_overlap& o = ctx->overlap;
-  for (hipEvent_t e : {o.start, o.shaded, o.flushed[0], o.flushed[1], o.done})
+  for (hipEvent_t e : {o.start, o.filled, o.shaded, o.flushed[0], o.flushed[1], o.done})
     if (e) (void)hipEventDestroy(e);
   if (o.aux) (void)hipStreamDestroy(o.aux);
   delete ctx;

@@ -27,7 +27,7 @@ struct rt_kernel_timer {
 // last any-hit pass runs there while the next sample is traced.
 struct rt_overlap {
   hipStream_t aux = nullptr;
-  hipEvent_t start = nullptr, shaded = nullptr, flushed[2] = {nullptr, nullptr}, done = nullptr;
+  hipEvent_t start = nullptr, filled = nullptr, shaded = nullptr, flushed[2] = {nullptr, nullptr}, done = nullptr;
 };
 
 struct rt_ctx {

@@ -2242,11 +2242,24 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   const size_t cap = ntiles * kThreads;
   const int tiles = (int)ntiles;
   const bool work = cnt || T;  // keep the replicated counts
-  if (multi) {
-    k_shade_fill<true><<<4096, 256, 0, s>>>(W0.scenes, (int)nscenes, D1);
-  } else if (D1.M.lshade) {
-    k_shade_fill<false><<<std::min(cdiv(D1.B.n, 256), 4096), 256, 0, s>>>(W0.scenes, 1, D1);
+  // The last bounce's any-hit pass, shadow resolve and flush of sample k run
+  // on the auxiliary stream, overlapping sample k+1's camera, closest-hit
+  // and shading passes on s: they share no buffer but the replicated
+  // counters (atomics). Sample k+2 reuses sample k's parity buffers, so its
+  // camera pass waits for sample k's flush. The shading table is first read
+  // by sample 0's shading pass, so it is filled on the auxiliary stream while
+  // sample 0's camera and closest-hit passes run.
+  hipStream_t sa = ov ? ov->aux : s;
+  if (ov) {
+    RT_TRY(hipEventRecord(ov->start, s));
+    RT_TRY(hipStreamWaitEvent(sa, ov->start, 0));
   }
+  if (multi) {
+    k_shade_fill<true><<<4096, 256, 0, sa>>>(W0.scenes, (int)nscenes, D1);
+  } else if (D1.M.lshade) {
+    k_shade_fill<false><<<std::min(cdiv(D1.B.n, 256), 4096), 256, 0, sa>>>(W0.scenes, 1, D1);
+  }
+  if (ov) RT_TRY(hipEventRecord(ov->filled, sa));
   const int grid = persistent_grid(num_cus, t, (uint32_t)cap);
   uint32_t* ovf = overflow ? overflow : W0.counts + 8;
   if (multi)
@@ -2256,16 +2269,6 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W0.fetch[0], 2 * kFetchWords);
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[1].fetch[1], kFetchWords);
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[1].shist, Wp[1].sbins);
-  // The last bounce's any-hit pass, shadow resolve and flush of sample k run
-  // on the auxiliary stream, overlapping sample k+1's camera, closest-hit
-  // and shading passes on s: they share no buffer but the replicated
-  // counters (atomics). Sample k+2 reuses sample k's parity buffers, so its
-  // camera pass waits for sample k's flush.
-  hipStream_t sa = ov ? ov->aux : s;
-  if (ov) {
-    RT_TRY(hipEventRecord(ov->start, s));
-    RT_TRY(hipStreamWaitEvent(sa, ov->start, 0));
-  }
   for (int it = 0; it < max_spp; ++it) {
     const int par = it & 1;
     const WfState& W = Wp[par];
@@ -2280,6 +2283,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
       launch_persistent<false>(t, grid, s, W.views, nscenes > 1 ? W.rscene[cur] : nullptr, R,
                                bounce == 0 ? 0.0 : kBounceTmin, O, W.fetch[0], ovf, work ? W.rep + 3 : nullptr);
       if (T) RT_TRY(timer_mark(T, 0, false, s));
+      if (ov && it == 0 && bounce == 0) RT_TRY(hipStreamWaitEvent(s, ov->filled, 0));
       RT_WF(s, W, k_wf_shade, bounce, cur, it, cnt);
       // the queue traced above is consumed: clear its count and the fetch counters
       k_wf_reset<<<1, 256, 0, s>>>(W.counts, 1u << cur, W.fetch[0], kFetchWords);
