@@ -23,6 +23,8 @@ c_u64 = ctypes.POINTER(ctypes.c_uint64)
 c_i64 = ctypes.POINTER(ctypes.c_int64)
 
 RANDVEC_RTL, CLAMP = 1, 2
+# planted mistakes (tests only): light jitter (rand - 0.5) * radius; aperture 1.25 / fstop; lens radius without sqrt
+PLANT_LIGHT_OFFSET, PLANT_APERTURE, PLANT_LENS_RADIUS = 1 << 24, 1 << 25, 1 << 26
 
 
 class Mesh(ctypes.Structure):
@@ -79,6 +81,7 @@ def lib():
         L.orc_expand_bits10.restype = ctypes.c_uint32
         L.orc_expand_bits10.argtypes = [ctypes.c_uint32]
         L.orc_sincos.argtypes = [ctypes.c_double, c_d, c_d]
+        L.orc_cosine_dir.argtypes = [c_d, ctypes.c_double, ctypes.c_double, c_d]
         L.orc_box_hit.restype = ctypes.c_int32
         L.orc_intersect_triangle.restype = ctypes.c_int32
         # entry points with 64-bit sizes need explicit argtypes
@@ -291,6 +294,13 @@ def sincos(x):
     c = ctypes.c_double()
     lib().orc_sincos(x, ctypes.byref(s), ctypes.byref(c))
     return s.value, c.value
+
+
+def cosine_dir(n, u1, u2):
+    """Cosine-weighted bounce direction about the unit normal n (src/ray.cpp:5-32 frame)."""
+    d = np.empty(3)
+    lib().orc_cosine_dir(_p(np.ascontiguousarray(n, np.float64), c_d), float(u1), float(u2), _p(d, c_d))
+    return d
 
 
 def _cam(cam):

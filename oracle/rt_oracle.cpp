@@ -364,6 +364,10 @@ static V3 shade(const RenderCtx &c, V3 p, V3 d, const Hit &h, uint32_t &rng, int
   V = normalize(V);
   V3 off = scale(rand_vec(rng, (c.o->flags & ORC_RANDVEC_RTL) != 0), c.L->radius);
   off = v3(off.x - 0.5, off.y - 0.5, off.z - 0.5);
+  if (c.o->flags & ORC_PLANT_LIGHT_OFFSET) {  // planted mistake: (rand - 0.5) * radius
+    const double h = 0.5 * c.L->radius;
+    off = v3(off.x + 0.5 - h, off.y + 0.5 - h, off.z + 0.5 - h);
+  }
   V3 L = normalize(sub(add(arr3(c.L->position), off), q));
   bool clamped;
   V3 atten = scale(shadow_atten(c, q, L, cnt, occ), dist_atten(c, q, &clamped));
@@ -446,10 +450,13 @@ static V3 trace_path(const RenderCtx &c, V3 p, V3 d, uint32_t &rng, int64_t *cnt
 }
 
 // kernels/raytracer.cu:142-154: two draws, thin-lens origin, direction.
-static void lens_ray(const orc_camera *cam, double xx, double yy, uint32_t &rng, V3 *o_out, V3 *d_out) {
+static void lens_ray(const orc_camera *cam, int32_t flags, double xx, double yy, uint32_t &rng, V3 *o_out,
+                     V3 *d_out) {
   double aperture = 1 / cam->fstop;
+  if (flags & ORC_PLANT_APERTURE) aperture *= 1.25;  // planted mistake
   double ang = 2 * M_PI * rand_double(rng);
-  double rad = aperture * sqrt(rand_double(rng)) / 2.0;
+  double u = rand_double(rng);
+  double rad = (flags & ORC_PLANT_LENS_RADIUS) ? aperture * u / 2.0 : aperture * sqrt(u) / 2.0;
   double s, co;
   orc_math::sincos(ang, &s, &co);
   V3 o = v3(rad * co, rad * s, 0);
@@ -480,7 +487,7 @@ static void render_pixel(const RenderCtx &c, int px, int py, float *rgb, double 
   int N = c.o->spp;
   for (int it = 0; it < N; ++it) {
     V3 o, d;
-    lens_ray(cam, xx, yy, rng, &o, &d);
+    lens_ray(cam, c.o->flags, xx, yy, rng, &o, &d);
     ++cnt[0];
     const int64_t slot = ((int64_t)(py * W + px) * N + it) * (c.o->depth + 1);
     int32_t *r = rec ? rec + 2 * slot : nullptr;
@@ -831,9 +838,17 @@ void orc_primary_ray(const orc_camera *cam, const orc_render_opts *opts, int32_t
   double xx, yy;
   pixel_xy(cam, px, py, &xx, &yy);
   V3 o, d;
-  for (int k = 0; k <= sample; ++k) lens_ray(cam, xx, yy, rng, &o, &d);
+  for (int k = 0; k <= sample; ++k) lens_ray(cam, opts->flags, xx, yy, rng, &o, &d);
   orig[0] = o.x; orig[1] = o.y; orig[2] = o.z;
   dir[0] = d.x; dir[1] = d.y; dir[2] = d.z;
+}
+
+void orc_cosine_dir(const double *n, double u1, double u2, double *dir) {
+  int axis;
+  V3 d = cosine_dir(v3(n[0], n[1], n[2]), u1, u2, &axis);
+  dir[0] = d.x;
+  dir[1] = d.y;
+  dir[2] = d.z;
 }
 
 void orc_render(const orc_mesh *m, const orc_bvh *b, const orc_material *mats, int32_t nmat, const orc_light *light,

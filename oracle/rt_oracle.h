@@ -59,7 +59,12 @@ typedef struct {
 
 enum {
   ORC_RANDVEC_RTL = 1,          /* evaluate randVec3d's three draws z,y,x */
-  ORC_CLAMP = 2                 /* clamp the averaged colour to [0,1] */
+  ORC_CLAMP = 2,                /* clamp the averaged colour to [0,1] */
+  /* Planted mistakes, for tests that show a statistic can see them; never
+   * set outside tests. */
+  ORC_PLANT_LIGHT_OFFSET = 1 << 24, /* light jitter (rand - 0.5) * radius instead of rand * radius - 0.5 */
+  ORC_PLANT_APERTURE = 1 << 25,     /* lens aperture 1.25 / fstop */
+  ORC_PLANT_LENS_RADIUS = 1 << 26   /* lens radius aperture * u / 2 (no square root) */
 };
 
 typedef struct {
@@ -122,6 +127,9 @@ double orc_rand_double(uint32_t *state);
 /* Seed index of pixel (px,py): the 16x16-block linear id (inc/indexing.h:4-9). */
 uint32_t orc_seed_index(int32_t px, int32_t py, int32_t width);
 void orc_sincos(double x, double *s, double *c);
+/* Cosine-weighted direction about the unit normal n from two uniforms, in the
+ * frame of src/ray.cpp:5-32 (the diffuse-bounce extension). */
+void orc_cosine_dir(const double *n, double u1, double u2, double *dir);
 
 /* A15: primary ray of sample `sample` of launch pixel (px,py) with the
  * reference's draw order (kernels/raytracer.cu:126-154). */

@@ -75,13 +75,37 @@ def test_cli_logcabin_matches_oracle_and_reference_image(ctx, logcabin):
     assert d.max() <= 1, d.max()
     assert (d == 0).all(2).mean() >= 0.999
 
-    # against the reference's own image: the noise-free parts agree, the
-    # depth-of-field and soft-shadow edges of the cabin differ by a few
-    # samples (measured: 92.8 % of pixels exact, 97.8 % within 1 LSB)
+    # against the reference's own image. Its random stream is not reproduced
+    # (seed 0 matches the draw-sensitive pixels no better than any other
+    # seed, tests/test_logcabin_stats.py), so per pixel the image is checked
+    # as one more draw of our renderer: its deviation from the mean of an
+    # 8-seed ensemble of GPU renders equals that of held-out seeds.
+    dm = rtamd.DeviceMesh(hm)
+    bvh = rtamd.BVH(ctx, dm, 30)
+    mt = rtamd.materials_tensor(mats)
+
+    def gpu_q(seed):
+        o = rtamd.render_opts(20, depth=0, seed=seed, flags=rtamd.RT_RENDER_CLAMP | rtamd.RT_RENDER_WAVEFRONT)
+        rgb = rtamd.render(ctx, bvh, dm, mt, len(mats), light, cam, o)
+        torch.cuda.synchronize()
+        return (255.0 * np.clip(rgb.cpu().numpy().astype(np.float64), 0, 1)).astype(np.uint8).reshape(1024, 1024, 3)
+
+    ens = np.stack([gpu_q(s) for s in range(1, 9)]).astype(float)
+    live = (ens.max(0) != ens.min(0)).any(2)
+    mu = ens.mean(0)[live]
+    want = ref["bgr"][:, :, ::-1].astype(float)[live]
+    held = [gpu_q(s).astype(float)[live] for s in (101, 102)]
+    mad = np.abs(want - mu).mean() / np.mean([np.abs(h - mu).mean() for h in held])
+    mse = ((want - mu) ** 2).mean() / np.mean([((h - mu) ** 2).mean() for h in held])
+    _say(f"reference image vs GPU ensemble: live {live.mean():.4f} MAD ratio {mad:.4f} MSE ratio {mse:.4f}")
+    assert live.mean() > 0.05
+    assert 0.95 <= mad <= 1.05 and 0.92 <= mse <= 1.08, (mad, mse)
+    # where the draws do not move the pixel the images agree (measured on the
+    # oracle: 99.2 % exact, 99.99 % within 1 LSB)
     dr = np.abs(ours.astype(int) - ref["bgr"].astype(int)).max(2)
-    assert (dr <= 1).mean() >= 0.975, (dr <= 1).mean()
-    assert (dr == 0).mean() >= 0.92, (dr == 0).mean()
+    assert (dr[~live] == 0).mean() >= 0.985 and (dr[~live] <= 1).mean() >= 0.999
     assert abs(ours.astype(float).mean() - ref["bgr"].astype(float).mean()) < 0.05
+    bvh.close()
 
 
 @pytest.mark.parametrize("ss", [1, 2, 3])

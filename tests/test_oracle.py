@@ -66,6 +66,20 @@ def test_wang_hash_is_a_bijection_sample():
     assert len(vals) == 1 << 18
 
 
+def test_wang_hash_known_answers():
+    """Thomas Wang's 32-bit integer hash in the form kernels/mrand.cu:13-21 uses
+    (as published in N. Reed, "Quick and Easy GPU Random Numbers in D3D11",
+    2013): s = (s ^ 61) ^ (s >> 16); s *= 9; s ^= s >> 4; s *= 0x27d4eb2d;
+    s ^= s >> 15, all mod 2^32. Worked by hand for s = 0:
+    61 -> 549 -> 549 ^ 34 = 519 -> 519 * 668265261 mod 2^32 = 3232286779
+    -> 3232286779 ^ 98641 = 3232319850. The other values follow the same five
+    steps; a changed shift, multiplier or xor constant changes every one."""
+    known = {0: 3232319850, 1: 663891101, 2: 3329832309, 256: 2763059176,
+             1000000: 4032016030, 0xFFFFFFFF: 1895078355}
+    for s, want in known.items():
+        assert O.wang_hash(s) == want, s
+
+
 def test_seed_index_16x16_blocks():
     """inc/indexing.h:4-9 with blockDim (16,16), gridDim.x = W/16."""
     W = 64
@@ -337,6 +351,76 @@ def test_primary_ray_thin_lens_focus():
             w = f - o
             w /= np.linalg.norm(w)
             assert np.allclose(w, d, atol=3e-7)
+
+
+def test_primary_ray_lens_draw_order_known_answer():
+    """kernels/raytracer.cu:143-145: the FIRST draw of a sample is the angle
+    (theta = 2 pi u1), the SECOND the radius (rho = (1/fstop) sqrt(u2) / 2), and
+    the origin is (rho cos theta, rho sin theta, 0). The draws come from the
+    pixel's own stream: state wang(seed_index), then xorshift steps, u =
+    float(x) / 2^32. Swapping the two draws, dropping the square root or
+    using another aperture fails here."""
+    W, H, fstop = 48, 32, 2.5
+    cam = rtamd.camera(W, H, fov_deg=50.0, fstop=fstop, focal_point=2.0)
+    for (px, py) in [(0, 0), (17, 5), (47, 31), (30, 16)]:
+        st = O.wang_hash(O.seed_index(px, py, W))
+        for s in range(3):
+            u1, st = O.rand_double(st)
+            u2, st = O.rand_double(st)
+            o, _ = O.primary_ray(cam, px, py, s)
+            rho = (1.0 / fstop) * math.sqrt(u2) / 2.0
+            assert abs(math.hypot(o[0], o[1]) - rho) <= 1e-15
+            if rho > 1e-9:
+                ang = math.atan2(o[1], o[0]) % (2 * math.pi)
+                want = (2 * math.pi * u1) % (2 * math.pi)
+                assert min(abs(ang - want), 2 * math.pi - abs(ang - want)) <= 1e-9, (px, py, s)
+            assert o[2] == 0.0
+
+
+def test_cosine_dir_known_answers():
+    """The bounce frame of src/ray.cpp:5-32 worked by hand for n = +z: h = n with
+    its smallest-|.| component (x) set to 1 -> h = (1,0,1); x = h ^ n = (0,-1,0);
+    z = x ^ n = (-1,0,0); sin(theta) = sqrt(u1), cos(theta) = sqrt(1 - u1),
+    phi = 2 pi u2, dir = sin cos(phi) x + cos n + sin sin(phi) z."""
+    n = np.array([0.0, 0.0, 1.0])
+    assert np.allclose(O.cosine_dir(n, 0.0, 0.3), [0, 0, 1], atol=1e-15)
+    assert np.allclose(O.cosine_dir(n, 0.25, 0.0), [0, -0.5, math.sqrt(0.75)], atol=1e-15)
+    assert np.allclose(O.cosine_dir(n, 0.25, 0.25), [-0.5, 0, math.sqrt(0.75)], atol=1e-15)
+    assert np.allclose(O.cosine_dir(n, 1.0, 0.5), [0, 1, 0], atol=1e-15)
+    # n = +x: the smallest component is y (ties go to x, then y): h = (1,1,0),
+    # x = h ^ n = (0,0,-1), z = x ^ n = (0,-1,0)
+    n = np.array([1.0, 0.0, 0.0])
+    assert np.allclose(O.cosine_dir(n, 0.64, 0.0), [0.6, 0, -0.8], atol=1e-15)
+    assert np.allclose(O.cosine_dir(n, 0.64, 0.25), [0.6, -0.8, 0], atol=1e-15)
+
+
+def test_cosine_dir_moments():
+    """Cosine-weighted hemisphere sampling, p(w) = cos(theta) / pi, has
+    E[cos] = 2/3, E[cos^2] = 1/2, a uniform azimuth (E[d_t] = 0 for both
+    tangents, E[sin^2 phi] = 1/2), and unit directions in the normal's
+    hemisphere. Stratified midpoints over (u1, u2) on a 200 x 200 grid."""
+    g = (np.arange(200) + 0.5) / 200
+    rng = np.random.default_rng(3)
+    for n in (np.array([0.0, 0.0, 1.0]), np.array([0.3, -0.5, 0.81]), np.array([-1.0, 0.2, 0.1])):
+        n = n / np.linalg.norm(n)
+        t1 = np.cross(n, [0.0, 0.0, 1.0] if abs(n[2]) < 0.9 else [1.0, 0.0, 0.0])
+        t1 /= np.linalg.norm(t1)
+        t2 = np.cross(n, t1)
+        c, c2, a1, a2, a11 = 0.0, 0.0, 0.0, 0.0, 0.0
+        for u1 in g:
+            for u2 in g:
+                d = O.cosine_dir(n, u1 + rng.uniform(-1e-4, 1e-4), u2)
+                # normalize() divides by sqrtf (inc/vec.h:119): unit to fp32 precision
+                assert abs(np.linalg.norm(d) - 1) < 1e-6 and d @ n >= -1e-6
+                ct = d @ n
+                c += ct
+                c2 += ct * ct
+                a1 += d @ t1
+                a2 += d @ t2
+                a11 += (d @ t1) ** 2 / max(1e-300, 1 - ct * ct)
+        m = 200 * 200
+        assert abs(c / m - 2 / 3) < 2e-3 and abs(c2 / m - 0.5) < 2e-3
+        assert abs(a1 / m) < 2e-3 and abs(a2 / m) < 2e-3 and abs(a11 / m - 0.5) < 2e-3
 
 
 def test_render_empty_scene_is_background():
