@@ -3,6 +3,7 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <mutex>
 #include <new>
 
 #include "rt_internal.h"
@@ -33,6 +34,40 @@ size_t align_up(size_t v) { return (v + 255) & ~(size_t)255; }
 
 bool mesh_ok(const rt_mesh_view* m) {
   return m && m->vx_dev && m->vy_dev && m->vz_dev && m->idx_dev && m->n > 0 && m->nv > 0;
+}
+
+// Scratch of rt_render_noalloc: the wavefront workspace followed by the
+// shading table. Each GPU the library runs on holds its own copy (48 MiB)
+// until the process ends. Host code never passes the array's name to a
+// kernel: the name is a host-side stand-in, and the device address comes
+// from hipGetSymbolAddress, once per device (noalloc_scratch_address).
+constexpr size_t kNoAllocScratchBytes = size_t(48) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices];
+
+// The device address of `scratch` on `device`, looked up on the first call
+// for that device with the device current (the caller's current device is
+// restored).
+hipError_t noalloc_scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    int prev = -1;
+    hipError_t e = hipGetDevice(&prev);
+    if (e == hipSuccess) e = hipSetDevice(device);
+    if (e != hipSuccess) return e;
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (prev >= 0 && prev != device) (void)hipSetDevice(prev);
+    if (e != hipSuccess) return e;
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return hipSuccess;
 }
 
 }  // namespace
@@ -69,6 +104,11 @@ rt_status rt_ctx_create(int32_t device, rt_ctx** out) {
   c->device = device;
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  if (noalloc_scratch_address(device, &c->noalloc_scratch) != hipSuccess) {
+    (void)hipGetLastError();
+    delete c;
+    return RT_ERR_DEVICE;
+  }
   *out = c;
   return RT_OK;
 }
@@ -326,19 +366,27 @@ static rt_status grow(rt_ctx* ctx, void** buf, size_t* have, size_t need, hipStr
   return RT_OK;
 }
 
-rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_material* mats_dev,
-                    int32_t nmat, const rt_light* light, const rt_camera* cam, const rt_render_opts* opts,
-                    float* rgb_dev, uint64_t* counters_dev, void* stream) {
-  if (!ctx) return RT_ERR_INVALID;
+static rt_status render_check(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_material* mats_dev,
+                              int32_t nmat, const rt_light* light, const rt_camera* cam, const rt_render_opts* opts,
+                              float* rgb_dev, const char* who) {
   if (!bvh || !mesh_ok(mesh) || !mats_dev || nmat <= 0 || !light || !cam || !opts || !rgb_dev)
-    return rt_set_error(ctx, RT_ERR_INVALID, "null argument to rt_render");
+    return rt_set_error(ctx, RT_ERR_INVALID, "null argument to %s", who);
   if (!mesh->nx_dev || !mesh->ny_dev || !mesh->nz_dev)
-    return rt_set_error(ctx, RT_ERR_INVALID, "rt_render needs shading normals");
+    return rt_set_error(ctx, RT_ERR_INVALID, "%s needs shading normals", who);
   if (cam->width <= 0 || cam->height <= 0 || opts->spp <= 0 || opts->depth < 0)
     return rt_set_error(ctx, RT_ERR_INVALID, "bad image size, spp or depth");
   if (mesh->n != bvh->n) return rt_set_error(ctx, RT_ERR_INVALID, "mesh and BVH triangle counts differ");
   if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT | RT_RENDER_ONE_STREAM))
     return rt_set_error(ctx, RT_ERR_INVALID, "unknown render flags");
+  return RT_OK;
+}
+
+rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_material* mats_dev,
+                    int32_t nmat, const rt_light* light, const rt_camera* cam, const rt_render_opts* opts,
+                    float* rgb_dev, uint64_t* counters_dev, void* stream) {
+  if (!ctx) return RT_ERR_INVALID;
+  rt_status chk = render_check(ctx, bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, "rt_render");
+  if (chk != RT_OK) return chk;
   void* ws = nullptr;
   void* shade = nullptr;
   if (opts->flags & RT_RENDER_WAVEFRONT) {
@@ -362,6 +410,31 @@ rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, co
   return RT_OK;
 }
 
+size_t rt_render_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+rt_status rt_render_noalloc(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_material* mats_dev,
+                            int32_t nmat, const rt_light* light, const rt_camera* cam, const rt_render_opts* opts,
+                            float* rgb_dev, uint64_t* counters_dev, void* stream) {
+  if (!ctx) return RT_ERR_INVALID;
+  rt_status chk = render_check(ctx, bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, "rt_render_noalloc");
+  if (chk != RT_OK) return chk;
+  if (!ctx->noalloc_scratch) return rt_set_error(ctx, RT_ERR_DEVICE, "rt_render_noalloc: no scratch on this device");
+  const size_t ws_bytes = rt_align256(rt_wavefront_bytes(cam->width, cam->height));
+  const size_t need = ws_bytes + rt_shade_table_bytes((size_t)mesh->n);
+  if (need > kNoAllocScratchBytes)
+    return rt_set_error(ctx, RT_ERR_INVALID,
+                        "rt_render_noalloc: a %dx%d image of %d triangles needs %zu bytes of scratch, more than its "
+                        "%zu (use rt_render)",
+                        cam->width, cam->height, mesh->n, need, kNoAllocScratchBytes);
+  // The wavefront passes on the caller's stream: no auxiliary stream, events
+  // or kernel timers, which would be created on first use.
+  rt_render_opts o = *opts;
+  o.flags |= RT_RENDER_WAVEFRONT | RT_RENDER_ONE_STREAM;
+  char* ws = (char*)ctx->noalloc_scratch;
+  RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, &o, rgb_dev, counters_dev, ws, ws + ws_bytes,
+                               (hipStream_t)stream, nullptr, nullptr, nullptr, nullptr));
+  return RT_OK;
+}
 
 static rt_status diff_check(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const double* albedo,
                             double light_scale, const rt_light* light, const rt_camera* cam,

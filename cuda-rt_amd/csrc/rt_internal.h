@@ -43,6 +43,7 @@ struct rt_ctx {
   rt_kernel_timer timer;
   rt_overlap overlap;
   bool overlap_ready = false;
+  void* noalloc_scratch = nullptr;  // this device's copy of rt_render_noalloc's static scratch array
 };
 
 // The context's overlap stream and events, created on first use (nullptr if

@@ -71,7 +71,7 @@ EXPORTED = [
     "rt_version", "rt_ctx_create", "rt_ctx_destroy", "rt_last_error", "rt_bvh_build", "rt_bvh_rebuild",
     "rt_bvh_refit", "rt_bvh_export", "rt_bvh_scene_bounds", "rt_bvh_num_triangles", "rt_bvh_diagnostics", "rt_bvh_destroy",
     "rt_trace", "rt_render", "rt_gen_soup", "rt_gen_icosphere_counts", "rt_gen_icosphere", "rt_gen_jitter",
-    "rt_ctx_set_kernel_timing", "rt_ctx_kernel_stats",
+    "rt_ctx_set_kernel_timing", "rt_ctx_kernel_stats", "rt_render_noalloc", "rt_render_noalloc_max_bytes",
 ]
 
 
@@ -107,6 +107,8 @@ def lib():
                            ctypes.POINTER(Hits), vp]
     L.rt_render.argtypes = [vp, vp, ctypes.POINTER(MeshView), vp, ctypes.c_int32, ctypes.POINTER(Light),
                             ctypes.POINTER(Camera), ctypes.POINTER(RenderOpts), vp, vp, vp]
+    L.rt_render_noalloc.argtypes = L.rt_render.argtypes
+    L.rt_render_noalloc_max_bytes.restype = ctypes.c_size_t
     L.rt_gen_soup.argtypes = [ctypes.c_int32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double, c_d,
                               ctypes.c_double] + [c_f] * 6 + [c_i32]
     L.rt_gen_icosphere_counts.argtypes = [ctypes.c_int32, c_i32, c_i32]
@@ -407,6 +409,17 @@ def render(ctx, bvh, mesh, mats_dev, nmat, light_, cam, opts, rgb=None, counters
     ctx.check(ctx.L.rt_render(ctx.h, bvh.h, ctypes.byref(mv), mats_dev.data_ptr(), nmat, ctypes.byref(light_),
                               ctypes.byref(cam), ctypes.byref(opts), rgb.data_ptr(),
                               counters.data_ptr() if counters is not None else None, _stream_handle(stream)))
+    return rgb
+
+
+def render_noalloc(ctx, bvh, mesh, mats_dev, nmat, light_, cam, opts, rgb, counters=None, stream=None):
+    """rt_render_noalloc: the wavefront render on one stream with the library's
+    static scratch (no device allocation; rgb must be given)."""
+    mv = mesh.view()
+    ctx.check(ctx.L.rt_render_noalloc(ctx.h, bvh.h, ctypes.byref(mv), mats_dev.data_ptr(), nmat,
+                                      ctypes.byref(light_), ctypes.byref(cam), ctypes.byref(opts), rgb.data_ptr(),
+                                      counters.data_ptr() if counters is not None else None,
+                                      _stream_handle(stream)))
     return rgb
 
 

@@ -11,6 +11,7 @@
  */
 #ifndef RT_API_H
 #define RT_API_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -182,6 +183,23 @@ rt_status rt_trace(const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_rays *r
 rt_status rt_render(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_material *mats_dev,
                     int32_t nmat, const rt_light *light, const rt_camera *cam, const rt_render_opts *opts,
                     float *rgb_dev, uint64_t *counters_dev, void *stream);
+
+/* rt_render without allocation. No reference counterpart: it is for callers
+ * that cannot allocate device memory (code under hipGraph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). It renders exactly as
+ * rt_render with RT_RENDER_WAVEFRONT | RT_RENDER_ONE_STREAM (same image,
+ * same counters), but keeps the wavefront workspace and the shading table
+ * in a library-owned static device array of rt_render_noalloc_max_bytes()
+ * (48 MiB) per GPU, found when the context is created. It calls no
+ * hipMalloc/hipFree and creates no stream or event. A call whose workspace
+ * (about 540 B per pixel) plus shading table (64 B per triangle) exceeds
+ * the array returns RT_ERR_INVALID; 256 x 256 fits with up to 230,000
+ * triangles. Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (other streams, other contexts, other threads) race. */
+rt_status rt_render_noalloc(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_material *mats_dev,
+                            int32_t nmat, const rt_light *light, const rt_camera *cam, const rt_render_opts *opts,
+                            float *rgb_dev, uint64_t *counters_dev, void *stream);
+size_t rt_render_noalloc_max_bytes(void);
 
 /* Per-kernel timing for benchmarks (no reference counterpart). While
  * enabled, every closest-hit and any-hit traversal launch of rt_render and
