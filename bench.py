@@ -1,6 +1,6 @@
 """Benchmark: LBVH build + thin-lens render of a 1M-triangle random soup on one MI355X.
 
-Workload (one step, "soup1M"): 1,000,000-triangle random soup (centroids
+Headline step ("soup1M"): a 1,000,000-triangle random soup (centroids
 uniform in the unit cube, lognormal edges with median 0.002 and sigma 0.5,
 uniformly random orientation, fp32), 63-bit Morton LBVH build (triangle
 AABBs + scene bounds -> Morton keys -> onesweep radix sort -> fused
@@ -8,8 +8,12 @@ hierarchy/refit climb), then a 1920x1080 thin-lens depth-of-field render
 (aperture 0.05, focus distance 1.5, fov 40) at 16 spp with one shadow ray
 per hit. Inputs are resident in HBM before the timed region.
 
-Prints one JSON line. value = primary rays per second of the whole step
-(build + render), in Mrays/s.
+Prints one JSON line. value = primary (+ bounce) rays per second of the
+whole step (build + render), in Mrays/s. The default run then measures the
+rest of the metric on its own workloads and adds them as extra keys:
+refit-only frames of a 10M-triangle displaced sphere (refit Mtris/s), 64
+batched scenes against the per-scene launch loop (speed-up), and the
+differentiable render at 200K triangles (backward / forward time).
 """
 import argparse
 import json
@@ -31,7 +35,10 @@ OPS_PER_TRI_TEST = 42       # Moller-Trumbore, evaluated in FP64
 FP64_FACTOR = 2
 # The default traversal layout (4-wide nodes) tests child boxes in fp32.
 BOX_TEST_FACTOR = 1
-# Algorithmic HBM bytes per triangle of a 63-bit build.
+# Algorithmic HBM bytes per triangle of a 63-bit build: vertices read twice
+# (72 B), key + id written (12 B), 8 sort passes of 24 B + 8 B of histogram,
+# sorted keys read (4 B), topology written (16 B), refit gathers (40 B) and
+# 2N boxes written (48 B).
 BUILD_BYTES_PER_TRI_63 = 396
 
 
@@ -52,6 +59,8 @@ def parse():
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront",
                     help="render kernel structure (identical images)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="soup1M: skip the refit, batch and differentiable legs of the metric")
     ap.add_argument("--cpu-pixel-stride", type=int, default=1,
                     help="CPU baseline renders every k-th pixel (launch order)")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -77,6 +86,23 @@ def committed_traffic(kind, args):
             "write_bytes": int(k["write_bytes"]), "source": "profiles/traffic_soup1M.json"}
 
 
+def committed_build_traffic(name, tris):
+    """HBM bytes of one build or refit (the sum over its kernels), from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes in
+    profiles/traffic_build.json, scaled by the copy calibration recorded
+    there; None if that workload was not profiled at this size."""
+    path = os.path.join(ROOT, "profiles", "traffic_build.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    w = rec.get("workloads", {}).get(name)
+    if not w or w["tris"] != tris:
+        return None
+    return {"bytes": int(w["bytes"]), "fetch_bytes": int(w["fetch_bytes"]), "write_bytes": int(w["write_bytes"]),
+            "bytes_per_tri": round(w["bytes"] / tris, 1), "source": "profiles/traffic_build.json"}
+
+
 def calibrate_copy(mib):
     import torch
     a = torch.empty(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -88,45 +114,87 @@ def calibrate_copy(mib):
                       "copies": 3}))
 
 
-def cpu_baseline(hm, mats, light, cam, opts, stride, tris):
-    """Time the FP64 CPU oracle (a restatement of the reference algorithm) on a
-    bounded sample: the full 63-bit build, then every stride-th launch pixel."""
+def host_info():
+    """What the CPU baseline ran on: logical CPUs visible to this process and
+    the CPU model (from /proc/cpuinfo, as lscpu reports it)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"nproc": affinity or os.cpu_count(), "cpu_count": os.cpu_count(), "model": model}
+
+
+def cpu_baseline(hm, mats, light, cam, opts, tris, stride_all=4, stride_one=64):
+    """Time the FP64 CPU oracle (a restatement of the reference algorithm) on
+    bounded samples of the same workload: the full 63-bit build, then the
+    render of every stride-th launch pixel, once on all host threads (at
+    most 16, the box's CPU share) and once on one thread."""
     import numpy as np
     from oracle import oracle as O
-    threads = min(os.cpu_count() or 1, 16)
-    t0 = time.perf_counter()
-    b = O.build(hm, 63, nthreads=threads)
-    t_build = time.perf_counter() - t0
-    pixels = np.arange(0, cam.width * cam.height, stride, dtype=np.int64)
-    t0 = time.perf_counter()
-    _, counts = O.render(hm, b, mats, light, cam, opts, pixels=pixels, nthreads=threads)
-    t_render = time.perf_counter() - t0
+    info = host_info()
+    threads = max(1, min(info["nproc"] or 1, 16))
+
+    def run(nthreads, stride):
+        t0 = time.perf_counter()
+        b = O.build(hm, 63, nthreads=nthreads)
+        t_build = time.perf_counter() - t0
+        pixels = np.arange(0, cam.width * cam.height, stride, dtype=np.int64)
+        t0 = time.perf_counter()
+        _, counts = O.render(hm, b, mats, light, cam, opts, pixels=pixels, nthreads=nthreads)
+        t_render = time.perf_counter() - t0
+        return t_build, t_render, pixels.size, counts
+
+    tb, tr, npx, c = run(threads, stride_all)
+    tb1, tr1, npx1, c1 = run(1, stride_one)
     return {
-        "value": round(counts[0] / t_render / 1e6, 4),
+        "value": round((c[0] + c[2]) / tr / 1e6, 4),
         "unit": "Mrays/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle FP64 build of all {tris} tris ({t_build:.2f} s, "
-                  f"{tris / t_build / 1e6:.3f} Mtris/s) + render of "
-                  f"{'every pixel' if stride == 1 else f'every {stride}th pixel'} "
-                  f"({pixels.size} px x {opts.spp} spp = {int(counts[0])} primary, {int(counts[1])} shadow rays, "
-                  f"{t_render:.2f} s)",
-        "build_mtris_s": round(tris / t_build / 1e6, 4),
+        "sample": f"oracle FP64 build of all {tris} tris ({tb:.2f} s) + render of every {stride_all}th launch "
+                  f"pixel ({npx} px x {opts.spp} spp = {int(c[0])} primary, {int(c[1])} shadow rays, "
+                  f"{tr:.2f} s) on {threads} threads",
+        "build_mtris_s": round(tris / tb / 1e6, 4),
+        "single_thread": {
+            "value": round((c1[0] + c1[2]) / tr1 / 1e6, 4), "unit": "Mrays/s", "cores": 1,
+            "build_mtris_s": round(tris / tb1 / 1e6, 4),
+            "sample": f"build of all {tris} tris ({tb1:.2f} s) + render of every {stride_one}th launch pixel "
+                      f"({npx1} px, {int(c1[0])} primary rays, {tr1:.2f} s)"},
+        "host": info,
     }
 
 
-def bench_batch(args):
-    """64 random soups (log-uniform 1K..2M triangles), cameras of 256^2, 512^2
-    or 1024^2 at 1..8 spp. Step = rebuild every BVH + render every scene.
-    Compares the per-scene launch loop with the batched form replayed from a
-    hipGraph, and checks that the images are identical."""
-    import numpy as np
+def _timed(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def run_batch(ctx, steps, warmup, count=64):
+    """C4: 64 random soups (log-uniform 1K..2M triangles), cameras of 256^2,
+    512^2 or 1024^2 at 1..8 spp. Step = rebuild every BVH + render every
+    scene. The per-scene launch loop against the batched form (autotuned,
+    replayed from a hipGraph); the images must be identical."""
     import torch
 
     import rtamd
-    from tests import scenes
-    ctx = rtamd.Context(0)
-    specs = scenes.batch_scenes(args.scenes, 0x5EED0004)
+    from rtamd import workloads
+    specs = workloads.batch_scenes(count)
     sc = []
     for sp in specs:
         dm = rtamd.DeviceMesh(sp["hm"])
@@ -144,17 +212,7 @@ def bench_batch(args):
             s["bvh"].rebuild(s["mesh"])
             rtamd.render(ctx, s["bvh"], s["mesh"], s["mats"], s["nmat"], s["light"], s["cam"], s["opts"], rgb=out)
 
-    def timed(fn, steps, warmup):
-        for _ in range(warmup):
-            fn()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            fn()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3 / steps
-
-    loop_ms = timed(loop_step, args.steps, args.warmup)
+    loop_ms = _timed(loop_step, steps, warmup)
     batch = rtamd.Batch(ctx, sc)
     best, tuned_ms = batch.autotune()
     batch.rebuild()
@@ -168,17 +226,12 @@ def bench_batch(args):
             batch.rebuild()
             batch.render()
     torch.cuda.current_stream().wait_stream(side)
-    batch_ms = timed(g.replay, args.steps, args.warmup)
+    batch_ms = _timed(g.replay, steps, warmup)
     same = all(torch.equal(a, s["rgb"]) for a, s in zip(loop_rgb, sc))
-    out = {
-        "metric": "batched hipGraph step vs per-scene launch loop (64 scenes, build + render)",
-        "value": round(loop_ms / batch_ms, 3),
-        "unit": "x speed-up",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(batch_ms, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"batch{len(sc)}_soups_1K-2M_tris_256-1024px_1-8spp", "scenes": len(sc),
-                   "tris": tris, "pixels": pixels, "primary_rays": primary},
+    res = {
+        "workload": f"batch{len(sc)}_soups_1K-2M_tris_256-1024px_1-8spp", "scenes": len(sc), "tris": tris,
+        "pixels": pixels, "primary_rays": primary, "steps": steps, "warmup": warmup,
+        "speedup": round(loop_ms / batch_ms, 3),
         "loop_ms": round(loop_ms, 3), "batch_graph_ms": round(batch_ms, 3),
         "batch_mrays_s": round(primary / (batch_ms * 1e-3) / 1e6, 2),
         "loop_mrays_s": round(primary / (loop_ms * 1e-3) / 1e6, 2),
@@ -186,14 +239,20 @@ def bench_batch(args):
         "tuning": best.as_dict(), "autotune_best_render_ms": round(tuned_ms, 3),
         "stack_overflows": batch.diagnostics(),
     }
-    print(json.dumps(out), flush=True)
+    batch.close()
+    for s in sc:
+        s["bvh"].close()
+    return res
 
 
-REFIT_BYTES_PER_TRI = 152  # refit-only frame: see DESIGN.md
+# Algorithmic HBM bytes per triangle of a refit-only frame: 12 B indices +
+# 36 B gathered vertices + 4 B ids + 48 B box writes + 48 B child-box reads
+# + 4 B arrival flags.
+REFIT_BYTES_PER_TRI = 152
 
 
-def bench_sphere(args):
-    """Refit-only frames: a 9,996,980-triangle icosphere (frequency 707,
+def run_sphere(ctx, steps, warmup):
+    """C3: refit-only frames of a 9,996,980-triangle icosphere (frequency 707,
     radius 1, 3-octave value-noise displacement of amplitude 0.05), 30-bit
     build once; each frame moves every vertex by Gaussian jitter (sigma
     1e-4, precomputed frames resident in HBM), refits the BVH and renders
@@ -203,25 +262,22 @@ def bench_sphere(args):
     import torch
 
     import rtamd
-    ctx = rtamd.Context(0)
-    hm = rtamd.gen_icosphere(707, 1.0, (0.0, 0.0, -3.0), 0.05, 0x5EED0003)
+    from rtamd import workloads
+    hm = workloads.sphere10m_mesh()
     frames = []
     for k in range(3):
-        j = rtamd.gen_jitter(hm, 1e-4, 0x5EED1000 + k)
+        j = workloads.sphere10m_frame(hm, k)
         frames.append([torch.from_numpy(a).cuda() for a in (j.vx, j.vy, j.vz)])
     dm = rtamd.DeviceMesh(hm)
-    mats = [rtamd.material(kd=(0.8, 0.8, 0.8))]
+    mats, light, cam, opts = workloads.sphere10m_setup()
+    opts.flags |= rtamd.RT_RENDER_WAVEFRONT
     md = rtamd.materials_tensor(mats)
-    light = rtamd.light(position=(2.0, 4.0, 0.0))
-    cam = rtamd.camera(1024, 1024, fov_deg=45.0, fstop=float("inf"), focal_point=1.0)
-    opts = rtamd.render_opts(4, depth=4, flags=rtamd.RT_RENDER_CLAMP | rtamd.RT_RENDER_WAVEFRONT)
     bvh = rtamd.BVH(ctx, dm, 30)
-    rgb = torch.empty(1024 * 1024 * 3, dtype=torch.float32, device="cuda")
-    cnt = torch.zeros(5, dtype=torch.int64, device="cuda")
+    rgb = torch.empty(cam.width * cam.height * 3, dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
     state = {"k": 0}
 
-    def step(ev=None):
+    def step(ev=None, cnt=None):
         f = frames[state["k"] % len(frames)]
         state["k"] += 1
         dm.vx, dm.vy, dm.vz = f
@@ -230,49 +286,47 @@ def bench_sphere(args):
         bvh.refit(dm)
         if ev:
             ev[1].record(stream)
-        rtamd.render(ctx, bvh, dm, md, 1, light, cam, opts, rgb=rgb, counters=cnt)
+        rtamd.render(ctx, bvh, dm, md, len(mats), light, cam, opts, rgb=rgb, counters=cnt)
         if ev:
             ev[2].record(stream)
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    cnt.zero_()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         step(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     refit_ms = float(np.median([e[0].elapsed_time(e[1]) for e in evs]))
     render_ms = float(np.median([e[1].elapsed_time(e[2]) for e in evs]))
-    c = cnt.cpu().numpy() // args.steps
-    primary, shadow, bounce = int(c[0]), int(c[1]), int(c[2])
-    ms = wall * 1e3 / args.steps
+    # ray counts (exact) from one more, untimed frame
+    cnt = torch.zeros(5, dtype=torch.int64, device="cuda")
+    step(cnt=cnt)
+    primary, shadow, bounce = (int(x) for x in cnt.cpu().numpy()[:3])
+    ms = wall * 1e3 / steps
     refit_gbs = REFIT_BYTES_PER_TRI * hm.n / (refit_ms * 1e-3) / 1e9
-    out = {
-        "metric": "Mrays/s (primary+bounce), refit-only frames @10M tris",
-        "value": round((primary + bounce) / (ms * 1e-3) / 1e6, 2),
-        "unit": "Mrays/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+    res = {
+        "workload": "sphere10M_refit_1024x1024_4spp_depth4", "tris": hm.n, "steps": steps, "warmup": warmup,
         "ms_per_step": round(ms, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "sphere10M_refit_1024x1024_4spp_depth4", "tris": hm.n, "width": 1024, "height": 1024,
-                   "spp": 4, "depth": 4, "jitter_sigma": 1e-4},
+        "mrays_s": round((primary + bounce) / (ms * 1e-3) / 1e6, 2),
         "refit_ms": round(refit_ms, 4), "refit_mtris_s": round(hm.n / (refit_ms * 1e-3) / 1e6, 1),
-        "roofline_refit": {"bound": "hbm", "achieved": round(refit_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                           "frac": round(refit_gbs / HBM_PEAK_GBS, 4), "bytes_per_tri": REFIT_BYTES_PER_TRI},
+        "roofline_refit": {"kernels": "k_init..k_wide_fill (one refit)", "bound": "hbm",
+                           "achieved": round(refit_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": round(refit_gbs / HBM_PEAK_GBS, 4), "bytes_per_tri": REFIT_BYTES_PER_TRI,
+                           "traffic": committed_build_traffic("refit_sphere10M", hm.n)},
         "render_ms": round(render_ms, 3),
         "render_mrays_s": round((primary + bounce) / (render_ms * 1e-3) / 1e6, 2),
         "primary_rays": primary, "bounce_rays": bounce, "shadow_rays": shadow,
         "diagnostics": dict(zip(("stack_overflows", "build_errors"), bvh.diagnostics())),
-        "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    bvh.close()
+    return res
 
 
-def bench_diff(args):
-    """Differentiable render step on a 200K-triangle displaced icosphere
+def run_diff(ctx, steps, warmup):
+    """C5: differentiable render step on a 200K-triangle displaced icosphere
     (frequency 100, displacement 0.05): 512x512, 16 spp, 2 diffuse bounces,
     per-triangle albedo U[0.2, 0.9], light intensity 10. The target image is
     rendered at the true albedo; the parameters are the albedo perturbed by
@@ -282,18 +336,14 @@ def bench_diff(args):
     import torch
 
     import rtamd
-    from rtamd import diff
-    ctx = rtamd.Context(0)
-    hm = rtamd.gen_icosphere(100, 1.0, (0.0, 0.0, -3.0), 0.05, 0x5EED0006)
+    from rtamd import diff, workloads
+    hm = workloads.diff200k_mesh()
     dm = rtamd.DeviceMesh(hm)
-    light = rtamd.light(position=(2.0, 3.0, 0.0), color=(1.0, 1.0, 1.0), radius=0.5, c0=0.5, c1=0.2, c2=0.05)
-    cam = rtamd.camera(512, 512, fov_deg=45.0, fstop=float("inf"), focal_point=1.0)
+    light, cam = workloads.diff200k_setup()
     spp, depth = 16, 2
     R = diff.DiffRenderer(ctx, dm, light, cam, spp, depth, seed=1)
-    rng = np.random.default_rng(0x5EED0007)
-    true_alb = rng.uniform(0.2, 0.9, (hm.n, 3))
-    alb = torch.tensor(np.clip(true_alb + rng.normal(0, 0.05, true_alb.shape), 0.0, 1.0), dtype=torch.float64,
-                       device="cuda")
+    true_alb, pert_alb = workloads.diff200k_albedo(hm.n)
+    alb = torch.tensor(pert_alb, dtype=torch.float64, device="cuda")
     target = R.forward(torch.tensor(true_alb, dtype=torch.float64, device="cuda"), 10.0).clone()
     s = 10.0
 
@@ -308,35 +358,36 @@ def bench_diff(args):
             ev[2].record()
         return g
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     torch.cuda.synchronize()
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(steps)]
     t0 = time.perf_counter()
-    for k in range(args.steps):
+    for k in range(steps):
         g_alb, g_light, g_pos = step(evs[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     fwd_ms = float(np.median([e[0].elapsed_time(e[1]) for e in evs]))
     bwd_ms = float(np.median([e[1].elapsed_time(e[2]) for e in evs]))
-    ms = wall * 1e3 / args.steps
+    ms = wall * 1e3 / steps
     primary = cam.width * cam.height * spp
-    out = {
-        "metric": "differentiable render step (forward + adjoint backward)",
-        "value": round(primary / (ms * 1e-3) / 1e6, 2),
-        "unit": "Mpaths/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "diff_icosphere200K_512x512_16spp_depth2", "tris": hm.n, "width": 512,
-                   "height": 512, "spp": spp, "depth": depth},
+    return {
+        "workload": "diff_icosphere200K_512x512_16spp_depth2", "tris": hm.n, "steps": steps, "warmup": warmup,
+        "ms_per_step": round(ms, 3), "mpaths_s": round(primary / (ms * 1e-3) / 1e6, 2),
         "forward_ms": round(fwd_ms, 3), "backward_ms": round(bwd_ms, 3),
-        "backward_over_forward": round(bwd_ms / fwd_ms, 3),
+        "bwd_over_fwd": round(bwd_ms / fwd_ms, 3),
         "loss": float(((target - R.forward(alb, s)) ** 2).sum()),
         "grad_norms": {"albedo": float(g_alb.norm()), "light": float(g_light), "positions": float(g_pos.norm())},
         "record_mib": round(R.records.numel() * 4 / 2**20, 1),
-        "cpu_baseline": None,
     }
+
+
+def _standalone(args, metric, value_key, unit, res, workload):
+    out = {"metric": metric, "value": res[value_key], "unit": unit, "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": res.get("ms_per_step", res.get("batch_graph_ms")),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": workload}, "cpu_baseline": None}
+    out.update({k: v for k, v in res.items() if k not in out})
     print(json.dumps(out), flush=True)
 
 
@@ -348,35 +399,41 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
         return
-    if args.workload == "batch64":
-        bench_batch(args)
-        return
-    if args.workload == "diff200k":
-        bench_diff(args)
-        return
-    if args.workload == "sphere10M":
-        bench_sphere(args)
-        return
     import numpy as np
     import torch
 
     import rtamd
-    from tests import scenes
+    from rtamd import workloads
+
+    if args.workload == "batch64":
+        res = run_batch(rtamd.Context(0), args.steps, args.warmup, args.scenes)
+        _standalone(args, "batched hipGraph step vs per-scene launch loop (64 scenes, build + render)", "speedup",
+                    "x speed-up", res, res["workload"])
+        return
+    if args.workload == "diff200k":
+        res = run_diff(rtamd.Context(0), args.steps, args.warmup)
+        _standalone(args, "differentiable render step (forward + adjoint backward)", "mpaths_s", "Mpaths/s", res,
+                    res["workload"])
+        return
+    if args.workload == "sphere10M":
+        res = run_sphere(rtamd.Context(0), args.steps, args.warmup)
+        _standalone(args, "Mrays/s (primary+bounce), refit-only frames @10M tris", "mrays_s", "Mrays/s", res,
+                    res["workload"])
+        return
 
     dev = torch.device("cuda:0")
     ctx = rtamd.Context(0)
-    hm = scenes.soup_mesh(args.tris)
+    hm = workloads.soup_mesh(args.tris)
     dm = rtamd.DeviceMesh(hm)
-    mats, light, cam, opts = scenes.c2_setup(args.width, args.height, args.spp)
+    mats, light, cam, opts = workloads.c2_setup(args.width, args.height, args.spp)
     if args.mode == "wavefront":
         opts.flags |= rtamd.RT_RENDER_WAVEFRONT
     md = rtamd.materials_tensor(mats)
     stream = torch.cuda.current_stream()
     bvh = rtamd.BVH(ctx, dm, 63)
     rgb = torch.empty(args.width * args.height * 3, dtype=torch.float32, device=dev)
-    cnt = torch.zeros(5, dtype=torch.int64, device=dev)
 
-    def step(events=None):
+    def step(events=None, cnt=None):
         if events:
             events[0].record(stream)
         bvh.rebuild(dm)
@@ -389,10 +446,10 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    cnt.zero_()
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(args.steps)]
     # HIP events around every traversal launch, on the stream the library
-    # launches on (the dominant kernel's live duration for the roofline)
+    # launches on (the dominant kernel's live duration for the roofline).
+    # The timed steps count no work; the counts come from an untimed render.
     ctx.set_kernel_timing(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -401,8 +458,13 @@ def main():
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     kstats = ctx.kernel_stats()
-    # one more render, untimed, with every pass on one stream: each traversal
-    # launch then runs alone on the GPU (in the timed steps a sample's any-hit
+    # one more step, untimed, with the work counters: ray counts (exact) and
+    # the child-box and triangle tests of each traversal kernel
+    cnt = torch.zeros(5, dtype=torch.int64, device=dev)
+    step(cnt=cnt)
+    kwork = ctx.kernel_stats()
+    # and one render with every pass on one stream: each traversal launch
+    # then runs alone on the GPU (in the timed steps a sample's any-hit
     # launch overlaps the next sample's closest-hit launch, so their spans
     # include time shared with the other kernel)
     iso = None
@@ -414,30 +476,32 @@ def main():
     diag = bvh.diagnostics()
     build_ms = float(np.median([e[0].elapsed_time(e[1]) for e in evs]))
     render_ms = float(np.median([e[1].elapsed_time(e[2]) for e in evs]))
-    c = cnt.cpu().numpy() // args.steps
-    primary, shadow, bounce, box_tests, tests = (int(x) for x in c)
+    primary, shadow, bounce, box_tests, tests = (int(x) for x in cnt.cpu().numpy())
     ms_per_step = wall * 1e3 / args.steps
     value = (primary + bounce) / (ms_per_step * 1e-3) / 1e6
 
-    def kernel_roofline(name, ks):
-        # algorithmic lane-ops per launch / the launch's average duration
-        if ks["launches"] == 0:
+    def kernel_roofline(name, ks, work):
+        # algorithmic lane-ops per launch (from the counting render) / the
+        # launch's average duration (from the timed steps)
+        if ks["launches"] == 0 or work["launches"] == 0:
             return None
-        launches = ks["launches"]
-        avg_ms = ks["ms"] / launches
-        ops = (ks["box_tests"] * OPS_PER_BOX_TEST * BOX_TEST_FACTOR + ks["tri_tests"] * OPS_PER_TRI_TEST * FP64_FACTOR)
-        ach = ops / launches / (avg_ms * 1e-3) / 1e12
+        avg_ms = ks["ms"] / ks["launches"]
+        box = work["box_tests"] // work["launches"]
+        tri = work["tri_tests"] // work["launches"]
+        ops = box * OPS_PER_BOX_TEST * BOX_TEST_FACTOR + tri * OPS_PER_TRI_TEST * FP64_FACTOR
+        ach = ops / (avg_ms * 1e-3) / 1e12
         return {"kernel": name, "bound": "valu", "achieved": round(ach, 3), "peak": FP64_LANE_OPS_PEAK_T,
                 "unit": "T lane-op/s", "frac": round(ach / FP64_LANE_OPS_PEAK_T, 4),
                 "avg_launch_ms": round(avg_ms, 4), "launches": ks["launches"],
-                "work_per_launch": {"box_tests": ks["box_tests"] // launches, "tri_tests": ks["tri_tests"] // launches,
-                                    "lane_ops": ops // launches, "ops_per_box_test": OPS_PER_BOX_TEST * BOX_TEST_FACTOR,
-                                    "ops_per_tri_test": OPS_PER_TRI_TEST * FP64_FACTOR}}
+                "work_per_launch": {"box_tests": box, "tri_tests": tri, "lane_ops": ops,
+                                    "ops_per_box_test": OPS_PER_BOX_TEST * BOX_TEST_FACTOR,
+                                    "ops_per_tri_test": OPS_PER_TRI_TEST * FP64_FACTOR,
+                                    "counted": "child-box tests of filled node slots; triangle tests"}}
 
     roof = kernel_roofline("k_trace_persistent<closest hit, 4-wide quantized> (one launch = one sample's primary rays)",
-                           kstats["closest"])
+                           kstats["closest"], kwork["closest"])
     roof_any = kernel_roofline("k_trace_persistent<any hit, 4-wide quantized> (one launch = one sample's shadow rays)",
-                               kstats["any"])
+                               kstats["any"], kwork["any"])
     if roof is None:
         # megakernel: one k_render launch per step does all traversal and shading
         ops = box_tests * OPS_PER_BOX_TEST * BOX_TEST_FACTOR + tests * OPS_PER_TRI_TEST * FP64_FACTOR
@@ -452,12 +516,13 @@ def main():
                         "previous sample's any-hit launch; roofline_isolated times the same kernel alone")
     roof_iso = None
     if iso is not None:
-        roof_iso = {"closest": kernel_roofline("k_trace_persistent<closest hit> alone", iso["closest"]),
-                    "any": kernel_roofline("k_trace_persistent<any hit> alone", iso["any"])}
+        roof_iso = {"closest": kernel_roofline("k_trace_persistent<closest hit> alone", iso["closest"],
+                                               kwork["closest"]),
+                    "any": kernel_roofline("k_trace_persistent<any hit> alone", iso["any"], kwork["any"])}
     build_bytes = BUILD_BYTES_PER_TRI_63 * args.tris
     build_gbs = build_bytes / (build_ms * 1e-3) / 1e9
     out = {
-        "metric": "Mrays/s (primary+bounce) & LBVH build Mtris/s @1M tris",
+        "metric": "Mrays/s (primary+bounce) & LBVH build Mtris/s @1M tris; bwd/fwd time ratio",
         "value": round(value, 2),
         "unit": "Mrays/s",
         "n_gpus": 1,
@@ -478,13 +543,13 @@ def main():
         "roofline_isolated": roof_iso,
         "render_work": {"box_tests": box_tests, "tri_tests": tests},
         "roofline_build": {
-            "kernels": "k_init..k_climb_top (one rebuild)",
+            "kernels": "k_init..k_wide_fill (one rebuild)",
             "bound": "hbm",
             "achieved": round(build_gbs, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(build_gbs / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": committed_build_traffic("build_soup1M", args.tris),
             "bytes_per_tri": BUILD_BYTES_PER_TRI_63,
         },
         "build_ms": round(build_ms, 4),
@@ -497,8 +562,24 @@ def main():
         "diagnostics": {"stack_overflows": diag[0], "build_errors": diag[1]},
         "cpu_baseline": None,
     }
+    bvh.close()
+    if not args.headline_only:
+        # the rest of the metric, each on its own workload (fixed step counts)
+        print("bench: C3 refit-only frames, 10M triangles", file=sys.stderr, flush=True)
+        c3 = run_sphere(ctx, 5, 2)
+        print("bench: C4 batch of 64 scenes", file=sys.stderr, flush=True)
+        c4 = run_batch(ctx, 3, 1)
+        print("bench: C5 differentiable render, 200K triangles", file=sys.stderr, flush=True)
+        c5 = run_diff(ctx, 5, 2)
+        out["refit_mtris_s"] = c3["refit_mtris_s"]
+        out["batch_speedup"] = c4["speedup"]
+        out["bwd_over_fwd"] = c5["bwd_over_fwd"]
+        out["c3_refit"] = c3
+        out["c4_batch"] = c4
+        out["c5_diff"] = c5
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(hm, mats, light, cam, opts, args.cpu_pixel_stride, args.tris)
+        print("bench: CPU baseline (oracle)", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(hm, mats, light, cam, opts, args.tris)
         out["gpu_vs_cpu"] = round(out["render_mrays_s"] / max(out["cpu_baseline"]["value"], 1e-9), 1)
     print(json.dumps(out), flush=True)
 

@@ -803,7 +803,12 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
 #pragma unroll
           for (int j = 0; j < NW; ++j) cref[j] = __float_as_int(f4c(w[6], j & 3));
         }
-        st.visits += NW;
+        // child-box tests of the filled slots only (an empty slot's box is
+        // empty and never entered); counted only when a caller asked
+        if (stats) {
+#pragma unroll
+          for (int j = 0; j < NW; ++j) st.visits += cref[j] != kWideEmpty ? 1u : 0u;
+        }
         bool ovf = false;
         int next = kNoNode;
         if (!ANY) {
@@ -2241,7 +2246,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   unsigned long long* cnt = (unsigned long long*)counters;
   const size_t cap = ntiles * kThreads;
   const int tiles = (int)ntiles;
-  const bool work = cnt || T;  // keep the replicated counts
+  const bool work = cnt != nullptr;  // count work only when the caller asked for counters
   // The last bounce's any-hit pass, shadow resolve and flush of sample k run
   // on the auxiliary stream, overlapping sample k+1's camera, closest-hit
   // and shading passes on s: they share no buffer but the replicated

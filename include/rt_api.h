@@ -170,8 +170,8 @@ rt_status rt_trace(const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_rays *r
  * rgb_dev receives width*height*3 floats, row-flipped like the reference
  * image (kernels/raytracer.cu:106). counters_dev, if not NULL, receives
  * 5 uint64 counts added atomically: primary, shadow and bounce rays,
- * child-box slab tests (two per binary node entered, four per 4-wide node)
- * and triangle tests. The ray counts are exact; the two work counts can
+ * child-box slab tests (two per binary node entered, one per filled child
+ * slot of a wide node entered) and triangle tests. The ray counts are exact; the two work counts can
  * differ slightly between identical calls, because the 4-wide traversal
  * batches leaf tests across a wave and closest-hit pruning then depends on
  * which rays share a wave. Images and hits never depend on it.
@@ -204,7 +204,8 @@ size_t rt_render_noalloc_max_bytes(void);
 /* Per-kernel timing for benchmarks (no reference counterpart). While
  * enabled, every closest-hit and any-hit traversal launch of rt_render and
  * rt_batch_render on this context is bracketed by HIP events on its stream,
- * and the traversals' work counts are also kept per kernel.
+ * and, for renders given counters_dev, the traversals' work counts are also
+ * kept per kernel (a render without counters counts no work).
  * rt_ctx_kernel_stats synchronises the device, returns the totals since the
  * last call (out[0] closest hit, out[1] any hit) and resets them. */
 typedef struct {
