@@ -51,10 +51,12 @@ def parse():
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--spp", type=int, default=16)
-    ap.add_argument("--workload", choices=["soup1M", "batch64", "diff200k", "sphere10M"], default="soup1M",
+    ap.add_argument("--workload", choices=["soup1M", "batch64", "diff200k", "sphere10M", "build1M", "refit10M"],
+                    default="soup1M",
                     help="soup1M: the headline single-scene step; batch64: 64 scenes, batched vs per-scene loop; "
                          "diff200k: differentiable render forward + adjoint backward; "
-                         "sphere10M: refit-only frames of a 10M-triangle displaced sphere with diffuse bounces")
+                         "sphere10M: refit-only frames of a 10M-triangle displaced sphere with diffuse bounces; "
+                         "build1M / refit10M: the soup's 63-bit rebuild / the sphere's refit alone (for counters)")
     ap.add_argument("--scenes", type=int, default=64, help="batch64: number of scenes")
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront",
                     help="render kernel structure (identical images)")
@@ -382,6 +384,61 @@ def run_diff(ctx, steps, warmup):
     }
 
 
+def run_build_only(ctx, steps, warmup, refit):
+    """Steps of the build alone: a 63-bit rebuild of the 1M soup (refit =
+    False) or a refit of the 10M sphere on alternating jittered positions
+    (refit = True). Reports the median step time from HIP events."""
+    import numpy as np
+    import torch
+
+    import rtamd
+    from rtamd import workloads
+    if refit:
+        hm = workloads.sphere10m_mesh()
+        frames = [[torch.from_numpy(a).cuda() for a in (f.vx, f.vy, f.vz)]
+                  for f in (workloads.sphere10m_frame(hm, k) for k in range(2))]
+        bits, bytes_per_tri = 30, REFIT_BYTES_PER_TRI
+    else:
+        hm = workloads.soup_mesh(1_000_000)
+        frames = None
+        bits, bytes_per_tri = 63, BUILD_BYTES_PER_TRI_63
+    dm = rtamd.DeviceMesh(hm)
+    bvh = rtamd.BVH(ctx, dm, bits)
+    stream = torch.cuda.current_stream()
+    state = {"k": 0}
+
+    def step(ev=None):
+        if frames:
+            dm.vx, dm.vy, dm.vz = frames[state["k"] % 2]
+            state["k"] += 1
+        if ev:
+            ev[0].record(stream)
+        if refit:
+            bvh.refit(dm)
+        else:
+            bvh.rebuild(dm)
+        if ev:
+            ev[1].record(stream)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(steps)]
+    for k in range(steps):
+        step(evs[k])
+    torch.cuda.synchronize()
+    ms = float(np.median([e[0].elapsed_time(e[1]) for e in evs]))
+    gbs = bytes_per_tri * hm.n / (ms * 1e-3) / 1e9
+    name = "refit_sphere10M" if refit else "build_soup1M"
+    res = {"workload": name, "tris": hm.n, "ms_per_step": round(ms, 4), "mtris_s": round(hm.n / (ms * 1e-3) / 1e6, 1),
+           "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_tri": bytes_per_tri,
+                        "traffic": committed_build_traffic(name, hm.n)},
+           "diagnostics": dict(zip(("stack_overflows", "build_errors"), bvh.diagnostics()))}
+    bvh.close()
+    return res
+
+
 def _standalone(args, metric, value_key, unit, res, workload):
     out = {"metric": metric, "value": res[value_key], "unit": unit, "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": res.get("ms_per_step", res.get("batch_graph_ms")),
@@ -414,6 +471,12 @@ def main():
         res = run_diff(rtamd.Context(0), args.steps, args.warmup)
         _standalone(args, "differentiable render step (forward + adjoint backward)", "mpaths_s", "Mpaths/s", res,
                     res["workload"])
+        return
+    if args.workload in ("build1M", "refit10M"):
+        refit = args.workload == "refit10M"
+        res = run_build_only(rtamd.Context(0), args.steps, args.warmup, refit)
+        _standalone(args, "refit Mtris/s @10M tris" if refit else "LBVH build Mtris/s @1M tris", "mtris_s",
+                    "Mtris/s", res, res["workload"])
         return
     if args.workload == "sphere10M":
         res = run_sphere(rtamd.Context(0), args.steps, args.warmup)

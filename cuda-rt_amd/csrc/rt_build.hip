@@ -528,13 +528,16 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const float* __restrict__ 
 // on. The per-split arrival counter plays the role of the reference's
 // per-node atomicCAS flag (kernels/bvh.cu:228).
 //
-// One workgroup handles a chunk of kClimbChunk sorted leaves and counts the
-// arrivals at splits inside its chunk with LDS atomics (workgroup scope, no
-// cache maintenance). A node whose parent split leaves the chunk is handed
-// to k_climb_top, a single workgroup that finishes the top of the tree with
-// global counters, so no agent-scope release/acquire is needed anywhere.
-// Each arriving child first writes a 32-byte deposit (Karras index, far end
-// of its range, box) into deposits[2*split + side].
+// One workgroup handles a chunk of kClimbChunk sorted leaves. Each arriving
+// child first writes a 32-byte deposit (Karras index, far end of its range,
+// box) for its side of the split, then counts its arrival; for splits inside
+// the chunk both live in LDS (workgroup scope, no cache maintenance). A node
+// whose parent split lies on or past the chunk's edge is handed to the
+// top-level passes (k_climb_pass, then k_climb_top) with a global deposit,
+// and so is every in-chunk split still waiting for its sibling when the
+// chunk is done. Those passes count arrivals with global counters, each
+// reading only deposits written by earlier launches, so no agent-scope
+// release/acquire is needed anywhere.
 template <typename K>
 __device__ __forceinline__ int climb_delta(const K* __restrict__ keys, int n, int i, int j) {
   if (i < 0 || j >= n) return -1;
@@ -616,85 +619,153 @@ __device__ __forceinline__ int climb_step(ClimbNode& c, const K* __restrict__ ke
   return split;
 }
 
+// Leaf record and box of sorted leaf i: copied from the triangle-order
+// record (trirec, a build) or gathered through the indices (trirec ==
+// nullptr, a refit, whose vertices moved). The box is formed from the three
+// vertices exactly as k_aabb_bounds forms it.
+__device__ __forceinline__ void leaf_record(int i, int tri, const float4* __restrict__ trirec,
+                                            const float* __restrict__ vx, const float* __restrict__ vy,
+                                            const float* __restrict__ vz, const int32_t* __restrict__ idx,
+                                            float4 (&q)[3], float (&bb)[6]) {
+  if (trirec) {
+    const float4* R = trirec + 3 * (size_t)tri;
+    q[0] = R[0];
+    q[1] = R[1];
+    q[2] = R[2];
+  } else {
+    const int a = idx[3 * (size_t)tri], b = idx[3 * (size_t)tri + 1], c = idx[3 * (size_t)tri + 2];
+    q[0] = make_float4(vx[a], vy[a], vz[a], vx[b]);
+    q[1] = make_float4(vy[b], vz[b], vx[c], vy[c]);
+    q[2] = make_float4(vz[c], __int_as_float(tri), 0.f, 0.f);
+  }
+  const float ax = q[0].x, ay = q[0].y, az = q[0].z, bx = q[0].w, by = q[1].x, bz = q[1].y, cx = q[1].z,
+              cy = q[1].w, cz = q[2].x;
+  bb[0] = fminf(cx, fminf(ax, bx));
+  bb[1] = fminf(cy, fminf(ay, by));
+  bb[2] = fminf(cz, fminf(az, bz));
+  bb[3] = fmaxf(cx, fmaxf(ax, bx));
+  bb[4] = fmaxf(cy, fmaxf(ay, by));
+  bb[5] = fmaxf(cz, fmaxf(az, bz));
+  (void)i;
+}
+
+// The chunk kernel keeps everything its climbers exchange in LDS: the
+// chunk's keys (plus one on each side) for the split decisions, and for
+// every split inside the chunk an arrival count and one 32-byte deposit per
+// side. Only a split left with a single arrival when the chunk is done
+// publishes its deposit and count to global memory, for the top-level
+// passes; the rest of the climb touches global memory only for its outputs.
+template <typename K>
+struct ClimbLds {
+  K key[kClimbChunk + 2];            // keys[s0 - 1 .. e0]
+  uint32_t arr[kClimbChunk];         // split s0 + k: arrivals (low byte) + 0x100 per right-child arrival
+  float4 dep[4 * kClimbChunk];       // split s0 + k, side: dep[2 * (2k + side)], dep[2 * (2k + side) + 1]
+};
+
+template <typename K>
+__device__ __forceinline__ int lds_delta(const ClimbLds<K>& S, int s0, int n, int i, int j) {
+  if (i < 0 || j >= n) return -1;
+  return KeyTraits<K>::delta(S.key[i - s0 + 1], S.key[j - s0 + 1], i, j);
+}
+
 template <typename K>
 __device__ __forceinline__ void climb_chunks_body(uint32_t bid, uint32_t nblk, 
     const K* __restrict__ keys, const uint32_t* __restrict__ ids, int n, const float4* __restrict__ trirec,
-    int32_t* __restrict__ parent, int32_t* __restrict__ left,
+    const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
+    const int32_t* __restrict__ idx, int32_t* __restrict__ parent, int32_t* __restrict__ left,
     int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode, float4* __restrict__ ltri,
     float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, uint32_t* __restrict__ deferred,
     uint32_t* __restrict__ misc, int32_t* __restrict__ leaf_of) {
-  __shared__ uint32_t s_arr[kClimbChunk];
+  __shared__ ClimbLds<K> S;
   const int s0 = bid * kClimbChunk;
   const int e0 = min(n, s0 + kClimbChunk);
-  for (int k = threadIdx.x; k < kClimbChunk; k += kClimbThreads) s_arr[k] = 0u;
-  // Leaves: node n-1+i = the triangle's box, plus its vertex record (the
-  // box formed from the record's vertices exactly as k_aabb_bounds forms it).
-  for (int i = s0 + threadIdx.x; i < e0; i += kClimbThreads) {
-    int tri = (int)ids[i];
-    const float4* R = trirec + 3 * (size_t)tri;
-    const float4 q0 = R[0], q1 = R[1], q2 = R[2];
-    float4* L = ltri + kLeafStride * (size_t)i;
-    L[0] = q0;
-    L[1] = q1;
-    L[2] = q2;
-    const float ax = q0.x, ay = q0.y, az = q0.z, bx = q0.w, by = q1.x, bz = q1.y, cx = q1.z, cy = q1.w, cz = q2.x;
-    float* dst = node_box + 6 * (size_t)(n - 1 + i);
-    dst[0] = fminf(cx, fminf(ax, bx));
-    dst[1] = fminf(cy, fminf(ay, by));
-    dst[2] = fminf(cz, fminf(az, bz));
-    dst[3] = fmaxf(cx, fmaxf(ax, bx));
-    dst[4] = fmaxf(cy, fmaxf(ay, by));
-    dst[5] = fmaxf(cz, fmaxf(az, bz));
-    if (leaf_of) leaf_of[tri] = i;  // only a build reorders the leaves
+  for (int k = threadIdx.x; k < kClimbChunk; k += kClimbThreads) S.arr[k] = 0u;
+  for (int k = threadIdx.x; k < e0 - s0 + 2; k += kClimbThreads) {
+    const int g = s0 - 1 + k;
+    if (g >= 0 && g < n) S.key[k] = keys[g];
   }
   if (s0 == 0 && threadIdx.x == 0 && parent) parent[0] = -1;
   __syncthreads();
-  if (n == 1) return;
   for (int i = s0 + threadIdx.x; i < e0; i += kClimbThreads) {
+    // leaf n-1+i: its record in sorted order and its box
+    const int tri = (int)ids[i];
+    float4 q[3];
     ClimbNode c;
+    leaf_record(i, tri, trirec, vx, vy, vz, idx, q, c.b);
+    float4* L = ltri + kLeafStride * (size_t)i;
+    L[0] = q[0];
+    L[1] = q[1];
+    L[2] = q[2];
+    float* dst = node_box + 6 * (size_t)(n - 1 + i);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) dst[k] = c.b[k];
+    if (leaf_of) leaf_of[tri] = i;  // only a build reorders the leaves
+    if (n == 1) continue;
     c.l = c.r = i;
     c.idx = n - 1 + i;
     c.lidx = c.ridx = 0;
-    const float* src = node_box + 6 * (size_t)(n - 1 + i);  // written above by this thread
-#pragma unroll
-    for (int k = 0; k < 6; ++k) c.b[k] = src[k];
     for (int guard = 0;; ++guard) {
-      int side;
-      int split = climb_step<K>(c, keys, n, &side, parent, left, right, node_box, tnode, deposits);
-      if (split < 0 || guard >= n) {
+      const bool is_left = lds_delta(S, s0, n, c.r, c.r + 1) > lds_delta(S, s0, n, c.l - 1, c.l);
+      const int split = is_left ? c.r : c.l - 1;
+      if (split < 0 || split >= n - 1 || guard >= n) {  // cannot happen for a consistent climb
         atomicAdd(&misc[kMiscBuildError], 1u);
         break;
       }
+      if (c.idx < 0) {
+        c.idx = is_left ? c.r : c.l;
+        write_node(c, c.idx, n, parent, left, right, node_box, tnode);
+      }
+      const int side = is_left ? 0 : 1;
+      const float4 d0 = make_float4(__int_as_float(c.idx), __int_as_float(is_left ? c.l : c.r), c.b[0], c.b[1]);
+      const float4 d1 = make_float4(c.b[2], c.b[3], c.b[4], c.b[5]);
       if (split >= s0 && split + 1 < e0) {
+        const int k = split - s0;
+        S.dep[2 * (2 * k + side)] = d0;
+        S.dep[2 * (2 * k + side) + 1] = d1;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-        uint32_t old = atomicAdd(&s_arr[split - s0], 1u);
+        const uint32_t old = atomicAdd(&S.arr[k], 1u + ((uint32_t)side << 8));
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-        if (old == 0u) {
-          // The sibling may only arrive in k_climb_top: leave the count there.
-          arrivals[split] = 1u;
-          break;
-        }
-        merge_deposit(c, side == 0, deposits + 2 * (2 * (size_t)split + (side ^ 1)));
+        if ((old & 0xffu) == 0u) break;  // the sibling arrives later (here or, after publishing, above)
+        merge_deposit(c, side == 0, &S.dep[2 * (2 * k + (side ^ 1))]);
         if (c.l == 0 && c.r == n - 1) {
           write_node(c, 0, n, parent, left, right, node_box, tnode);
           break;
         }
       } else {
-        uint32_t slot = atomicAdd(&misc[kMiscDeferred], 1u);
+        // the split lies on or beyond the chunk's edge: hand off to the top-level passes
+        float4* d = deposits + 2 * (2 * (size_t)split + side);
+        d[0] = d0;
+        d[1] = d1;
+        const uint32_t slot = atomicAdd(&misc[kMiscDeferred], 1u);
         if (slot < (uint32_t)n) deferred[slot] = 2u * (uint32_t)split + (uint32_t)side;
         break;
       }
     }
   }
+  __syncthreads();
+  // splits still waiting for a sibling from outside the chunk: publish the
+  // arrival count and the deposit for the top-level passes
+  for (int k = threadIdx.x; k + 1 < e0 - s0; k += kClimbThreads) {
+    const uint32_t a = S.arr[k];
+    if ((a & 0xffu) != 1u) continue;
+    const int side = (int)(a >> 8);
+    const int split = s0 + k;
+    arrivals[split] = 1u;
+    float4* d = deposits + 2 * (2 * (size_t)split + side);
+    d[0] = S.dep[2 * (2 * k + side)];
+    d[1] = S.dep[2 * (2 * k + side) + 1];
+  }
 }
 template <typename K>
 __global__ __launch_bounds__(kClimbThreads) void k_climb_chunks(
     const K* __restrict__ keys, const uint32_t* __restrict__ ids, int n, const float4* __restrict__ trirec,
-    int32_t* __restrict__ parent, int32_t* __restrict__ left,
+    const float* __restrict__ vx, const float* __restrict__ vy, const float* __restrict__ vz,
+    const int32_t* __restrict__ idx, int32_t* __restrict__ parent, int32_t* __restrict__ left,
     int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode, float4* __restrict__ ltri,
     float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, uint32_t* __restrict__ deferred,
     uint32_t* __restrict__ misc, int32_t* __restrict__ leaf_of) {
-  climb_chunks_body(blockIdx.x, gridDim.x, keys, ids, n, trirec, parent, left, right, node_box, tnode, ltri, deposits, arrivals, deferred, misc, leaf_of);
+  climb_chunks_body(blockIdx.x, gridDim.x, keys, ids, n, trirec, vx, vy, vz, idx, parent, left, right, node_box,
+                    tnode, ltri, deposits, arrivals, deferred, misc, leaf_of);
 }
 
 // The climber of hand-off `code` (split * 2 + side) from its deposit.
@@ -806,7 +877,7 @@ template <typename K>
 __global__ __launch_bounds__(256) void k_export(int n, const K* __restrict__ keys, const uint32_t* __restrict__ ids,
                                                 const int32_t* __restrict__ parent, const int32_t* __restrict__ left,
                                                 const int32_t* __restrict__ right, const float* __restrict__ node_box,
-                                                const float* __restrict__ tri_box, K* codes_out, uint32_t* ids_out,
+                                                const float4* __restrict__ ltri, K* codes_out, uint32_t* ids_out,
                                                 int32_t* parent_out, int32_t* left_out, int32_t* right_out,
                                                 float* lo_out, float* hi_out, float* tlo_out, float* thi_out) {
   int i = blockIdx.x * 256 + threadIdx.x;
@@ -815,11 +886,22 @@ __global__ __launch_bounds__(256) void k_export(int n, const K* __restrict__ key
   if (i < n) {
     if (codes_out) codes_out[i] = keys[i];
     if (ids_out) ids_out[i] = ids[i];
-    if (tlo_out) {
-      for (int k = 0; k < 3; ++k) tlo_out[3 * i + k] = tri_box[6 * (size_t)i + k];
-    }
-    if (thi_out) {
-      for (int k = 0; k < 3; ++k) thi_out[3 * i + k] = tri_box[6 * (size_t)i + 3 + k];
+    if (tlo_out || thi_out) {
+      // the triangle of sorted leaf i: its box from the leaf record's
+      // vertices, formed as k_aabb_bounds forms it (kernels/scene.cu:110-120)
+      const float4* R = ltri + kLeafStride * (size_t)i;
+      const float4 q0 = R[0], q1 = R[1], q2 = R[2];
+      const size_t t = (size_t)__float_as_int(q2.y);
+      if (tlo_out) {
+        tlo_out[3 * t + 0] = fminf(q1.z, fminf(q0.x, q0.w));
+        tlo_out[3 * t + 1] = fminf(q1.w, fminf(q0.y, q1.x));
+        tlo_out[3 * t + 2] = fminf(q2.x, fminf(q0.z, q1.y));
+      }
+      if (thi_out) {
+        thi_out[3 * t + 0] = fmaxf(q1.z, fmaxf(q0.x, q0.w));
+        thi_out[3 * t + 1] = fmaxf(q1.w, fmaxf(q0.y, q1.x));
+        thi_out[3 * t + 2] = fmaxf(q2.x, fmaxf(q0.z, q1.y));
+      }
     }
   }
   if (parent_out) parent_out[i] = parent[i];
@@ -1218,8 +1300,9 @@ __global__ __launch_bounds__(kClimbThreads) void k_seg_climb_chunks(const SegBvh
                                                                     int ns) {
   const SegBlock b = seg_block(segs, off, ns);
   const SegBvh& g = *b.g;
-  climb_chunks_body<K>(b.bid, b.nblk, (const K*)g.keys[0], g.ids[0], g.n, g.trirec, g.parent, g.left, g.right,
-                       g.node_box, g.tnode, g.ltri, g.deposits, g.arrivals, g.deferred, g.misc, g.leaf_of);
+  climb_chunks_body<K>(b.bid, b.nblk, (const K*)g.keys[0], g.ids[0], g.n, g.trirec, g.vx, g.vy, g.vz, g.idx,
+                       g.parent, g.left, g.right, g.node_box, g.tnode, g.ltri, g.deposits, g.arrivals, g.deferred,
+                       g.misc, g.leaf_of);
 }
 
 template <typename K>
@@ -1367,8 +1450,8 @@ hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
       k_refit<<<cdiv(n, 256), 256, 0, s>>>(n, b->parent, b->left, b->right, b->flags, b->node_box, b->tnode);
   } else {
     k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
-        keys, b->ids[0], n, b->trirec, b->parent, b->left, b->right, b->node_box, tnode_out(b), b->ltri, b->deposits,
-        b->arrivals, b->deferred, b->misc, b->leaf_of);
+        keys, b->ids[0], n, b->trirec, m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, b->parent, b->left, b->right,
+        b->node_box, tnode_out(b), b->ltri, b->deposits, b->arrivals, b->deferred, b->misc, b->leaf_of);
     launch_climb_top<K>(b, keys, s);
   }
   launch_wide(b, true, s);
@@ -1391,12 +1474,12 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
   // from the moved vertices.
   size_t zero_words = (size_t)(n > 1 ? n - 1 : 1);
   k_init<<<min(cdiv((long)zero_words, 256), 2048), 256, 0, s>>>(b->misc, b->arrivals, zero_words, 0);
-  k_aabb_bounds<<<min(cdiv(n, 256), 1024), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box,
-                                                        b->trirec, b->misc, 0);
+  // leaf records and boxes come straight from the moved vertices (no
+  // triangle-order pass); the scene bounds of the build are kept
   const K* keys = (const K*)b->keys[0];
   k_climb_chunks<K><<<cdiv(n, kClimbChunk), kClimbThreads, 0, s>>>(
-      keys, b->ids[0], n, b->trirec, nullptr, nullptr, nullptr, b->node_box, tnode_out(b), b->ltri, b->deposits,
-      b->arrivals, b->deferred, b->misc, nullptr);
+      keys, b->ids[0], n, nullptr, m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, nullptr, nullptr, nullptr,
+      b->node_box, tnode_out(b), b->ltri, b->deposits, b->arrivals, b->deferred, b->misc, nullptr);
   launch_climb_top<K>(b, keys, s, false);
   launch_wide(b, false, s);
   return hipGetLastError();
@@ -1587,12 +1670,12 @@ hipError_t rt_launch_export(const rt_bvh* b, const rt_bvh_export_bufs* d, hipStr
   const int n = b->n;
   if (b->bits == 30)
     k_export<uint32_t><<<cdiv(2L * n - 1, 256), 256, 0, s>>>(
-        n, (const uint32_t*)b->keys[0], b->ids[0], b->parent, b->left, b->right, b->node_box, b->tri_box,
+        n, (const uint32_t*)b->keys[0], b->ids[0], b->parent, b->left, b->right, b->node_box, b->ltri,
         (uint32_t*)d->codes_dev, d->ids_dev, d->parent_dev, d->left_dev, d->right_dev, d->lo_dev, d->hi_dev,
         d->tri_lo_dev, d->tri_hi_dev);
   else
     k_export<uint64_t><<<cdiv(2L * n - 1, 256), 256, 0, s>>>(
-        n, (const uint64_t*)b->keys[0], b->ids[0], b->parent, b->left, b->right, b->node_box, b->tri_box,
+        n, (const uint64_t*)b->keys[0], b->ids[0], b->parent, b->left, b->right, b->node_box, b->ltri,
         (uint64_t*)d->codes_dev, d->ids_dev, d->parent_dev, d->left_dev, d->right_dev, d->lo_dev, d->hi_dev,
         d->tri_lo_dev, d->tri_hi_dev);
   return hipGetLastError();
