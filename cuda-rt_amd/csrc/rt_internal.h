@@ -169,13 +169,18 @@ constexpr int kMiscWords = 64;
 #endif
 constexpr int kLeafStride = RT_LEAF_STRIDE;
 
-// Fused hierarchy + refit climb: one workgroup per chunk of sorted leaves.
+// Fused hierarchy + refit climb: one workgroup per chunk of sorted leaves
+// (its LDS holds about 76 B per leaf of the chunk).
+#ifndef RT_CLIMB_CHUNK
+#define RT_CLIMB_CHUNK 512
+#endif
 #ifndef RT_CLIMB_THREADS
-#define RT_CLIMB_THREADS 1024
+#define RT_CLIMB_THREADS RT_CLIMB_CHUNK
 #endif
 constexpr int kClimbThreads = RT_CLIMB_THREADS;
-constexpr int kClimbLeavesPerThread = 1024 / RT_CLIMB_THREADS;
-constexpr int kClimbChunk = kClimbThreads * kClimbLeavesPerThread;
+constexpr int kClimbChunk = RT_CLIMB_CHUNK;
+constexpr int kClimbLeavesPerThread = kClimbChunk / kClimbThreads;
+static_assert(kClimbChunk % kClimbThreads == 0, "a climb chunk is a whole number of leaves per thread");
 constexpr int kClimbTopThreads = 1024;
 
 // Sort tiling.
