@@ -71,9 +71,9 @@ def test_batch_rebuild_matches_single_builds(ctx, bits):
     """rt_batch_rebuild builds every scene in shared launches (one per build
     stage): each BVH equals a single-scene build of the moved positions, bit
     for bit, and its image equals a single-scene render. Sizes cover one
-    triangle, a few, chunk edges (1024 leaves per climb chunk) and scenes
+    triangle, a few, chunk edges (512 leaves per climb chunk) and scenes
     that need several top-level passes."""
-    sizes = [1, 2, 3, 1023, 1025, 4097, 70_000, 250_000]
+    sizes = [1, 2, 3, 511, 513, 1025, 4097, 70_000, 250_000]
     print(f"segmented batch rebuild bits={bits} sizes={sizes}", flush=True)
     specs = []
     for k, n in enumerate(sizes):
@@ -100,8 +100,8 @@ def test_batch_rebuild_matches_single_builds(ctx, bits):
         single = _single(ctx, s)
         torch.cuda.synchronize()
         assert np.array_equal(s["rgb"].cpu().numpy(), single.cpu().numpy()), f"scene {k} image differs"
-    b0 = sc[5]["bvh"].export()
-    ref = O.build(moved[5], bits)
+    b0 = sc[6]["bvh"].export()
+    ref = O.build(moved[6], bits)
     assert np.array_equal(b0["parent"][1:], ref["parent"][1:]), "parent links differ from the oracle"
     for key in ("codes", "ids", "left", "right"):
         assert np.array_equal(b0[key], ref[key]), f"{key} differs from the oracle"

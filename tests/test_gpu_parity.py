@@ -59,7 +59,7 @@ def test_build_parity(ctx, bits, scene):
     assert np.array_equal(mn, ref["mn"]) and np.array_equal(mx, ref["mx"])
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 17, 1023, 1024, 1025, 4095, 4096, 4097, 70_000])
+@pytest.mark.parametrize("n", [1, 2, 3, 17, 511, 512, 513, 1023, 1024, 1025, 4095, 4096, 4097, 70_000])
 def test_build_sizes(ctx, n):
     _say(f"build sizes n={n}")
     hm = scenes.soup_mesh(n, seed=1234 + n)
