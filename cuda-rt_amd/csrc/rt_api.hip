@@ -208,7 +208,7 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   size_t o_nb = take(nn * 6 * 4), o_tn = take(ni * 64), o_lt = take((size_t)n * 16 * kLeafStride);
   size_t o_w = take(ni * 128), o_wq = take(ni * 64), o_w8 = take(ni * 128), o_wf = take(ni);
   size_t o_lo = take((size_t)n * 4), o_def2 = take((size_t)n * 4), o_tr = take((size_t)n * 48);
-  size_t o_wc = take(ni * 4), o_ws = take(((ni + 1023) / 1024 + 1) * 4);
+  size_t o_wc = take(ni * 4), o_ws = take(((ni + 1023) / 1024 + 1) * 4), o_wsrc = take(ni * 16);
   void* p = nullptr;
   hipError_t e = hipMalloc(&p, off);
   if (e != hipSuccess || !p) {
@@ -230,6 +230,7 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   b->trirec = (float4*)(a + o_tr);
   b->wcomp = (int32_t*)(a + o_wc);
   b->wscan = (uint32_t*)(a + o_ws);
+  b->wsrc = (int4*)(a + o_wsrc);
   b->fetch = (uint32_t*)(a + o_fetch);
   b->keys[0] = a + o_k0;
   b->keys[1] = a + o_k1;

@@ -514,6 +514,54 @@ __global__ __launch_bounds__(256) void k_refit_leaves(const float* __restrict__ 
   L[2] = make_float4(cz, __int_as_float(tri), 0.f, 0.f);
 }
 
+// The quantized planes of a 4-wide node (q[0..2]): corner = the node's box
+// minimum nb[0..2]; per axis the smallest power-of-two step with 255 steps
+// covering the box (stored as a float, 0 for a flat axis); per child slot j
+// < k the 8-bit planes rounded outward, so the decoded box contains the
+// child's box; empty slots decode as empty (lo 255 above hi 0).
+__device__ __forceinline__ void quantize4(const float (&nb)[6], const float (&lo)[3][4], const float (&hi)[3][4],
+                                          int k, float4* __restrict__ q) {
+  uint32_t ebits = 0, qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+  for (int a = 0; a < 3; ++a) {
+    const double o = nb[a], ext = (double)nb[3 + a] - o;
+    int e = 0;  // biased exponent of the step; 0 = zero extent
+    if (ext > 0) {
+      int ex2;
+      frexp(ext / 255.0, &ex2);  // ext/255 < 2^ex2
+      ex2 = ex2 < -126 ? -126 : ex2;
+      e = ex2 + 127;
+    }
+    ebits |= (uint32_t)e << (8 * a);
+    const double step = e ? ldexp(1.0, e - 127) : 0.0;
+    // step is a power of two (2^-126 at least, far below DBL_MAX), so
+    // x / step == x * inv exactly
+    const double inv = e ? ldexp(1.0, 127 - e) : 0.0;
+    for (int j = 0; j < 4; ++j) {
+      uint32_t ql = 255, qh = 0;  // empty slot: lo above hi
+      if (j < k) {
+        if (step == 0) {
+          ql = 0;
+          qh = 0;
+        } else {
+          double l = floor((lo[a][j] - o) * inv), h = ceil((hi[a][j] - o) * inv);
+          while (l > 0 && o + l * step > lo[a][j]) l -= 1;
+          while (h < 255 && o + h * step < hi[a][j]) h += 1;
+          ql = (uint32_t)(l < 0 ? 0 : (l > 255 ? 255 : l));
+          qh = (uint32_t)(h < 0 ? 0 : (h > 255 ? 255 : h));
+        }
+      }
+      qlo[a] |= ql << (8 * j);
+      qhi[a] |= qh << (8 * j);
+    }
+  }
+  // the steps as floats (2^(e-127), or 0), one in each spare word
+  q[0] = make_float4(nb[0], nb[1], nb[2], __uint_as_float((ebits & 0xffu) << 23));
+  q[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]),
+                     __uint_as_float(((ebits >> 8) & 0xffu) << 23));
+  q[2] = make_float4(__uint_as_float(qhi[0]), __uint_as_float(qhi[1]), __uint_as_float(qhi[2]),
+                     __uint_as_float(((ebits >> 16) & 0xffu) << 23));
+}
+
 // ------------------------------------------------------------ fused climb
 // Hierarchy and refit in one bottom-up pass (the agglomerative LBVH
 // construction of Apetrei, "Fast and Simple Agglomerative LBVH
@@ -831,7 +879,8 @@ __global__ __launch_bounds__(256) void k_climb_pass(
     int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode,
     float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, const uint32_t* __restrict__ in,
     uint32_t* __restrict__ out, uint32_t* __restrict__ misc, int in_slot, int out_slot, int clear_slot) {
-  climb_pass_body(blockIdx.x, gridDim.x, keys, n, parent, left, right, node_box, tnode, deposits, arrivals, in, out, misc, in_slot, out_slot, clear_slot);
+  climb_pass_body(blockIdx.x, gridDim.x, keys, n, parent, left, right, node_box, tnode, deposits, arrivals, in, out,
+                  misc, in_slot, out_slot, clear_slot);
 }
 
 template <typename K>
@@ -869,7 +918,8 @@ __global__ __launch_bounds__(kClimbTopThreads) void k_climb_top(
     int32_t* __restrict__ right, float* __restrict__ node_box, float4* __restrict__ tnode,
     float4* __restrict__ deposits, uint32_t* __restrict__ arrivals, const uint32_t* __restrict__ deferred,
     uint32_t* __restrict__ misc, int in_slot) {
-  climb_top_body(blockIdx.x, gridDim.x, keys, n, parent, left, right, node_box, tnode, deposits, arrivals, deferred, misc, in_slot);
+  climb_top_body(blockIdx.x, gridDim.x, keys, n, parent, left, right, node_box, tnode, deposits, arrivals, deferred,
+                 misc, in_slot);
 }
 
 // ------------------------------------------------------------ export
@@ -979,6 +1029,8 @@ __global__ __launch_bounds__(1024) void k_pack_count(int m, const uint8_t* __res
   pack_count_body(blockIdx.x, gridDim.x, m, flag, sums);
 }
 
+// Exclusive scan of the per-block counts in place; sums[blocks] receives the
+// total (the number of 4-wide nodes).
 __device__ __forceinline__ void pack_offsets_body(uint32_t bid, uint32_t nblk, int blocks, uint32_t* sums) {
   __shared__ uint32_t s_w[17];
   __shared__ uint32_t s_carry;
@@ -995,6 +1047,7 @@ __device__ __forceinline__ void pack_offsets_body(uint32_t bid, uint32_t nblk, i
     if (threadIdx.x == 0) s_carry = carry + total;
     __syncthreads();
   }
+  if (threadIdx.x == 0) sums[blocks] = s_carry;
 }
 __global__ __launch_bounds__(1024) void k_pack_offsets(int blocks, uint32_t* sums) {
   pack_offsets_body(blockIdx.x, gridDim.x, blocks, sums);
@@ -1015,12 +1068,15 @@ __global__ __launch_bounds__(1024) void k_pack_index(int m, const uint8_t* __res
 
 // Fill the 4-wide node of every even-depth internal node from the binary
 // topology and the refit boxes (node numbering of the reference: internal
-// [0, n-2], leaf of sorted position p at n-1+p).
+// [0, n-2], leaf of sorted position p at n-1+p). wsrc (if not null)
+// receives, per packed node, the binary nodes of its child slots (-1:
+// empty), which a refit reads instead of walking the topology again.
 __device__ __forceinline__ void wide_fill_body(uint32_t bid, uint32_t nblk, int n, const int32_t* __restrict__ left,
                                                    const int32_t* __restrict__ right,
                                                    const float* __restrict__ node_box,
                                                    const uint8_t* __restrict__ flag, float4* __restrict__ wide,
-                                                   float4* __restrict__ wideq, const int32_t* __restrict__ wcomp) {
+                                                   float4* __restrict__ wideq, const int32_t* __restrict__ wcomp,
+                                                   int4* __restrict__ wsrc) {
   const int i = bid * blockDim.x + threadIdx.x;
   if (i >= n - 1 || !(flag[i] & 1)) return;
   int slot[4];
@@ -1064,58 +1120,71 @@ __device__ __forceinline__ void wide_fill_body(uint32_t bid, uint32_t nblk, int 
     w[7] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   if (!wideq) return;
-  // quantized form: corner = the node's box minimum, per axis the smallest
-  // power-of-two step with 255 steps covering the box
-  const float* nb = node_box + 6 * (size_t)i;
-  uint32_t ebits = 0, qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
-  for (int a = 0; a < 3; ++a) {
-    const double o = nb[a], ext = (double)nb[3 + a] - o;
-    int e = 0;  // biased exponent of the step; 0 = zero extent
-    if (ext > 0) {
-      int ex2;
-      frexp(ext / 255.0, &ex2);  // ext/255 < 2^ex2
-      ex2 = ex2 < -126 ? -126 : ex2;
-      e = ex2 + 127;
-    }
-    ebits |= (uint32_t)e << (8 * a);
-    const double step = e ? ldexp(1.0, e - 127) : 0.0;
-    for (int j = 0; j < 4; ++j) {
-      uint32_t ql = 255, qh = 0;  // empty slot: lo above hi
-      if (j < k) {
-        if (step == 0) {
-          ql = 0;
-          qh = 0;
-        } else {
-          double l = floor((lo[a][j] - o) / step), h = ceil((hi[a][j] - o) / step);
-          while (l > 0 && o + l * step > lo[a][j]) l -= 1;
-          while (h < 255 && o + h * step < hi[a][j]) h += 1;
-          ql = (uint32_t)(l < 0 ? 0 : (l > 255 ? 255 : l));
-          qh = (uint32_t)(h < 0 ? 0 : (h > 255 ? 255 : h));
-        }
-      }
-      qlo[a] |= ql << (8 * j);
-      qhi[a] |= qh << (8 * j);
-    }
-  }
+  const float* nbp = node_box + 6 * (size_t)i;
+  const float nb[6] = {nbp[0], nbp[1], nbp[2], nbp[3], nbp[4], nbp[5]};
   // packed: this node's slot, and its internal children's slots as refs
-  float4* q = wideq + 4 * (size_t)wcomp[i];
-  // the steps as floats (2^(e-127), or 0), one in each spare word
-  q[0] = make_float4(nb[0], nb[1], nb[2], __uint_as_float((ebits & 0xffu) << 23));
-  q[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qlo[1]), __uint_as_float(qlo[2]),
-                     __uint_as_float(((ebits >> 8) & 0xffu) << 23));
-  q[2] = make_float4(__uint_as_float(qhi[0]), __uint_as_float(qhi[1]), __uint_as_float(qhi[2]),
-                     __uint_as_float(((ebits >> 16) & 0xffu) << 23));
+  const int w = wcomp[i];
+  float4* q = wideq + 4 * (size_t)w;
+  quantize4(nb, lo, hi, k, q);
   int cref[4];
   for (int j = 0; j < 4; ++j) cref[j] = ref[j] >= 0 && ref[j] != kWideEmpty ? wcomp[ref[j]] : ref[j];
   q[3] = make_float4(__int_as_float(cref[0]), __int_as_float(cref[1]), __int_as_float(cref[2]),
                      __int_as_float(cref[3]));
+  if (wsrc) wsrc[w] = make_int4(k > 0 ? slot[0] : -1, k > 1 ? slot[1] : -1, k > 2 ? slot[2] : -1, k > 3 ? slot[3] : -1);
 }
+
+// A refit's 4-wide nodes: the topology is unchanged, so each packed node
+// keeps its child refs (q[3]) and takes its planes from the child slots'
+// new boxes, found through wsrc (written by the build's wide fill). The
+// node's own box is the union of its child slots' boxes (fminf/fmaxf, exact
+// and order-independent), so no topology is read.
+__global__ __launch_bounds__(256) void k_wide_refit(int n, const uint32_t* __restrict__ wscan,
+                                                    const int4* __restrict__ wsrc,
+                                                    const float* __restrict__ node_box, float4* __restrict__ wideq) {
+  const uint32_t count = wscan[(n - 1 + 1023) / 1024];
+  const uint32_t w = blockIdx.x * 256 + threadIdx.x;
+  if (w >= count) return;
+  const int4 src = wsrc[w];
+  const int sl[4] = {src.x, src.y, src.z, src.w};
+  float lo[3][4], hi[3][4];
+  float nb[6] = {__int_as_float(0x7f800000), __int_as_float(0x7f800000), __int_as_float(0x7f800000),
+                 __int_as_float(0xff800000), __int_as_float(0xff800000), __int_as_float(0xff800000)};
+  int k = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    if (sl[j] >= 0) {
+      const float* b = node_box + 6 * (size_t)sl[j];
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        lo[a][j] = b[a];
+        hi[a][j] = b[3 + a];
+        nb[a] = fminf(nb[a], lo[a][j]);
+        nb[3 + a] = fmaxf(nb[3 + a], hi[a][j]);
+      }
+      ++k;
+    } else {
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        lo[a][j] = __int_as_float(0x7f800000);
+        hi[a][j] = __int_as_float(0xff800000);
+      }
+    }
+  }
+  float4 q[3];
+  quantize4(nb, lo, hi, k, q);
+  float4* dst = wideq + 4 * (size_t)w;
+  dst[0] = q[0];
+  dst[1] = q[1];
+  dst[2] = q[2];
+}
+
 __global__ __launch_bounds__(256) void k_wide_fill(int n, const int32_t* __restrict__ left,
                                                    const int32_t* __restrict__ right,
                                                    const float* __restrict__ node_box,
                                                    const uint8_t* __restrict__ flag, float4* __restrict__ wide,
-                                                   float4* __restrict__ wideq, const int32_t* __restrict__ wcomp) {
-  wide_fill_body(blockIdx.x, gridDim.x, n, left, right, node_box, flag, wide, wideq, wcomp);
+                                                   float4* __restrict__ wideq, const int32_t* __restrict__ wcomp,
+                                                   int4* __restrict__ wsrc) {
+  wide_fill_body(blockIdx.x, gridDim.x, n, left, right, node_box, flag, wide, wideq, wcomp, wsrc);
 }
 
 // Quantize up to W child boxes against the node box nb: per axis the
@@ -1137,6 +1206,7 @@ __device__ void quantize_children(const float* nb, int k, const float (&lo)[3][W
     }
     ebits |= (uint32_t)e << (8 * a);
     const double step = e ? ldexp(1.0, e - 127) : 0.0;
+    const double inv = e ? ldexp(1.0, 127 - e) : 0.0;  // exact: step is a power of two
     for (int w = 0; w < W / 4; ++w) qlo[a][w] = qhi[a][w] = 0;
     for (int j = 0; j < W; ++j) {
       uint32_t ql = 255, qh = 0;
@@ -1144,7 +1214,7 @@ __device__ void quantize_children(const float* nb, int k, const float (&lo)[3][W
         if (step == 0) {
           ql = qh = 0;
         } else {
-          double l = floor((lo[a][j] - o) / step), h = ceil((hi[a][j] - o) / step);
+          double l = floor((lo[a][j] - o) * inv), h = ceil((hi[a][j] - o) * inv);
           while (l > 0 && o + l * step > lo[a][j]) l -= 1;
           while (h < 255 && o + h * step < hi[a][j]) h += 1;
           ql = (uint32_t)(l < 0 ? 0 : (l > 255 ? 255 : l));
@@ -1237,6 +1307,7 @@ struct SegBvh {
   int32_t *parent, *left, *right, *leaf_of, *wcomp;
   float* node_box;
   float4 *tnode, *ltri, *deposits, *wideq;
+  int4* wsrc;
   uint8_t* wide_flag;
 };
 
@@ -1347,7 +1418,7 @@ __global__ __launch_bounds__(1024) void k_seg_pack_index(const SegBvh* segs, con
 __global__ __launch_bounds__(256) void k_seg_wide_fill(const SegBvh* segs, const uint32_t* off, int ns) {
   const SegBlock b = seg_block(segs, off, ns);
   const SegBvh& g = *b.g;
-  wide_fill_body(b.bid, b.nblk, g.n, g.left, g.right, g.node_box, g.wide_flag, nullptr, g.wideq, g.wcomp);
+  wide_fill_body(b.bid, b.nblk, g.n, g.left, g.right, g.node_box, g.wide_flag, nullptr, g.wideq, g.wcomp, g.wsrc);
 }
 
 // Binary traversal records of every internal node from the topology and
@@ -1373,6 +1444,9 @@ inline float4* tnode_out(const rt_bvh* b) { return b->want_tnode ? b->tnode : nu
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// The wide nodes after a build (topology_changed) or a refit. A refit keeps
+// the packed 4-wide nodes' refs and requantizes their planes (k_wide_refit)
+// unless the unquantized 4-wide nodes are wanted too.
 void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
   const int n = b->n;
   if (n < 2) return;
@@ -1383,8 +1457,11 @@ void launch_wide(rt_bvh* b, bool topology_changed, hipStream_t s) {
     k_pack_offsets<<<1, 1024, 0, s>>>(blocks, b->wscan);
     k_pack_index<<<blocks, 1024, 0, s>>>(m, b->wide_flag, b->wscan, b->wcomp);
   }
-  k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag,
-                                               b->want_wide4 ? b->wide : nullptr, b->wideq, b->wcomp);
+  if (!topology_changed && !b->want_wide4)
+    k_wide_refit<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->wscan, b->wsrc, b->node_box, b->wideq);
+  else
+    k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag,
+                                                 b->want_wide4 ? b->wide : nullptr, b->wideq, b->wcomp, b->wsrc);
   if (b->want_wide8)
     k_wide8_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);
 }
@@ -1496,7 +1573,7 @@ hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s) {
     b->want_wide4 = true;
     if (n >= 2)
       k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide, nullptr,
-                                                   b->wcomp);
+                                                   b->wcomp, nullptr);
   } else if (layout == 8 && !b->want_wide8) {
     b->want_wide8 = true;
     if (n >= 2)
@@ -1582,6 +1659,7 @@ hipError_t rt_seg_build_create(const rt_batch_item* items, int count, rt_seg_bui
     g.ltri = b->ltri;
     g.deposits = b->deposits;
     g.wideq = b->wideq;
+    g.wsrc = b->wsrc;
     g.wide_flag = b->wide_flag;
     // workgroups of each stage, as the single-BVH launches use them
     const int wide = n >= 2;

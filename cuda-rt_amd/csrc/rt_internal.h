@@ -135,6 +135,7 @@ struct rt_bvh {
   float4* wide = nullptr;        // (n-1)*8
   float4* wideq = nullptr;       // (n-1)*4 reserved; the packed nodes fill the front
   int32_t* wcomp = nullptr;      // n-1: packed index of each even-depth internal node
+  int4* wsrc = nullptr;          // per packed node: the binary nodes of its four child slots (-1: empty)
   uint32_t* wscan = nullptr;     // per-1024-node block sums of the packing scan
   float4* wide8 = nullptr;       // (n-1)*8
   uint8_t* wide_flag = nullptr;  // n-1: bit 0 = even depth (4-wide node), bit 1 = depth divisible by 3 (8-wide)

@@ -103,6 +103,38 @@ def test_refit_parity(ctx, flags):
         assert b.diagnostics() == (0, 0)
 
 
+@pytest.mark.parametrize("n", [300, 30_000])
+def test_refit_far_motion_traces(ctx, n):
+    """Refit after every triangle moved far (another soup with the same
+    triangle count): the tree keeps the old topology, so every box and every
+    quantized 4-wide node changes. Closest and any hits on the refit BVH
+    must equal the oracle's on the same topology with refit boxes; a wide
+    node the refit left stale would hide hits. n = 300 fits one climb chunk;
+    30,000 needs the top-level passes."""
+    _say(f"refit far motion n={n}")
+    hm = scenes.soup_mesh(n, seed=41)
+    dm = rtamd.DeviceMesh(hm)
+    b = rtamd.BVH(ctx, dm, 63)
+    ref = O.build(hm, 63)
+    moved = rtamd.gen_soup(n, 4242, edge_median=0.02 if n < 1000 else 0.004)
+    dm.set_positions(*[torch.from_numpy(a).cuda() for a in (moved.vx, moved.vy, moved.vz)])
+    b.refit(dm)
+    tlo, thi = O.tri_aabb(moved)
+    lo, hi = O.refit(ref["parent"], ref["left"], ref["right"], tlo, thi)
+    ref_moved = dict(ref, lo=lo, hi=hi)
+    got = b.export()
+    assert np.array_equal(got["lo"].view(np.uint32), lo.view(np.uint32))
+    orig, dirs = scenes.random_rays(6000, 8, origin=(0.0, 0.0, 0.0), spread=0.4)
+    o_t, d_t = torch.from_numpy(orig).cuda(), torch.from_numpy(dirs).cuda()
+    for any_hit in (False, True):
+        tri = b.trace(dm, o_t, d_t, rtamd.RT_TRACE_ANY if any_hit else rtamd.RT_TRACE_CLOSEST)[0]
+        torch.cuda.synchronize()
+        rtri = O.trace(moved, ref_moved, orig, dirs, any_hit=any_hit)[0]
+        assert np.array_equal(tri.cpu().numpy(), rtri), f"any={any_hit}: {np.sum(tri.cpu().numpy() != rtri)} rays"
+        assert (rtri >= 0).sum() > 100
+    assert b.diagnostics() == (0, 0)
+
+
 def test_megakernel_after_refit_and_rebuild(ctx):
     """The megakernel's binary node records are filled on demand: the first
     megakernel render after a refit fills them from the refit boxes, and from
