@@ -621,6 +621,7 @@ def main():
         "render_mrays_s": round((primary + bounce) / (render_ms * 1e-3) / 1e6, 2),
         "primary_rays": primary,
         "shadow_rays": shadow,
+        "shadow_rays_traced": kwork["any"]["rays"] if kwork["any"]["launches"] else None,
         "total_rays_incl_shadow_mrays_s": round((primary + bounce + shadow) / (render_ms * 1e-3) / 1e6, 2),
         "diagnostics": {"stack_overflows": diag[0], "build_errors": diag[1]},
         "cpu_baseline": None,

@@ -138,11 +138,11 @@ rt_status rt_ctx_set_kernel_timing(rt_ctx* ctx, int32_t enable) {
   if (!ctx) return RT_ERR_INVALID;
   if (enable && !ctx->timer.dev_work) {
     void* p = nullptr;
-    if (hipMalloc(&p, 4 * sizeof(unsigned long long)) != hipSuccess) {
+    if (hipMalloc(&p, 6 * sizeof(unsigned long long)) != hipSuccess) {
       (void)hipGetLastError();
       return rt_set_error(ctx, RT_ERR_OOM, "kernel timing counters");
     }
-    RT_HIP(ctx, hipMemset(p, 0, 4 * sizeof(unsigned long long)));
+    RT_HIP(ctx, hipMemset(p, 0, 6 * sizeof(unsigned long long)));
     ctx->timer.dev_work = (unsigned long long*)p;
   }
   ctx->timer.enabled = enable != 0;
@@ -152,7 +152,7 @@ rt_status rt_ctx_set_kernel_timing(rt_ctx* ctx, int32_t enable) {
 rt_status rt_ctx_kernel_stats(rt_ctx* ctx, rt_kernel_stat out[2]) {
   if (!ctx || !out) return RT_ERR_INVALID;
   RT_HIP(ctx, hipDeviceSynchronize());
-  for (int k = 0; k < 2; ++k) out[k] = rt_kernel_stat{0, 0.0, 0, 0};
+  for (int k = 0; k < 2; ++k) out[k] = rt_kernel_stat{0, 0.0, 0, 0, 0};
   rt_kernel_timer& T = ctx->timer;
   for (auto& r : T.recs) {
     float ms = 0.f;
@@ -164,13 +164,15 @@ rt_status rt_ctx_kernel_stats(rt_ctx* ctx, rt_kernel_stat out[2]) {
   }
   T.recs.clear();
   if (T.dev_work) {
-    unsigned long long w[4];
+    unsigned long long w[6];
     RT_HIP(ctx, hipMemcpy(w, T.dev_work, sizeof(w), hipMemcpyDeviceToHost));
     RT_HIP(ctx, hipMemset(T.dev_work, 0, sizeof(w)));
     out[0].box_tests = w[0];
     out[0].tri_tests = w[1];
     out[1].box_tests = w[2];
     out[1].tri_tests = w[3];
+    out[0].rays = w[4];
+    out[1].rays = w[5];
   }
   return RT_OK;
 }

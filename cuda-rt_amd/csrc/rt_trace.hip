@@ -1186,6 +1186,15 @@ __device__ __forceinline__ d3 shade_finish(const RenderParams& P, d3 I1, double 
   return I1 + rt::hadamard(rt::hadamard(sh * fo, ds), arr3(P.lcol));
 }
 
+// True when the shadow query cannot change shade_finish's result: in every
+// channel the unshadowed term ds (max(0, N.L) kd + specular ks) or the light
+// colour is zero, so (sh * fo) % ds % colour is zero for every shadow factor
+// sh (each is finite: 1, 0 or kd % (1 - kt); fo is at most 1). The image is
+// then I1 whatever the query returns, and the query is not made.
+__device__ __forceinline__ bool shadow_irrelevant(const RenderParams& P, d3 ds) {
+  return (ds.x == 0 || P.lcol[0] == 0) && (ds.y == 0 || P.lcol[1] == 0) && (ds.z == 0 || P.lcol[2] == 0);
+}
+
 __device__ __forceinline__ void pixel_of_tile(const RenderParams& P, int tile, int lane, int& px, int& py) {
   const int tiles_x = (P.W + 15) / 16;
   px = (tile % tiles_x) * 16 + (lane & 15);
@@ -1267,10 +1276,14 @@ __global__ __launch_bounds__(kThreads) void k_render(BvhView B, MeshShade M, con
         const int tri = leaf_tri(B, h.pos);
         const rt_material& m = mat_of(mats, P.nmat, M, tri);
         ShadePrep S = shade_prep(P, m, shading_normal(M, tri, h.al, h.be), o, d, h.t, rng);
-        HitRec s;
-        ++n_shadow;
-        bool sh_hit = any_hit(B, make_ray(S.q, S.L), s, overflow, st, col);
-        d3 I = shade_finish(P, S.I1, S.fo, S.ds, shadow_factor(P, B, mats, M, S.q, sh_hit, s));
+        ++n_shadow;  // the reference's shadow ray, traced only if it can change the image
+        d3 sh = mk(1, 1, 1);
+        if (!shadow_irrelevant(P, S.ds)) {
+          HitRec s;
+          const bool sh_hit = any_hit(B, make_ray(S.q, S.L), s, overflow, st, col);
+          sh = shadow_factor(P, B, mats, M, S.q, sh_hit, s);
+        }
+        d3 I = shade_finish(P, S.I1, S.fo, S.ds, sh);
         acc = acc + rt::hadamard(thr, I);
         if (bounce >= P.depth) break;
         double u1 = rt::rand_unit(rng);
@@ -1618,7 +1631,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
   uint32_t p = i < n_in ? W.pix[cur][i] : 0xffffffffu;
   const bool live = p != 0xffffffffu;
-  bool want_shadow = false, want_next = false;
+  bool want_shadow = false, want_next = false, dark = false;
   ShadePrep S;
   d3 thr = mk(0, 0, 0), no = mk(0, 0, 0), nd = mk(0, 0, 0);
   uint16_t sc = 0, skey = 0;
@@ -1640,7 +1653,14 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
       const LeafShade LS = leaf_shade(D.mats, D.P.nmat, D.M, D.B, pos);
       const rt_material& m = *LS.m;
       S = shade_prep(D.P, m, blend_normal(LS.na, LS.nb, LS.nc, hh[1], hh[2]), o, d, hh[0], rng);
-      want_shadow = true;
+      // The shadow ray is queued only if its answer can change the image
+      // (always for a path record, which names the occluder). Otherwise the
+      // shading result is added here, exactly as the finish pass would.
+      want_shadow = W.rec || !shadow_irrelevant(D.P, S.ds);
+      dark = !want_shadow;
+      if (dark)
+        stv(W.acc, p, W.npix,
+            ldv(W.acc, p, W.npix) + rt::hadamard(thr, shade_finish(D.P, S.I1, S.fo, S.ds, mk(1, 1, 1))));
       if (MULTI && kShadowSort)
         skey = (uint16_t)(((uint32_t)sc << W.scell_bits) |
                           (shadow_key(shadow_cells(D.P, D.B.node_box), S.q) >> (2 * kShadowBits - W.scell_bits)));
@@ -1679,8 +1699,14 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
     stv(W.ray[cur ^ 1], ns, W.cap, no);
     stv(W.ray[cur ^ 1] + 3 * W.cap, ns, W.cap, nd);
   }
-  // rays traced by the closest-hit pass of this bounce
-  if (counters) rep_count(W.rep, bounce == 0 ? 0 : 2, live);
+  // rays traced by the closest-hit pass of this bounce; the reference's
+  // shadow rays not traced (the finish pass counts the traced ones), and the
+  // traced ones for the any-hit kernel's statistics
+  if (counters) {
+    rep_count(W.rep, bounce == 0 ? 0 : 2, live);
+    rep_count(W.rep, 1, dark);
+    rep_count(W.rep, 7, want_shadow);
+  }
 }
 
 // Counting sort of a sample's shadow queue by cell key, in three passes:
@@ -1877,12 +1903,13 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 
 // Sum the replicas: counters[0..2] += the ray counts, counters[3..4] += the
 // work counts of both traversals; per_kernel[0..3] (if given) += the work
-// counts of the closest-hit and the any-hit traversal apart.
+// counts of the closest-hit and the any-hit traversal apart, per_kernel[4..5]
+// += the rays each traced.
 __global__ void k_wf_counters_out(const unsigned long long* rep, unsigned long long* counters,
                                   unsigned long long* per_kernel) {
   __shared__ unsigned long long s_sum[8];
   const int k = threadIdx.x;
-  if (k < 7) {
+  if (k < 8) {
     unsigned long long sum = 0;
     for (int r = 0; r < kCtrReplicas; ++r) sum += rep[r * kCtrStride + k];
     s_sum[k] = sum;
@@ -1894,8 +1921,11 @@ __global__ void k_wf_counters_out(const unsigned long long* rep, unsigned long l
     if (s_sum[3] + s_sum[5]) atomicAdd(&counters[3], s_sum[3] + s_sum[5]);
     if (s_sum[4] + s_sum[6]) atomicAdd(&counters[4], s_sum[4] + s_sum[6]);
   }
-  if (k == 1 && per_kernel)
+  if (k == 1 && per_kernel) {
     for (int j = 0; j < 4; ++j) per_kernel[j] += s_sum[3 + j];
+    per_kernel[4] += s_sum[0] + s_sum[2];
+    per_kernel[5] += s_sum[7];
+  }
 }
 
 // Reset the listed counters and, if fetch != nullptr, `words` fetch

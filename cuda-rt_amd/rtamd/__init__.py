@@ -77,7 +77,7 @@ EXPORTED = [
 
 class KernelStat(ctypes.Structure):
     _fields_ = [("launches", ctypes.c_uint64), ("ms", ctypes.c_double), ("box_tests", ctypes.c_uint64),
-                ("tri_tests", ctypes.c_uint64)]
+                ("tri_tests", ctypes.c_uint64), ("rays", ctypes.c_uint64)]
 
 _lib = None
 
@@ -293,7 +293,7 @@ class Context:
         out = (KernelStat * 2)()
         self.check(self.L.rt_ctx_kernel_stats(self.h, out))
         return {k: {"launches": int(o.launches), "ms": float(o.ms), "box_tests": int(o.box_tests),
-                    "tri_tests": int(o.tri_tests)} for k, o in zip(("closest", "any"), out)}
+                    "tri_tests": int(o.tri_tests), "rays": int(o.rays)} for k, o in zip(("closest", "any"), out)}
 
     def close(self):
         if self.h:

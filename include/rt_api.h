@@ -171,7 +171,11 @@ rt_status rt_trace(const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_rays *r
  * image (kernels/raytracer.cu:106). counters_dev, if not NULL, receives
  * 5 uint64 counts added atomically: primary, shadow and bounce rays,
  * child-box slab tests (two per binary node entered, one per filled child
- * slot of a wide node entered) and triangle tests. The ray counts are exact; the two work counts can
+ * slot of a wide node entered) and triangle tests. The ray counts are
+ * exact and count every shadow ray the reference casts (one per hit); a
+ * shadow ray is traced only when its answer can change the image, i.e.
+ * when some channel of (max(0, N.L) kd + specular ks) % light colour is
+ * non-zero (the image is identical either way). The two work counts can
  * differ slightly between identical calls, because the 4-wide traversal
  * batches leaf tests across a wave and closest-hit pruning then depends on
  * which rays share a wave. Images and hits never depend on it.
@@ -213,6 +217,7 @@ typedef struct {
   double ms;            /* summed launch durations */
   uint64_t box_tests;   /* child-box tests */
   uint64_t tri_tests;   /* triangle tests */
+  uint64_t rays;        /* rays traced (closest hit: primary + bounce; any hit: shadow rays queried) */
 } rt_kernel_stat;
 rt_status rt_ctx_set_kernel_timing(rt_ctx *ctx, int32_t enable);
 rt_status rt_ctx_kernel_stats(rt_ctx *ctx, rt_kernel_stat out[2]);

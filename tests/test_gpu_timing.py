@@ -44,6 +44,10 @@ def test_kernel_stats_count_launches_and_work(ctx, depth):
     assert ks["closest"]["box_tests"] + ks["any"]["box_tests"] == c[3]
     assert ks["closest"]["tri_tests"] + ks["any"]["tri_tests"] == c[4]
     assert ks["closest"]["box_tests"] > 0 and ks["any"]["box_tests"] > 0
+    # rays per kernel: every primary and bounce ray; the shadow rays whose
+    # answer can change the image (the counters count every one cast)
+    assert ks["closest"]["rays"] == c[0] + c[2]
+    assert 0 < ks["any"]["rays"] <= c[1]
     # the stats reset on reading, and nothing is recorded once disabled
     rtamd.render(ctx, bvh, dm, md, len(mats), light, cam, opts)
     ks2 = ctx.kernel_stats()
