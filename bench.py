@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--scenes", type=int, default=64, help="batch64: number of scenes")
     ap.add_argument("--mode", choices=["wavefront", "megakernel"], default="wavefront",
                     help="render kernel structure (identical images)")
+    ap.add_argument("--one-stream", action="store_true",
+                    help="soup1M: every render pass on one stream (for per-kernel profiles; same image)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--headline-only", action="store_true",
                     help="soup1M: skip the refit, batch and differentiable legs of the metric")
@@ -491,6 +493,8 @@ def main():
     mats, light, cam, opts = workloads.c2_setup(args.width, args.height, args.spp)
     if args.mode == "wavefront":
         opts.flags |= rtamd.RT_RENDER_WAVEFRONT
+    if args.one_stream:
+        opts.flags |= rtamd.RT_RENDER_ONE_STREAM
     md = rtamd.materials_tensor(mats)
     stream = torch.cuda.current_stream()
     bvh = rtamd.BVH(ctx, dm, 63)
