@@ -288,6 +288,7 @@ rt_status rt_bvh_rebuild(rt_bvh* bvh, const rt_mesh_view* mesh, void* stream) {
   if (!bvh) return RT_ERR_INVALID;
   if (!mesh_ok(mesh) || mesh->n != bvh->n)
     return rt_set_error(bvh->ctx, RT_ERR_INVALID, "rebuild needs a mesh with the same triangle count");
+  RT_HIP(bvh->ctx, rt_note_capture(bvh, (hipStream_t)stream));
   RT_HIP(bvh->ctx, rt_launch_build(bvh, mesh, (hipStream_t)stream));
   return RT_OK;
 }
@@ -296,6 +297,7 @@ rt_status rt_bvh_refit(rt_bvh* bvh, const rt_mesh_view* mesh, void* stream) {
   if (!bvh) return RT_ERR_INVALID;
   if (!mesh_ok(mesh) || mesh->n != bvh->n)
     return rt_set_error(bvh->ctx, RT_ERR_INVALID, "refit needs a mesh with the same triangle count");
+  RT_HIP(bvh->ctx, rt_note_capture(bvh, (hipStream_t)stream));
   RT_HIP(bvh->ctx, rt_launch_refit(bvh, mesh, (hipStream_t)stream));
   return RT_OK;
 }

@@ -138,6 +138,7 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
 rt_status rt_batch_rebuild(rt_batch* b, void* stream) {
   if (!b) return RT_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
+  for (int k : b->builds) RT_HIP(b->ctx, rt_note_capture(b->items[k].bvh, s));
   // Every stage in one launch over all scenes, unless a scene also keeps
   // the nodes of layout 2, 4 or 8 (the segmented build fills layout 5's).
   bool seg = b->seg != nullptr;

@@ -1564,17 +1564,28 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
 
 }  // namespace
 
+hipError_t rt_note_capture(const rt_bvh* b, hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(s, &st);
+  if (e != hipSuccess || st != hipStreamCaptureStatusActive) return e;
+  b->cap_tnode = (b->captured ? b->cap_tnode : true) && b->want_tnode;
+  b->cap_wide4 = (b->captured ? b->cap_wide4 : true) && b->want_wide4;
+  b->cap_wide8 = (b->captured ? b->cap_wide8 : true) && b->want_wide8;
+  b->captured = true;
+  return hipSuccess;
+}
+
 hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s) {
   const int n = b->n;
-  if (layout == 2 && !b->want_tnode) {
+  if (layout == 2 && (!b->want_tnode || (b->captured && !b->cap_tnode))) {
     b->want_tnode = true;
     if (n >= 2) k_tnode_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->tnode);
-  } else if (layout == 4 && !b->want_wide4) {
+  } else if (layout == 4 && (!b->want_wide4 || (b->captured && !b->cap_wide4))) {
     b->want_wide4 = true;
     if (n >= 2)
       k_wide_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide, nullptr,
                                                    b->wcomp, nullptr);
-  } else if (layout == 8 && !b->want_wide8) {
+  } else if (layout == 8 && (!b->want_wide8 || (b->captured && !b->cap_wide8))) {
     b->want_wide8 = true;
     if (n >= 2)
       k_wide8_fill<<<cdiv(n - 1, 256), 256, 0, s>>>(n, b->left, b->right, b->node_box, b->wide_flag, b->wide8);

@@ -146,6 +146,11 @@ struct rt_bvh {
   mutable bool want_tnode = false;  // binary records (layout 2, the megakernel); the search build always writes them
   mutable bool want_wide4 = false;
   mutable bool want_wide8 = false;
+  // A rebuild or refit recorded in a hipGraph writes the layouts wanted when
+  // it was recorded (cap_*, the intersection over every recording); a layout
+  // wanted only later is refilled before every traversal that reads it, since
+  // replays of that graph leave it stale.
+  mutable bool captured = false, cap_tnode = true, cap_wide4 = true, cap_wide8 = true;
 
   BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide, wideq, wide8, leaf_of}; }
 };
@@ -196,6 +201,9 @@ hipError_t rt_launch_build(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
 hipError_t rt_launch_refit(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
 // Before the first traversal of a BVH with node layout 4 or 8: fill those
 // nodes now and keep them filled by every later build and refit.
+// Record that a rebuild or refit of b is being launched on s; if s is
+// capturing a hipGraph, the layouts it writes are noted (see rt_bvh::captured).
+hipError_t rt_note_capture(const rt_bvh* b, hipStream_t s);
 hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s);
 // Builds and refits stop filling the binary records and the unquantized
 // 4-wide and 8-wide nodes unless `layout` traverses them; rt_ensure_layout

@@ -1539,7 +1539,9 @@ __device__ __forceinline__ bool tile_pixel(const WfState& W, const SceneDesc& D1
 template <bool MULTI>
 __global__ __launch_bounds__(kThreads) void k_wf_init(WfState W, SceneDesc D1) {
   if (blockIdx.x == 0) {
-    if (threadIdx.x < 16) W.counts[threadIdx.x] = 0u;
+    // the queue counts; counts[8], a batch's stack-overflow count, keeps
+    // accumulating over the batch's lifetime
+    if (threadIdx.x < 16 && threadIdx.x != 8) W.counts[threadIdx.x] = 0u;
     for (int k = threadIdx.x; k < kCtrReplicas * kCtrStride; k += kThreads) W.rep[k] = 0ull;
     for (uint32_t k = threadIdx.x; k < W.sbins; k += kThreads) W.shist[k] = 0u;
   }

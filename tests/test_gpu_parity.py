@@ -164,6 +164,43 @@ def test_megakernel_after_refit_and_rebuild(ctx):
     assert b.diagnostics() == (0, 0)
 
 
+def test_captured_rebuild_then_megakernel_layout(ctx):
+    """A rebuild captured in a hipGraph before any megakernel render: the
+    binary records the megakernel reads are first requested after the
+    capture, so the graph does not write them. Every later megakernel render
+    of that BVH must still see the replayed geometry (the library refills
+    the records before each such render), for two replays with moved
+    vertices, against the oracle and the wavefront image."""
+    _say("captured rebuild, then megakernel renders across replays")
+    hm = scenes.soup_mesh(20_000, seed=17)
+    mats, light, cam, opts = scenes.c2_setup(64, 48, 2)
+    dm = rtamd.DeviceMesh(hm)
+    b = rtamd.BVH(ctx, dm, 63)
+    md = rtamd.materials_tensor(mats)
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g, stream=side):
+            b.rebuild(dm)
+    torch.cuda.current_stream().wait_stream(side)
+    prev = None
+    for frame in range(3):
+        moved = rtamd.gen_jitter(hm, 0.02, 900 + frame)  # keeps the normals; moves triangles across boxes
+        dm.set_positions(*[torch.from_numpy(a).cuda() for a in (moved.vx, moved.vy, moved.vz)])
+        g.replay()
+        torch.cuda.synchronize()
+        mega = rtamd.render(ctx, b, dm, md, len(mats), light, cam, opts).cpu().numpy()
+        o = rtamd.render_opts(opts.spp, opts.depth, opts.seed, opts.flags | rtamd.RT_RENDER_WAVEFRONT)
+        wave = rtamd.render(ctx, b, dm, md, len(mats), light, cam, o).cpu().numpy()
+        ref, _ = O.render(moved, O.build(moved, 63), mats, light, cam, opts)
+        assert np.abs(mega - ref).max() <= 1e-4, f"frame {frame}: megakernel image is stale"
+        assert np.array_equal(mega, wave), f"frame {frame}"
+        assert prev is None or not np.array_equal(mega, prev)
+        prev = mega
+    assert b.diagnostics() == (0, 0)
+
+
 def test_build_duplicate_keys(ctx):
     """Every triangle in a tiny region: 30-bit codes collide, the index fallback orders the tree."""
     _say("build duplicate keys")
