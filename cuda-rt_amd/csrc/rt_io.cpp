@@ -6,6 +6,34 @@
 // (:312-316), face triples (:523-554), the fan triangulation and face
 // grouping of exportFaceGroupToShape (:626-687), and the MTL keys and
 // defaults of LoadMtl / InitMaterial (:589-615, :689-1000).
+//
+// parse_number, fix_index and parse_corner below follow tinyobjloader's
+// tryParseDouble, fixIndex and parseTriple step for step (the same
+// arithmetic and delimiters, so that vertex and material values come out
+// bit-identical), and carry its licence:
+//
+//   The MIT License (MIT)
+//
+//   Copyright (c) 2012-2016 Syoyo Fujita and many contributors.
+//
+//   Permission is hereby granted, free of charge, to any person obtaining a
+//   copy of this software and associated documentation files (the
+//   "Software"), to deal in the Software without restriction, including
+//   without limitation the rights to use, copy, modify, merge, publish,
+//   distribute, sublicense, and/or sell copies of the Software, and to
+//   permit persons to whom the Software is furnished to do so, subject to
+//   the following conditions:
+//
+//   The above copyright notice and this permission notice shall be included
+//   in all copies or substantial portions of the Software.
+//
+//   THE SOFTWARE IS PROVIDED "AS IS", WITHOUT WARRANTY OF ANY KIND, EXPRESS
+//   OR IMPLIED, INCLUDING BUT NOT LIMITED TO THE WARRANTIES OF
+//   MERCHANTABILITY, FITNESS FOR A PARTICULAR PURPOSE AND NONINFRINGEMENT.
+//   IN NO EVENT SHALL THE AUTHORS OR COPYRIGHT HOLDERS BE LIABLE FOR ANY
+//   CLAIM, DAMAGES OR OTHER LIABILITY, WHETHER IN AN ACTION OF CONTRACT,
+//   TORT OR OTHERWISE, ARISING FROM, OUT OF OR IN CONNECTION WITH THE
+//   SOFTWARE OR THE USE OR OTHER DEALINGS IN THE SOFTWARE.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
