@@ -90,6 +90,12 @@ const rt_overlap* rt_ctx_overlap(rt_ctx* ctx) {
   return &o;
 }
 
+rt_status rt_bvh_require_built(const rt_bvh* bvh, const char* who) {
+  if (bvh->stage == kStageBuilt) return RT_OK;
+  return rt_set_error(bvh->ctx, RT_ERR_INVALID, "%s: the BVH is not built (run rt_bvh_build_tree or rt_bvh_rebuild)",
+                      who);
+}
+
 extern "C" {
 
 const char* rt_version(void) { return "rtamd 0.1 (gfx950)"; }
@@ -261,28 +267,97 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   return RT_OK;
 }
 
+static rt_status check_bits(rt_ctx* ctx, const rt_build_opts* opts, int* bits, int* flags);
+
 rt_status rt_bvh_build(rt_ctx* ctx, const rt_mesh_view* mesh, const rt_build_opts* opts, void* stream,
                        rt_bvh** out) {
   if (!ctx || !out) return rt_set_error(ctx, RT_ERR_INVALID, "null context or output");
   *out = nullptr;
   if (!mesh_ok(mesh)) return rt_set_error(ctx, RT_ERR_INVALID, "mesh must have n > 0 triangles and positions");
-  int bits = opts ? opts->morton_bits : 30;
-  if (bits != 30 && bits != 63) return rt_set_error(ctx, RT_ERR_INVALID, "morton_bits must be 30 or 63");
+  int bits = 30, flags = 0;
+  rt_status st = check_bits(ctx, opts, &bits, &flags);
+  if (st != RT_OK) return st;
   if (mesh->n >= (1 << 30)) return rt_set_error(ctx, RT_ERR_INVALID, "n must be < 2^30");
   RT_HIP(ctx, hipSetDevice(ctx->device));
   rt_bvh* b = nullptr;
-  int flags = opts ? opts->flags : 0;
-  if (flags & ~RT_BUILD_KARRAS_SEARCH) return rt_set_error(ctx, RT_ERR_INVALID, "unknown build flags");
-  rt_status st = alloc_bvh(ctx, mesh->n, bits, flags, &b);
+  st = alloc_bvh(ctx, mesh->n, bits, flags, &b);
   if (st != RT_OK) return st;
   hipError_t e = rt_launch_build(b, mesh, (hipStream_t)stream);
   if (e != hipSuccess) {
     rt_bvh_destroy(b);
     return rt_set_error(ctx, RT_ERR_DEVICE, "build launch failed: %s", hipGetErrorString(e));
   }
+  b->stage = kStageBuilt;
   *out = b;
   return RT_OK;
 }
+
+static rt_status check_bits(rt_ctx* ctx, const rt_build_opts* opts, int* bits, int* flags) {
+  *bits = opts ? opts->morton_bits : 30;
+  *flags = opts ? opts->flags : 0;
+  if (*bits != 30 && *bits != 63) return rt_set_error(ctx, RT_ERR_INVALID, "morton_bits must be 30 or 63");
+  if (*flags & ~RT_BUILD_KARRAS_SEARCH) return rt_set_error(ctx, RT_ERR_INVALID, "unknown build flags");
+  return RT_OK;
+}
+
+rt_status rt_bvh_create(rt_ctx* ctx, int32_t n, const rt_build_opts* opts, rt_bvh** out) {
+  if (!ctx || !out) return rt_set_error(ctx, RT_ERR_INVALID, "null context or output");
+  *out = nullptr;
+  if (n <= 0 || n >= (1 << 30)) return rt_set_error(ctx, RT_ERR_INVALID, "n must be in [1, 2^30)");
+  int bits = 30, flags = 0;
+  rt_status st = check_bits(ctx, opts, &bits, &flags);
+  if (st != RT_OK) return st;
+  RT_HIP(ctx, hipSetDevice(ctx->device));
+  return alloc_bvh(ctx, n, bits, flags, out);
+}
+
+static rt_status stage_check(rt_bvh* bvh, int32_t need_min, int32_t need_max, const char* who, const char* before) {
+  if (bvh->stage < need_min || bvh->stage > need_max)
+    return rt_set_error(bvh->ctx, RT_ERR_INVALID, "%s: call %s first", who, before);
+  return RT_OK;
+}
+
+rt_status rt_bvh_compute_bounds(rt_bvh* bvh, const rt_mesh_view* mesh, float* tri_lo_dev, float* tri_hi_dev,
+                                void* stream) {
+  if (!bvh) return RT_ERR_INVALID;
+  if (!mesh_ok(mesh) || mesh->n != bvh->n)
+    return rt_set_error(bvh->ctx, RT_ERR_INVALID, "rt_bvh_compute_bounds needs a mesh of the BVH's triangle count");
+  RT_HIP(bvh->ctx, rt_launch_stage_bounds(bvh, mesh, tri_lo_dev, tri_hi_dev, (hipStream_t)stream));
+  bvh->stage = kStageBounds;
+  return RT_OK;
+}
+
+rt_status rt_bvh_compute_morton(rt_bvh* bvh, const double mn[3], const double mx[3], void* stream) {
+  if (!bvh) return RT_ERR_INVALID;
+  if ((mn == nullptr) != (mx == nullptr))
+    return rt_set_error(bvh->ctx, RT_ERR_INVALID, "rt_bvh_compute_morton: give both bounds or neither");
+  rt_status st = stage_check(bvh, kStageBounds, kStageBuilt, "rt_bvh_compute_morton", "rt_bvh_compute_bounds");
+  if (st != RT_OK) return st;
+  RT_HIP(bvh->ctx, rt_launch_stage_morton(bvh, mn, mx, (hipStream_t)stream));
+  bvh->stage = kStageMorton;
+  return RT_OK;
+}
+
+rt_status rt_bvh_sort_morton(rt_bvh* bvh, void* stream) {
+  if (!bvh) return RT_ERR_INVALID;
+  rt_status st = stage_check(bvh, kStageMorton, kStageMorton, "rt_bvh_sort_morton", "rt_bvh_compute_morton");
+  if (st != RT_OK) return st;
+  RT_HIP(bvh->ctx, rt_launch_stage_sort(bvh, (hipStream_t)stream));
+  bvh->stage = kStageSorted;
+  return RT_OK;
+}
+
+rt_status rt_bvh_build_tree(rt_bvh* bvh, const rt_mesh_view* mesh, void* stream) {
+  if (!bvh) return RT_ERR_INVALID;
+  if (!mesh_ok(mesh) || mesh->n != bvh->n)
+    return rt_set_error(bvh->ctx, RT_ERR_INVALID, "rt_bvh_build_tree needs a mesh of the BVH's triangle count");
+  rt_status st = stage_check(bvh, kStageSorted, kStageSorted, "rt_bvh_build_tree", "rt_bvh_sort_morton");
+  if (st != RT_OK) return st;
+  RT_HIP(bvh->ctx, rt_launch_stage_tree(bvh, mesh, (hipStream_t)stream));
+  bvh->stage = kStageBuilt;
+  return RT_OK;
+}
+
 
 rt_status rt_bvh_rebuild(rt_bvh* bvh, const rt_mesh_view* mesh, void* stream) {
   if (!bvh) return RT_ERR_INVALID;
@@ -290,6 +365,7 @@ rt_status rt_bvh_rebuild(rt_bvh* bvh, const rt_mesh_view* mesh, void* stream) {
     return rt_set_error(bvh->ctx, RT_ERR_INVALID, "rebuild needs a mesh with the same triangle count");
   RT_HIP(bvh->ctx, rt_note_capture(bvh, (hipStream_t)stream));
   RT_HIP(bvh->ctx, rt_launch_build(bvh, mesh, (hipStream_t)stream));
+  bvh->stage = kStageBuilt;
   return RT_OK;
 }
 
@@ -297,6 +373,8 @@ rt_status rt_bvh_refit(rt_bvh* bvh, const rt_mesh_view* mesh, void* stream) {
   if (!bvh) return RT_ERR_INVALID;
   if (!mesh_ok(mesh) || mesh->n != bvh->n)
     return rt_set_error(bvh->ctx, RT_ERR_INVALID, "refit needs a mesh with the same triangle count");
+  rt_status st = rt_bvh_require_built(bvh, "rt_bvh_refit");
+  if (st != RT_OK) return st;
   RT_HIP(bvh->ctx, rt_note_capture(bvh, (hipStream_t)stream));
   RT_HIP(bvh->ctx, rt_launch_refit(bvh, mesh, (hipStream_t)stream));
   return RT_OK;
@@ -304,12 +382,16 @@ rt_status rt_bvh_refit(rt_bvh* bvh, const rt_mesh_view* mesh, void* stream) {
 
 rt_status rt_bvh_export(const rt_bvh* bvh, const rt_bvh_export_bufs* dst, void* stream) {
   if (!bvh || !dst) return RT_ERR_INVALID;
+  rt_status st = rt_bvh_require_built(bvh, "rt_bvh_export");
+  if (st != RT_OK) return st;
   RT_HIP(bvh->ctx, rt_launch_export(bvh, dst, (hipStream_t)stream));
   return RT_OK;
 }
 
 rt_status rt_bvh_scene_bounds(const rt_bvh* bvh, void* stream, double mn[3], double mx[3]) {
   if (!bvh || !mn || !mx) return RT_ERR_INVALID;
+  if (bvh->stage < kStageBounds)
+    return rt_set_error(bvh->ctx, RT_ERR_INVALID, "rt_bvh_scene_bounds: call rt_bvh_compute_bounds first");
   uint32_t h[6 * kBoundsReplicas];
   RT_HIP(bvh->ctx, hipMemcpyAsync(h, bvh->misc + kMiscBounds, sizeof(h), hipMemcpyDeviceToHost, (hipStream_t)stream));
   RT_HIP(bvh->ctx, hipStreamSynchronize((hipStream_t)stream));
@@ -350,6 +432,8 @@ rt_status rt_trace(const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_rays* r
     return rt_set_error(bvh->ctx, RT_ERR_INVALID, "null ray arrays");
   if (mode != RT_TRACE_CLOSEST && mode != RT_TRACE_ANY)
     return rt_set_error(bvh->ctx, RT_ERR_INVALID, "mode must be RT_TRACE_CLOSEST or RT_TRACE_ANY");
+  rt_status st = rt_bvh_require_built(bvh, "rt_trace");
+  if (st != RT_OK) return st;
   RT_HIP(bvh->ctx, rt_launch_trace(bvh, mesh, rays, mode, tmin, hits, (hipStream_t)stream));
   return RT_OK;
 }
@@ -383,7 +467,7 @@ static rt_status render_check(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view
   if (mesh->n != bvh->n) return rt_set_error(ctx, RT_ERR_INVALID, "mesh and BVH triangle counts differ");
   if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT | RT_RENDER_ONE_STREAM))
     return rt_set_error(ctx, RT_ERR_INVALID, "unknown render flags");
-  return RT_OK;
+  return rt_bvh_require_built(bvh, who);
 }
 
 rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_material* mats_dev,
@@ -457,7 +541,7 @@ static rt_status diff_check(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* 
   if (!(light_scale > 0)) return rt_set_error(ctx, RT_ERR_INVALID, "light_scale must be positive");
   if (opts->flags & ~(RT_RENDER_RANDVEC_RTL | RT_RENDER_CLAMP | RT_RENDER_WAVEFRONT | RT_RENDER_ONE_STREAM))
     return rt_set_error(ctx, RT_ERR_INVALID, "unknown render flags");
-  return RT_OK;
+  return rt_bvh_require_built(bvh, "the differentiable render");
 }
 
 size_t rt_diff_record_bytes(const rt_camera* cam, const rt_render_opts* opts) {

@@ -144,6 +144,7 @@ rt_status rt_batch_rebuild(rt_batch* b, void* stream) {
   bool seg = b->seg != nullptr;
   for (const rt_batch_item& it : b->items)
     seg = seg && !it.bvh->want_tnode && !it.bvh->want_wide4 && !it.bvh->want_wide8;
+  for (int k : b->builds) const_cast<rt_bvh*>(b->items[k].bvh)->stage = kStageBuilt;
   if (seg) {
     RT_HIP(b->ctx, rt_seg_build_launch(b->seg, s));
     return RT_OK;
@@ -165,6 +166,10 @@ rt_status rt_batch_rebuild(rt_batch* b, void* stream) {
 
 rt_status rt_batch_render(rt_batch* b, uint64_t* counters_dev, void* stream) {
   if (!b) return RT_ERR_INVALID;
+  for (const rt_batch_item& it : b->items) {
+    rt_status st = rt_bvh_require_built(it.bvh, "rt_batch_render");
+    if (st != RT_OK) return st;
+  }
   for (const rt_batch_item& it : b->items)
     RT_HIP(b->ctx, rt_ensure_layout(it.bvh, b->tuning.node_layout, (hipStream_t)stream));
   RT_HIP(b->ctx, rt_launch_wavefront(b->ws, b->npix, b->ntiles, b->nscenes, b->max_spp, b->max_depth,

@@ -174,15 +174,32 @@ struct KeyTraits<uint64_t> {
 // scaled, clamped with fmax/fmin (NaN -> 0), rounded to float, truncated.
 // Each workgroup also builds the digit histograms of every sort pass in
 // LDS and adds them to the global histogram once.
+// Bounds handed to BVH_d::computeMortonCodes(mMin, mMax) by the caller
+// (kernels/bvh.cu:260-265); when !use the keys are normalised by the scene
+// bounds the build reduced into misc[].
+struct GivenBounds {
+  double mn[3], mx[3];
+  int use;
+};
+
 template <typename K>
 __device__ __forceinline__ void morton_body(uint32_t bid, uint32_t nblk, const float* __restrict__ tri_box, int n,
                                                 const uint32_t* __restrict__ misc, K* __restrict__ keys,
-                                                uint32_t* __restrict__ hist, int per_block) {
+                                                uint32_t* __restrict__ hist, int per_block,
+                                                const GivenBounds& gb) {
   constexpr int P = KeyTraits<K>::kPasses;
   __shared__ uint32_t s_hist[P * 256];
   for (int k = threadIdx.x; k < P * 256; k += 256) s_hist[k] = 0u;
   double mn[3], mx[3], ext[3];
-  load_bounds(misc, mn, mx);
+  if (gb.use) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = gb.mn[k];
+      mx[k] = gb.mx[k];
+    }
+  } else {
+    load_bounds(misc, mn, mx);
+  }
 #pragma unroll
   for (int k = 0; k < 3; ++k) ext[k] = mx[k] - mn[k];
   __syncthreads();
@@ -212,8 +229,8 @@ __device__ __forceinline__ void morton_body(uint32_t bid, uint32_t nblk, const f
 template <typename K>
 __global__ __launch_bounds__(256) void k_morton(const float* __restrict__ tri_box, int n,
                                                 const uint32_t* __restrict__ misc, K* __restrict__ keys,
-                                                uint32_t* __restrict__ hist, int per_block) {
-  morton_body(blockIdx.x, gridDim.x, tri_box, n, misc, keys, hist, per_block);
+                                                uint32_t* __restrict__ hist, int per_block, GivenBounds gb) {
+  morton_body(blockIdx.x, gridDim.x, tri_box, n, misc, keys, hist, per_block, gb);
 }
 
 // ------------------------------------------------------------ onesweep sort
@@ -1352,7 +1369,7 @@ template <typename K>
 __global__ __launch_bounds__(256) void k_seg_morton(const SegBvh* segs, const uint32_t* off, int ns) {
   const SegBlock b = seg_block(segs, off, ns);
   const SegBvh& g = *b.g;
-  morton_body<K>(b.bid, b.nblk, g.tri_box, g.n, g.misc, (K*)g.keys[0], g.hist, kMortonPerBlock);
+  morton_body<K>(b.bid, b.nblk, g.tri_box, g.n, g.misc, (K*)g.keys[0], g.hist, kMortonPerBlock, GivenBounds{});
 }
 
 template <typename K>
@@ -1500,24 +1517,34 @@ void launch_climb_top(rt_bvh* b, const K* keys, hipStream_t s, bool topology = t
                                                 hand_off_slot(passes));
 }
 
+// hist, look-back status and arrival counters: one contiguous range
+size_t counter_words(const rt_bvh* b) {
+  return (size_t)b->passes * 256 + (size_t)b->passes * b->ntiles * 256 + (size_t)(b->n > 1 ? b->n - 1 : 1);
+}
+
 template <typename K>
-hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
-  const int n = b->n;
-  constexpr int P = KeyTraits<K>::kPasses;
-  // hist, look-back status and arrival counters are one contiguous range
-  size_t zero_words = (size_t)P * 256 + (size_t)P * b->ntiles * 256 + (size_t)(n > 1 ? n - 1 : 1);
-  k_init<<<min(cdiv((long)zero_words, 256), 2048), 256, 0, s>>>(b->misc, b->hist, zero_words, 1);
-  k_aabb_bounds<<<min(cdiv(n, 256), 1024), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box,
-                                                        b->trirec, b->misc, 1);
+void launch_morton(rt_bvh* b, const GivenBounds& gb, hipStream_t s) {
   const int per_block = kMortonPerBlock;
-  k_morton<K><<<cdiv(n, per_block), 256, 0, s>>>(b->tri_box, n, b->misc, (K*)b->keys[0], b->hist, per_block);
+  k_morton<K><<<cdiv(b->n, per_block), 256, 0, s>>>(b->tri_box, b->n, b->misc, (K*)b->keys[0], b->hist, per_block,
+                                                      gb);
+}
+
+template <typename K>
+void launch_sort(rt_bvh* b, hipStream_t s) {
+  constexpr int P = KeyTraits<K>::kPasses;
   for (int p = 0; p < P; ++p) {
     int src = p & 1, dst = src ^ 1;
     k_onesweep<K><<<b->ntiles, kSortThreads, 0, s>>>(
-        (const K*)b->keys[src], p == 0 ? nullptr : b->ids[src], (K*)b->keys[dst], b->ids[dst], n, 8 * p,
+        (const K*)b->keys[src], p == 0 ? nullptr : b->ids[src], (K*)b->keys[dst], b->ids[dst], b->n, 8 * p,
         b->hist + 256 * p, b->status + (size_t)p * b->ntiles * 256, b->misc + kMiscTileCtr + p);
   }
-  // P is even: the sorted keys and ids are back in buffer 0.
+}
+
+// Hierarchy, node boxes and the traversal layouts from the sorted keys (P is
+// even: the sorted keys and ids are back in buffer 0).
+template <typename K>
+void launch_tree(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
+  const int n = b->n;
   const K* keys = (const K*)b->keys[0];
   if (b->build_flags & RT_BUILD_KARRAS_SEARCH) {
     k_hierarchy<K><<<cdiv(n, 256), 256, 0, s>>>(keys, b->ids[0], n, b->tri_box, m->vx_dev, m->vy_dev, m->vz_dev,
@@ -1532,7 +1559,29 @@ hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
     launch_climb_top<K>(b, keys, s);
   }
   launch_wide(b, true, s);
+}
+
+template <typename K>
+hipError_t build_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
+  const int n = b->n;
+  k_init<<<min(cdiv((long)counter_words(b), 256), 2048), 256, 0, s>>>(b->misc, b->hist, counter_words(b), 1);
+  k_aabb_bounds<<<min(cdiv(n, 256), 1024), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box,
+                                                        b->trirec, b->misc, 1);
+  launch_morton<K>(b, GivenBounds{}, s);
+  launch_sort<K>(b, s);
+  launch_tree<K>(b, m, s);
   return hipGetLastError();
+}
+
+// Triangle boxes in triangle order as two n*3 arrays.
+__global__ void k_split_boxes(int n, const float* __restrict__ tri_box, float* __restrict__ lo, float* __restrict__ hi) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float* b = tri_box + 6 * (size_t)i;
+  for (int k = 0; k < 3; ++k) {
+    if (lo) lo[3 * (size_t)i + k] = b[k];
+    if (hi) hi[3 * (size_t)i + k] = b[3 + k];
+  }
 }
 
 template <typename K>
@@ -1563,6 +1612,48 @@ hipError_t refit_typed(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
 }
 
 }  // namespace
+
+hipError_t rt_launch_stage_bounds(rt_bvh* b, const rt_mesh_view* m, float* lo, float* hi, hipStream_t s) {
+  const int n = b->n;
+  k_init<<<1, 256, 0, s>>>(b->misc, nullptr, 0, 1);
+  k_aabb_bounds<<<min(cdiv(n, 256), 1024), 256, 0, s>>>(m->vx_dev, m->vy_dev, m->vz_dev, m->idx_dev, n, b->tri_box,
+                                                        b->trirec, b->misc, 1);
+  if (lo || hi) k_split_boxes<<<cdiv(n, 256), 256, 0, s>>>(n, b->tri_box, lo, hi);
+  return hipGetLastError();
+}
+
+hipError_t rt_launch_stage_morton(rt_bvh* b, const double* mn, const double* mx, hipStream_t s) {
+  GivenBounds gb{};
+  if (mn && mx) {
+    for (int k = 0; k < 3; ++k) {
+      gb.mn[k] = mn[k];
+      gb.mx[k] = mx[k];
+    }
+    gb.use = 1;
+  }
+  k_init<<<min(cdiv((long)counter_words(b), 256), 2048), 256, 0, s>>>(b->misc, b->hist, counter_words(b), 0);
+  if (b->bits == 30)
+    launch_morton<uint32_t>(b, gb, s);
+  else
+    launch_morton<uint64_t>(b, gb, s);
+  return hipGetLastError();
+}
+
+hipError_t rt_launch_stage_sort(rt_bvh* b, hipStream_t s) {
+  if (b->bits == 30)
+    launch_sort<uint32_t>(b, s);
+  else
+    launch_sort<uint64_t>(b, s);
+  return hipGetLastError();
+}
+
+hipError_t rt_launch_stage_tree(rt_bvh* b, const rt_mesh_view* m, hipStream_t s) {
+  if (b->bits == 30)
+    launch_tree<uint32_t>(b, m, s);
+  else
+    launch_tree<uint64_t>(b, m, s);
+  return hipGetLastError();
+}
 
 hipError_t rt_note_capture(const rt_bvh* b, hipStream_t s) {
   hipStreamCaptureStatus st = hipStreamCaptureStatusNone;

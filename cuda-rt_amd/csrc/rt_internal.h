@@ -151,9 +151,15 @@ struct rt_bvh {
   // wanted only later is refilled before every traversal that reads it, since
   // replays of that graph leave it stale.
   mutable bool captured = false, cap_tnode = true, cap_wide4 = true, cap_wide8 = true;
+  // How far the staged build has got (kStage*); rt_bvh_build and
+  // rt_bvh_rebuild go straight to kStageBuilt. Traversal, refit and export
+  // need kStageBuilt.
+  int32_t stage = 0;
 
   BvhView view() const { return BvhView{n, node_box, tnode, ltri, wide, wideq, wide8, leaf_of}; }
 };
+
+enum : int32_t { kStageCreated = 0, kStageBounds = 1, kStageMorton = 2, kStageSorted = 3, kStageBuilt = 4 };
 
 // misc[] slots
 constexpr int kBoundsReplicas = 8;  // scene-bound accumulators, spread to cut atomic contention
@@ -195,15 +201,25 @@ constexpr int kSortKeysPerThread = 16;
 constexpr int kSortTile = kSortThreads * kSortKeysPerThread;
 
 rt_status rt_set_error(rt_ctx* ctx, rt_status s, const char* fmt, ...);
+// RT_OK if the BVH is built, else RT_ERR_INVALID with a message naming `who`.
+rt_status rt_bvh_require_built(const rt_bvh* bvh, const char* who);
 
 // Launchers (rt_build.hip).
 hipError_t rt_launch_build(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
 hipError_t rt_launch_refit(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
-// Before the first traversal of a BVH with node layout 4 or 8: fill those
-// nodes now and keep them filled by every later build and refit.
+// The build's stages one at a time (rt_launch_build runs all four):
+// triangle boxes + scene bounds (optionally copied out as n*3 arrays),
+// Morton keys (normalised by mn/mx when both are given, else by the scene
+// bounds), the radix sort, and the hierarchy with the traversal layouts.
+hipError_t rt_launch_stage_bounds(rt_bvh* b, const rt_mesh_view* m, float* lo, float* hi, hipStream_t s);
+hipError_t rt_launch_stage_morton(rt_bvh* b, const double* mn, const double* mx, hipStream_t s);
+hipError_t rt_launch_stage_sort(rt_bvh* b, hipStream_t s);
+hipError_t rt_launch_stage_tree(rt_bvh* b, const rt_mesh_view* m, hipStream_t s);
 // Record that a rebuild or refit of b is being launched on s; if s is
 // capturing a hipGraph, the layouts it writes are noted (see rt_bvh::captured).
 hipError_t rt_note_capture(const rt_bvh* b, hipStream_t s);
+// Before the first traversal of a BVH with node layout 2, 4 or 8: fill those
+// nodes now and keep them filled by every later build and refit.
 hipError_t rt_ensure_layout(const rt_bvh* b, int layout, hipStream_t s);
 // Builds and refits stop filling the binary records and the unquantized
 // 4-wide and 8-wide nodes unless `layout` traverses them; rt_ensure_layout

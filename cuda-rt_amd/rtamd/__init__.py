@@ -72,6 +72,7 @@ EXPORTED = [
     "rt_bvh_refit", "rt_bvh_export", "rt_bvh_scene_bounds", "rt_bvh_num_triangles", "rt_bvh_diagnostics", "rt_bvh_destroy",
     "rt_trace", "rt_render", "rt_gen_soup", "rt_gen_icosphere_counts", "rt_gen_icosphere", "rt_gen_jitter",
     "rt_ctx_set_kernel_timing", "rt_ctx_kernel_stats", "rt_render_noalloc", "rt_render_noalloc_max_bytes",
+    "rt_bvh_create", "rt_bvh_compute_bounds", "rt_bvh_compute_morton", "rt_bvh_sort_morton", "rt_bvh_build_tree",
 ]
 
 
@@ -96,6 +97,11 @@ def lib():
     L.rt_ctx_create.argtypes = [ctypes.c_int32, ctypes.POINTER(vp)]
     L.rt_ctx_destroy.argtypes = [vp]
     L.rt_bvh_build.argtypes = [vp, ctypes.POINTER(MeshView), ctypes.POINTER(BuildOpts), vp, ctypes.POINTER(vp)]
+    L.rt_bvh_create.argtypes = [vp, ctypes.c_int32, ctypes.POINTER(BuildOpts), ctypes.POINTER(vp)]
+    L.rt_bvh_compute_bounds.argtypes = [vp, ctypes.POINTER(MeshView), vp, vp, vp]
+    L.rt_bvh_compute_morton.argtypes = [vp, c_d, c_d, vp]
+    L.rt_bvh_sort_morton.argtypes = [vp, vp]
+    L.rt_bvh_build_tree.argtypes = [vp, ctypes.POINTER(MeshView), vp]
     L.rt_bvh_rebuild.argtypes = [vp, ctypes.POINTER(MeshView), vp]
     L.rt_bvh_refit.argtypes = [vp, ctypes.POINTER(MeshView), vp]
     L.rt_bvh_export.argtypes = [vp, ctypes.POINTER(ExportBufs), vp]
@@ -308,15 +314,48 @@ class Context:
 
 
 class BVH:
-    def __init__(self, ctx, mesh, morton_bits=30, stream=None, flags=RT_BUILD_DEFAULT):
+    def __init__(self, ctx, mesh, morton_bits=30, stream=None, flags=RT_BUILD_DEFAULT, staged_n=None):
+        """Build the BVH of `mesh` (rt_bvh_build), or, with mesh=None and
+        staged_n=n, allocate an unbuilt BVH for n triangles (rt_bvh_create)
+        to build stage by stage."""
         self.ctx, self.L = ctx, ctx.L
         self.h = vp()
-        self.n = mesh.n
+        self.n = mesh.n if mesh is not None else int(staged_n)
         self.bits = morton_bits
         o = BuildOpts(morton_bits, flags)
+        if mesh is None:
+            ctx.check(self.L.rt_bvh_create(ctx.h, self.n, ctypes.byref(o), ctypes.byref(self.h)))
+            return
         mv = mesh.view()
         ctx.check(self.L.rt_bvh_build(ctx.h, ctypes.byref(mv), ctypes.byref(o), _stream_handle(stream),
                                       ctypes.byref(self.h)))
+
+    @classmethod
+    def create(cls, ctx, n, morton_bits=30, flags=RT_BUILD_DEFAULT):
+        """An unbuilt BVH for n triangles (rt_bvh_create)."""
+        return cls(ctx, None, morton_bits, flags=flags, staged_n=n)
+
+    def compute_bounds(self, mesh, tri_lo=None, tri_hi=None, stream=None):
+        """Triangle boxes and scene bounds (rt_bvh_compute_bounds); tri_lo /
+        tri_hi: optional float32 CUDA tensors of n*3 that receive the boxes."""
+        mv = mesh.view()
+        ptr = lambda t: None if t is None else t.data_ptr()
+        self.ctx.check(self.L.rt_bvh_compute_bounds(self.h, ctypes.byref(mv), ptr(tri_lo), ptr(tri_hi),
+                                                    _stream_handle(stream)))
+
+    def compute_morton(self, mn=None, mx=None, stream=None):
+        """Morton keys normalised by mn / mx (3 doubles each), or by the scene
+        bounds when both are None (rt_bvh_compute_morton)."""
+        a = None if mn is None else (ctypes.c_double * 3)(*[float(v) for v in mn])
+        b = None if mx is None else (ctypes.c_double * 3)(*[float(v) for v in mx])
+        self.ctx.check(self.L.rt_bvh_compute_morton(self.h, a, b, _stream_handle(stream)))
+
+    def sort_morton(self, stream=None):
+        self.ctx.check(self.L.rt_bvh_sort_morton(self.h, _stream_handle(stream)))
+
+    def build_tree(self, mesh, stream=None):
+        mv = mesh.view()
+        self.ctx.check(self.L.rt_bvh_build_tree(self.h, ctypes.byref(mv), _stream_handle(stream)))
 
     def rebuild(self, mesh, stream=None):
         mv = mesh.view()

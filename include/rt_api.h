@@ -139,6 +139,33 @@ const char *rt_last_error(const rt_ctx *ctx);
  * buildTree (src/bvh.cpp:3-32, kernels/bvh.cu:260-292). Allocates the BVH. */
 rt_status rt_bvh_build(rt_ctx *ctx, const rt_mesh_view *mesh, const rt_build_opts *opts, void *stream,
                        rt_bvh **out);
+/* Staged build: the calls of BVH_d::setUp one at a time (src/bvh.cpp:3-32,
+ * inc/bvh.cuh:53-59), for callers written against the reference's stage
+ * methods. rt_bvh_create allocates a BVH for n triangles without building
+ * it (BVH_d::setUp's cudaMallocs, src/bvh.cpp:12-18); the four stages then
+ * run in this order, each stream-ordered and allocation-free. After
+ * rt_bvh_build_tree the BVH is the one rt_bvh_build makes when the Morton
+ * bounds are the scene bounds. Out of order, a stage returns RT_ERR_INVALID;
+ * traversal, refit and export need a built BVH (rt_bvh_build_tree,
+ * rt_bvh_build or rt_bvh_rebuild). */
+rt_status rt_bvh_create(rt_ctx *ctx, int32_t n, const rt_build_opts *opts, rt_bvh **out);
+/* Scene_d::computeBoundingBoxes (kernels/scene.cu:48-57,88-120) with the
+ * reduction of Scene_d::findMinMax (:72-85): per-triangle boxes, and the
+ * scene bounds (read them with rt_bvh_scene_bounds). If tri_lo_dev /
+ * tri_hi_dev are not NULL they also receive the boxes, n*3 floats each in
+ * triangle order (the reference's BBoxs). */
+rt_status rt_bvh_compute_bounds(rt_bvh *bvh, const rt_mesh_view *mesh, float *tri_lo_dev, float *tri_hi_dev,
+                                void *stream);
+/* BVH_d::computeMortonCodes(mMin, mMax) (kernels/bvh.cu:22-61,260-265):
+ * keys of the box centroids normalised by mn / mx, or by the scene bounds of
+ * rt_bvh_compute_bounds if both are NULL; object ids 0..n-1. */
+rt_status rt_bvh_compute_morton(rt_bvh *bvh, const double mn[3], const double mx[3], void *stream);
+/* BVH_d::sortMortonCodes (kernels/bvh.cu:267-273): stable sort of the keys
+ * with their object ids. */
+rt_status rt_bvh_sort_morton(rt_bvh *bvh, void *stream);
+/* BVH_d::buildTree (kernels/bvh.cu:278-292): hierarchy and node boxes from
+ * the sorted keys, then the traversal node layouts. */
+rt_status rt_bvh_build_tree(rt_bvh *bvh, const rt_mesh_view *mesh, void *stream);
 /* Rebuild the whole hierarchy for a mesh with the same triangle count into
  * an existing BVH without allocating (stream-ordered, graph-capturable). */
 rt_status rt_bvh_rebuild(rt_bvh *bvh, const rt_mesh_view *mesh, void *stream);

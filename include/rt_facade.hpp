@@ -188,7 +188,17 @@ class Scene_h {
   std::vector<float> image;  // width*height*3, row-flipped, filled by RayTracer::pullRaytracedImage
 };
 
-// Device BVH (inc/bvh.cuh:31-71): owns the LBVH built by rt_bvh_build.
+// Per-triangle boxes on the device (the reference's BoundingBox* BBoxs,
+// kernels/scene.cu:48-57): lo and hi, n*3 floats each, in triangle order.
+struct BBoxes {
+  float* lo_dev = nullptr;
+  float* hi_dev = nullptr;
+};
+
+// Device BVH (inc/bvh.cuh:31-71): owns the LBVH. Built either the
+// reference's way, setUp(..., mMin, mMax, ...) running computeMortonCodes,
+// sortMortonCodes and buildTree (src/bvh.cpp:3-32), each a stage call of the
+// C-ABI, or in one call (setUp(ctx, mesh, bits), rt_bvh_build).
 class BVH_d {
  public:
   BVH_d() = default;
@@ -196,19 +206,75 @@ class BVH_d {
   BVH_d& operator=(const BVH_d&) = delete;
   ~BVH_d() { destroy(); }
 
-  // setUp -> computeMortonCodes, sortMortonCodes, buildTree
-  // (src/bvh.cpp:3-29): one stream-ordered build.
+  // One stream-ordered build (rt_bvh_build).
   void setUp(rt_ctx* ctx, const rt_mesh_view& mesh, int morton_bits = 30, void* stream = nullptr) {
     destroy();
     ctx_ = ctx;
     rt_build_opts o{morton_bits, RT_BUILD_DEFAULT};
     detail::check(rt_bvh_build(ctx, &mesh, &o, stream, &bvh_), ctx);
+    bounds_ready_ = false;
   }
+
+  // BVH_d::setUp(vertices, normals, BBoxs, t_indices, numTriangles,
+  // materials, mMin, mMax, material_ids) (inc/bvh.cuh:53, src/bvh.cpp:3-32).
+  // The mesh view carries the vertices, normals, triangle indices and
+  // material ids. The BVH keeps its own copy of the triangle boxes: it reuses
+  // the ones computeBoundingBoxes just made for this mesh, else recomputes
+  // them (the same values as BBoxs). The Morton keys use mMin / mMax as
+  // given, as the reference's computeMortonCodes does.
+  void setUp(rt_ctx* ctx, const rt_mesh_view& mesh, const BBoxes& BBoxs, int numTriangles,
+             const rt_material* materials, Vec3d mMin, Vec3d mMax, int morton_bits = 30, void* stream = nullptr) {
+    (void)BBoxs;
+    if (numTriangles != mesh.n) throw Error(RT_ERR_INVALID, "setUp: numTriangles differs from the mesh");
+    materials_ = materials;
+    stream_ = stream;
+    ensure(ctx, mesh, morton_bits);
+    if (!bounds_ready_ || !same_mesh(mesh)) {
+      detail::check(rt_bvh_compute_bounds(bvh_, &mesh, nullptr, nullptr, stream), ctx_);
+      mesh_ = mesh;
+    }
+    computeMortonCodes(mMin, mMax);
+    sortMortonCodes();
+    setupLeafNodes();
+    buildTree();
+  }
+  // The triangle boxes of `mesh` into this BVH and into `out` (if its
+  // pointers are set), with the scene bounds for Scene_d::findMinMax.
+  void computeBoundingBoxes(rt_ctx* ctx, const rt_mesh_view& mesh, const BBoxes& out, int morton_bits = 30,
+                            void* stream = nullptr) {
+    stream_ = stream;
+    ensure(ctx, mesh, morton_bits);
+    detail::check(rt_bvh_compute_bounds(bvh_, &mesh, out.lo_dev, out.hi_dev, stream), ctx_);
+    mesh_ = mesh;
+    bounds_ready_ = true;
+  }
+  // kernels/bvh.cu:260-265.
+  void computeMortonCodes(Vec3d& mMin, Vec3d& mMax) {
+    const double mn[3] = {mMin.x, mMin.y, mMin.z}, mx[3] = {mMax.x, mMax.y, mMax.z};
+    detail::check(rt_bvh_compute_morton(bvh_, mn, mx, stream_), ctx_);
+  }
+  // kernels/bvh.cu:267-273.
+  void sortMortonCodes() { detail::check(rt_bvh_sort_morton(bvh_, stream_), ctx_); }
+  // kernels/bvh.cu:275-277: does nothing in the reference either.
+  void setupLeafNodes() {}
+  // kernels/bvh.cu:278-292.
+  void buildTree() {
+    detail::check(rt_bvh_build_tree(bvh_, &mesh_, stream_), ctx_);
+    bounds_ready_ = false;  // a later setUp recomputes the boxes (the vertices may have moved)
+  }
+
   void rebuild(const rt_mesh_view& mesh, void* stream = nullptr) {
     detail::check(rt_bvh_rebuild(bvh_, &mesh, stream), ctx_);
   }
   void refit(const rt_mesh_view& mesh, void* stream = nullptr) {
     detail::check(rt_bvh_refit(bvh_, &mesh, stream), ctx_);
+  }
+  // Scene bounds of the last build or computeBoundingBoxes (synchronises).
+  void sceneBounds(Vec3d& mn, Vec3d& mx, void* stream = nullptr) const {
+    double a[3], b[3];
+    detail::check(rt_bvh_scene_bounds(bvh_, stream, a, b), ctx_);
+    mn = Vec3d(a[0], a[1], a[2]);
+    mx = Vec3d(b[0], b[1], b[2]);
   }
   // Batched forms of the per-ray __device__ intersect / intersectAny
   // (kernels/bvh.cu:382-449).
@@ -222,14 +288,33 @@ class BVH_d {
     detail::check(rt_bvh_export(bvh_, &dst, stream), ctx_);
   }
   rt_bvh* handle() const { return bvh_; }
+  const rt_material* materials() const { return materials_; }
 
  private:
+  void ensure(rt_ctx* ctx, const rt_mesh_view& mesh, int morton_bits) {
+    if (bvh_ && ctx == ctx_ && rt_bvh_num_triangles(bvh_) == mesh.n && bits_ == morton_bits) return;
+    destroy();
+    ctx_ = ctx;
+    bits_ = morton_bits;
+    rt_build_opts o{morton_bits, RT_BUILD_DEFAULT};
+    detail::check(rt_bvh_create(ctx, mesh.n, &o, &bvh_), ctx);
+  }
+  bool same_mesh(const rt_mesh_view& m) const {
+    return m.vx_dev == mesh_.vx_dev && m.vy_dev == mesh_.vy_dev && m.vz_dev == mesh_.vz_dev &&
+           m.idx_dev == mesh_.idx_dev && m.n == mesh_.n;
+  }
   void destroy() {
     if (bvh_) rt_bvh_destroy(bvh_);
     bvh_ = nullptr;
+    bounds_ready_ = false;
   }
   rt_ctx* ctx_ = nullptr;
   rt_bvh* bvh_ = nullptr;
+  int bits_ = 30;
+  rt_mesh_view mesh_{};
+  const rt_material* materials_ = nullptr;
+  void* stream_ = nullptr;
+  bool bounds_ready_ = false;
 };
 
 // Device scene (inc/scene.h:60-117).
@@ -238,7 +323,8 @@ class Scene_d {
   explicit Scene_d(rt_ctx* ctx) : ctx_(ctx) {}
 
   // Scene_d::operator=(const Scene_h&) (src/scene.cpp:82-161): upload the
-  // mesh, then computeBoundingBoxes + findMinMax + bvh.setUp in one build.
+  // mesh, then computeBoundingBoxes, findMinMax and bvh.setUp, in the
+  // reference's order.
   Scene_d& operator=(const Scene_h& h) {
     numTriangles = h.numTriangles();
     numVertices = (int)h.vx.size();
@@ -258,15 +344,20 @@ class Scene_d {
     } else {
       mats_.upload(h.materials);
     }
-    bvh.setUp(ctx_, mesh(), morton_bits);
+    computeBoundingBoxes();
+    Vec3d mMin, mMax;
+    findMinMax(mMin, mMax);
+    bvh.setUp(ctx_, mesh(), BBoxs(), numTriangles, materials(), mMin, mMax, morton_bits);
     return *this;
   }
-  void findMinMax(Vec3d& mn, Vec3d& mx) const {
-    double a[3], b[3];
-    detail::check(rt_bvh_scene_bounds(bvh.handle(), nullptr, a, b), ctx_);
-    mn = Vec3d(a[0], a[1], a[2]);
-    mx = Vec3d(b[0], b[1], b[2]);
+  // kernels/scene.cu:48-57: per-triangle boxes into BBoxs().
+  void computeBoundingBoxes() {
+    bbox_lo_.resize(3 * (size_t)numTriangles);
+    bbox_hi_.resize(3 * (size_t)numTriangles);
+    bvh.computeBoundingBoxes(ctx_, mesh(), BBoxs(), morton_bits);
   }
+  // kernels/scene.cu:72-85: the bounds of the boxes (+-1e9 start values).
+  void findMinMax(Vec3d& mn, Vec3d& mx) const { bvh.sceneBounds(mn, mx); }
   rt_mesh_view mesh() const {
     rt_mesh_view m{};
     m.vx_dev = vx_.get(), m.vy_dev = vy_.get(), m.vz_dev = vz_.get();
@@ -277,6 +368,7 @@ class Scene_d {
     m.n = numTriangles;
     return m;
   }
+  BBoxes BBoxs() const { return BBoxes{bbox_lo_.get(), bbox_hi_.get()}; }
   const rt_material* materials() const { return mats_.get(); }
 
   int numVertices = 0, numTriangles = 0, numMaterials = 0, imageWidth = 0, imageHeight = 0;
@@ -285,7 +377,7 @@ class Scene_d {
 
  private:
   rt_ctx* ctx_;
-  detail::DevBuf<float> vx_, vy_, vz_, nx_, ny_, nz_;
+  detail::DevBuf<float> vx_, vy_, vz_, nx_, ny_, nz_, bbox_lo_, bbox_hi_;
   detail::DevBuf<int32_t> idx_, mat_id_;
   detail::DevBuf<rt_material> mats_;
 };
