@@ -90,6 +90,25 @@ const rt_overlap* rt_ctx_overlap(rt_ctx* ctx) {
   return &o;
 }
 
+hipError_t rt_dev_alloc(rt_ctx* ctx, void** p, size_t bytes) {
+  *p = nullptr;
+  if (ctx && ctx->fail_alloc_in > 0 && --ctx->fail_alloc_in == 0) return hipErrorOutOfMemory;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess || !*p) {
+    (void)hipGetLastError();
+    *p = nullptr;
+    return e != hipSuccess ? e : hipErrorOutOfMemory;
+  }
+  if (ctx) ctx->dev_bytes += bytes;
+  return hipSuccess;
+}
+
+void rt_dev_free(rt_ctx* ctx, void* p, size_t bytes) {
+  if (!p) return;
+  (void)hipFree(p);
+  if (ctx) ctx->dev_bytes -= bytes;
+}
+
 rt_status rt_bvh_require_built(const rt_bvh* bvh, const char* who) {
   if (bvh->stage == kStageBuilt) return RT_OK;
   return rt_set_error(bvh->ctx, RT_ERR_INVALID, "%s: the BVH is not built (run rt_bvh_build_tree or rt_bvh_rebuild)",
@@ -121,15 +140,15 @@ rt_status rt_ctx_create(int32_t device, rt_ctx** out) {
 
 void rt_ctx_destroy(rt_ctx* ctx) {
   if (!ctx) return;
-  if (ctx->workspace) (void)hipFree(ctx->workspace);
-  if (ctx->shade) (void)hipFree(ctx->shade);
-  if (ctx->diff) (void)hipFree(ctx->diff);
+  rt_dev_free(ctx, ctx->workspace, ctx->workspace_bytes);
+  rt_dev_free(ctx, ctx->shade, ctx->shade_bytes);
+  rt_dev_free(ctx, ctx->diff, ctx->diff_bytes);
   for (auto& r : ctx->timer.recs) {
     (void)hipEventDestroy(r.a);
     (void)hipEventDestroy(r.b);
   }
   for (hipEvent_t e : ctx->timer.pool) (void)hipEventDestroy(e);
-  if (ctx->timer.dev_work) (void)hipFree(ctx->timer.dev_work);
+  rt_dev_free(ctx, ctx->timer.dev_work, 6 * sizeof(unsigned long long));
   rt_overlap& o = ctx->overlap;
   for (hipEvent_t e : {o.start, o.filled, o.shaded, o.flushed[0], o.flushed[1], o.done})
     if (e) (void)hipEventDestroy(e);
@@ -140,14 +159,24 @@ void rt_ctx_destroy(rt_ctx* ctx) {
 const char* rt_last_error(const rt_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 
+rt_status rt_ctx_fail_alloc(rt_ctx* ctx, int32_t k) {
+  if (!ctx || k < 0) return RT_ERR_INVALID;
+  ctx->fail_alloc_in = k;
+  return RT_OK;
+}
+
+rt_status rt_ctx_device_bytes(const rt_ctx* ctx, uint64_t* bytes) {
+  if (!ctx || !bytes) return RT_ERR_INVALID;
+  *bytes = ctx->dev_bytes;
+  return RT_OK;
+}
+
 rt_status rt_ctx_set_kernel_timing(rt_ctx* ctx, int32_t enable) {
   if (!ctx) return RT_ERR_INVALID;
   if (enable && !ctx->timer.dev_work) {
     void* p = nullptr;
-    if (hipMalloc(&p, 6 * sizeof(unsigned long long)) != hipSuccess) {
-      (void)hipGetLastError();
+    if (rt_dev_alloc(ctx, &p, 6 * sizeof(unsigned long long)) != hipSuccess)
       return rt_set_error(ctx, RT_ERR_OOM, "kernel timing counters");
-    }
     RT_HIP(ctx, hipMemset(p, 0, 6 * sizeof(unsigned long long)));
     ctx->timer.dev_work = (unsigned long long*)p;
   }
@@ -218,11 +247,10 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   size_t o_lo = take((size_t)n * 4), o_def2 = take((size_t)n * 4), o_tr = take((size_t)n * 48);
   size_t o_wc = take(ni * 4), o_ws = take(((ni + 1023) / 1024 + 1) * 4), o_wsrc = take(ni * 16);
   void* p = nullptr;
-  hipError_t e = hipMalloc(&p, off);
-  if (e != hipSuccess || !p) {
+  hipError_t e = rt_dev_alloc(ctx, &p, off);
+  if (e != hipSuccess) {
     delete b;
-    (void)hipGetLastError();
-    return rt_set_error(ctx, RT_ERR_OOM, "hipMalloc(%zu) failed", off);
+    return rt_set_error(ctx, RT_ERR_OOM, "device allocation of %zu bytes for the BVH failed", off);
   }
   char* a = (char*)p;
   b->arena = p;
@@ -259,7 +287,7 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   // The diagnostic counters in misc[] accumulate over the BVH's lifetime.
   e = hipMemset(b->misc, 0, kMiscWords * sizeof(uint32_t));
   if (e != hipSuccess) {
-    (void)hipFree(p);
+    rt_dev_free(ctx, p, off);
     delete b;
     return rt_set_error(ctx, RT_ERR_DEVICE, "hipMemset failed: %s", hipGetErrorString(e));
   }
@@ -421,7 +449,7 @@ rt_status rt_bvh_diagnostics(const rt_bvh* bvh, void* stream, uint32_t out[2]) {
 
 void rt_bvh_destroy(rt_bvh* bvh) {
   if (!bvh) return;
-  if (bvh->arena) (void)hipFree(bvh->arena);
+  rt_dev_free(bvh->ctx, bvh->arena, bvh->arena_bytes);
   delete bvh;
 }
 
@@ -440,17 +468,22 @@ rt_status rt_trace(const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_rays* r
 
 static rt_status grow(rt_ctx* ctx, void** buf, size_t* have, size_t need, hipStream_t stream, const char* what) {
   if (*have >= need) return RT_OK;
+  // Under hipGraph capture nothing may be (re)allocated: the graph would keep
+  // pointers into memory freed here.
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  RT_HIP(ctx, hipStreamIsCapturing(stream, &cap));
+  if (cap != hipStreamCaptureStatusNone)
+    return rt_set_error(ctx, RT_ERR_INVALID,
+                        "%s of %zu bytes needed under graph capture, but the context holds %zu: make the same call "
+                        "once before capturing",
+                        what, need, *have);
   if (*buf) {
     RT_HIP(ctx, hipStreamSynchronize(stream));
-    RT_HIP(ctx, hipFree(*buf));
+    rt_dev_free(ctx, *buf, *have);
     *buf = nullptr;
     *have = 0;
   }
-  if (hipMalloc(buf, need) != hipSuccess) {
-    *buf = nullptr;
-    (void)hipGetLastError();
-    return rt_set_error(ctx, RT_ERR_OOM, "%s of %zu bytes", what, need);
-  }
+  if (rt_dev_alloc(ctx, buf, need) != hipSuccess) return rt_set_error(ctx, RT_ERR_OOM, "%s of %zu bytes", what, need);
   *have = need;
   return RT_OK;
 }

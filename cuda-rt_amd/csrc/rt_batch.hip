@@ -27,6 +27,7 @@ struct rt_batch {
   std::vector<int> builds;  // the first item of each distinct BVH: rebuilds build each BVH once
   void* ws = nullptr;
   void* shade = nullptr;  // the scenes' shading tables, back to back
+  size_t ws_bytes = 0, shade_bytes = 0;
   size_t npix = 0, ntiles = 0, nscenes = 0;
   int max_spp = 0, max_depth = 0;
   rt_tuning tuning;
@@ -45,9 +46,9 @@ void rt_batch_destroy(rt_batch* b) {
     if (b->streams[k]) (void)hipStreamDestroy(b->streams[k]);
   }
   if (b->fork) (void)hipEventDestroy(b->fork);
-  rt_seg_build_destroy(b->seg);
-  if (b->ws) (void)hipFree(b->ws);
-  if (b->shade) (void)hipFree(b->shade);
+  rt_seg_build_destroy(b->ctx, b->seg);
+  rt_dev_free(b->ctx, b->ws, b->ws_bytes);
+  rt_dev_free(b->ctx, b->shade, b->shade_bytes);
   delete b;
 }
 
@@ -88,8 +89,11 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
     return rt_set_error(ctx, RT_ERR_INVALID, "batch has too many pixels or triangles");
   }
   size_t bytes = rt_wavefront_bytes_batch(b->npix, b->ntiles, b->nscenes);
-  if (hipMalloc(&b->ws, bytes) != hipSuccess || hipMalloc(&b->shade, shade_bytes) != hipSuccess) {
-    (void)hipGetLastError();
+  hipError_t ae = rt_dev_alloc(ctx, &b->ws, bytes);
+  if (ae == hipSuccess) b->ws_bytes = bytes;
+  if (ae == hipSuccess) ae = rt_dev_alloc(ctx, &b->shade, shade_bytes);
+  if (ae == hipSuccess) b->shade_bytes = shade_bytes;
+  if (ae != hipSuccess) {
     rt_batch_destroy(b);
     return rt_set_error(ctx, RT_ERR_OOM, "batch workspace of %zu + %zu bytes", bytes, shade_bytes);
   }
@@ -125,11 +129,12 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
     if (seen.insert(items[i].bvh).second) b->builds.push_back(i);
   std::vector<rt_batch_item> uniq;
   for (int k : b->builds) uniq.push_back(items[k]);
-  if (e == hipSuccess) e = rt_seg_build_create(uniq.data(), (int)uniq.size(), &b->seg);
+  if (e == hipSuccess) e = rt_seg_build_create(ctx, uniq.data(), (int)uniq.size(), &b->seg);
   (void)rt_ctx_overlap(ctx);  // created here, outside any graph capture of rt_batch_render
   if (e != hipSuccess) {
     rt_batch_destroy(b);
-    return rt_set_error(ctx, RT_ERR_DEVICE, "batch setup failed: %s", hipGetErrorString(e));
+    return rt_set_error(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_DEVICE, "batch setup failed: %s",
+                        hipGetErrorString(e));
   }
   *out = b;
   return RT_OK;

@@ -545,7 +545,7 @@ __device__ __forceinline__ void quantize4(const float (&nb)[6], const float (&lo
     if (ext > 0) {
       int ex2;
       frexp(ext / 255.0, &ex2);  // ext/255 < 2^ex2
-      ex2 = ex2 < -126 ? -126 : ex2;
+      ex2 = ex2 < -126 ? -126 : (ex2 > 127 ? 127 : ex2);  // an infinite extent leaves ex2 unspecified
       e = ex2 + 127;
     }
     ebits |= (uint32_t)e << (8 * a);
@@ -561,6 +561,11 @@ __device__ __forceinline__ void quantize4(const float (&nb)[6], const float (&lo
           qh = 0;
         } else {
           double l = floor((lo[a][j] - o) * inv), h = ceil((hi[a][j] - o) * inv);
+          // within [0, 255] before the adjustment (NaN from non-finite
+          // coordinates spans the whole node), so each loop runs at most
+          // 255 times whatever the box
+          l = l >= 0 ? (l > 255 ? 255 : l) : 0;
+          h = h <= 255 ? (h < 0 ? 0 : h) : 255;
           while (l > 0 && o + l * step > lo[a][j]) l -= 1;
           while (h < 255 && o + h * step < hi[a][j]) h += 1;
           ql = (uint32_t)(l < 0 ? 0 : (l > 255 ? 255 : l));
@@ -1218,7 +1223,7 @@ __device__ void quantize_children(const float* nb, int k, const float (&lo)[3][W
     if (ext > 0) {
       int ex2;
       frexp(ext / 255.0, &ex2);
-      ex2 = ex2 < -126 ? -126 : ex2;
+      ex2 = ex2 < -126 ? -126 : (ex2 > 127 ? 127 : ex2);
       e = ex2 + 127;
     }
     ebits |= (uint32_t)e << (8 * a);
@@ -1232,6 +1237,11 @@ __device__ void quantize_children(const float* nb, int k, const float (&lo)[3][W
           ql = qh = 0;
         } else {
           double l = floor((lo[a][j] - o) * inv), h = ceil((hi[a][j] - o) * inv);
+          // within [0, 255] before the adjustment (NaN from non-finite
+          // coordinates spans the whole node), so each loop runs at most
+          // 255 times whatever the box
+          l = l >= 0 ? (l > 255 ? 255 : l) : 0;
+          h = h <= 255 ? (h < 0 ? 0 : h) : 255;
           while (l > 0 && o + l * step > lo[a][j]) l -= 1;
           while (h < 255 && o + h * step < hi[a][j]) h += 1;
           ql = (uint32_t)(l < 0 ? 0 : (l > 255 ? 255 : l));
@@ -1701,18 +1711,19 @@ enum SegStage {
 struct rt_seg_build {
   int ns = 0, bits = 0, passes = 0;
   void* dev = nullptr;  // SegBvh[ns], then kSegStages tables of ns + 1 block offsets
+  size_t dev_bytes = 0;
   const SegBvh* segs = nullptr;
   const uint32_t* off[kSegStages] = {};
   uint32_t total[kSegStages] = {};
 };
 
-void rt_seg_build_destroy(rt_seg_build* p) {
+void rt_seg_build_destroy(rt_ctx* ctx, rt_seg_build* p) {
   if (!p) return;
-  if (p->dev) (void)hipFree(p->dev);
+  rt_dev_free(ctx, p->dev, p->dev_bytes);
   delete p;
 }
 
-hipError_t rt_seg_build_create(const rt_batch_item* items, int count, rt_seg_build** out) {
+hipError_t rt_seg_build_create(rt_ctx* ctx, const rt_batch_item* items, int count, rt_seg_build** out) {
   *out = nullptr;
   if (count < 1) return hipSuccess;
   const int bits = items[0].bvh->bits;
@@ -1789,12 +1800,13 @@ hipError_t rt_seg_build_create(const rt_batch_item* items, int count, rt_seg_bui
     p->passes = std::max(p->passes, top_passes(n));
   }
   const size_t seg_bytes = rt_align256(sizeof(SegBvh) * count);
-  hipError_t e = hipMalloc(&p->dev, seg_bytes + off.size() * sizeof(uint32_t));
+  hipError_t e = rt_dev_alloc(ctx, &p->dev, seg_bytes + off.size() * sizeof(uint32_t));
+  if (e == hipSuccess) p->dev_bytes = seg_bytes + off.size() * sizeof(uint32_t);
   if (e == hipSuccess) e = hipMemcpy(p->dev, segs.data(), sizeof(SegBvh) * count, hipMemcpyHostToDevice);
   if (e == hipSuccess)
     e = hipMemcpy((char*)p->dev + seg_bytes, off.data(), off.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
-    rt_seg_build_destroy(p);
+    rt_seg_build_destroy(ctx, p);
     return e;
   }
   p->segs = (const SegBvh*)p->dev;

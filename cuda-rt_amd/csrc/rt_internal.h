@@ -44,7 +44,16 @@ struct rt_ctx {
   rt_overlap overlap;
   bool overlap_ready = false;
   void* noalloc_scratch = nullptr;  // this device's copy of rt_render_noalloc's static scratch array
+  uint64_t dev_bytes = 0;           // device memory held by this context's objects (rt_dev_alloc)
+  int32_t fail_alloc_in = 0;        // > 0: the fail_alloc_in-th next rt_dev_alloc fails (rt_ctx_fail_alloc)
 };
+
+// Every device allocation of the library: hipMalloc, counted in
+// ctx->dev_bytes, failing with hipErrorOutOfMemory when the context's
+// fault-injection countdown reaches it. rt_dev_free takes the size that was
+// allocated.
+hipError_t rt_dev_alloc(rt_ctx* ctx, void** p, size_t bytes);
+void rt_dev_free(rt_ctx* ctx, void* p, size_t bytes);
 
 // The context's overlap stream and events, created on first use (nullptr if
 // that fails: the passes then run on one stream).
@@ -233,9 +242,9 @@ void rt_keep_only_layout(const rt_bvh* b, int layout);
 // binary records and the unquantized 4-wide and 8-wide nodes: use it only
 // while no BVH wants them (want_tnode, want_wide4, want_wide8).
 struct rt_seg_build;
-hipError_t rt_seg_build_create(const rt_batch_item* items, int count, rt_seg_build** out);
+hipError_t rt_seg_build_create(rt_ctx* ctx, const rt_batch_item* items, int count, rt_seg_build** out);
 hipError_t rt_seg_build_launch(const rt_seg_build* p, hipStream_t s);
-void rt_seg_build_destroy(rt_seg_build* p);
+void rt_seg_build_destroy(rt_ctx* ctx, rt_seg_build* p);
 hipError_t rt_launch_export(const rt_bvh* b, const rt_bvh_export_bufs* dst, hipStream_t s);
 
 // Launchers (rt_trace.hip).

@@ -95,7 +95,9 @@ __device__ __forceinline__ bool tri_test(const RayD& r, const float4* __restrict
   if (be < 0 || al + be > 1) return false;
   t = rt::dot(AC, Q) * inv;
   if ((double)fabsf((float)t) < kRayEps) return false;
-  return true;
+  // only a non-finite vertex or ray coordinate gets here with a NaN or
+  // infinite t, alpha or beta; such a triangle is not hit
+  return fabs(t) < INFINITY && al == al && be == be;
 }
 
 struct HitRec {

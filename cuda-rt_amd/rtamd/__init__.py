@@ -73,6 +73,7 @@ EXPORTED = [
     "rt_trace", "rt_render", "rt_gen_soup", "rt_gen_icosphere_counts", "rt_gen_icosphere", "rt_gen_jitter",
     "rt_ctx_set_kernel_timing", "rt_ctx_kernel_stats", "rt_render_noalloc", "rt_render_noalloc_max_bytes",
     "rt_bvh_create", "rt_bvh_compute_bounds", "rt_bvh_compute_morton", "rt_bvh_sort_morton", "rt_bvh_build_tree",
+    "rt_ctx_fail_alloc", "rt_ctx_device_bytes",
 ]
 
 
@@ -122,6 +123,8 @@ def lib():
         [c_f] * 6 + [c_i32]
     L.rt_gen_jitter.argtypes = [ctypes.c_int32, c_f, c_f, c_f, ctypes.c_double, ctypes.c_uint64, c_f, c_f, c_f]
     L.rt_ctx_set_kernel_timing.argtypes = [vp, ctypes.c_int32]
+    L.rt_ctx_fail_alloc.argtypes = [vp, ctypes.c_int32]
+    L.rt_ctx_device_bytes.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64)]
     L.rt_ctx_kernel_stats.argtypes = [vp, ctypes.POINTER(KernelStat)]
     _lib = L
     _bind_batch(L)
@@ -289,6 +292,16 @@ class Context:
     def check(self, st):
         if st != RT_OK:
             raise RtError(st, self.L.rt_last_error(self.h).decode())
+
+    def fail_alloc(self, k):
+        """Make the k-th device allocation from now on fail (k = 0 disarms; rt_ctx_fail_alloc)."""
+        self.check(self.L.rt_ctx_fail_alloc(self.h, int(k)))
+
+    def device_bytes(self):
+        """Device bytes this context's objects hold (rt_ctx_device_bytes)."""
+        out = ctypes.c_uint64()
+        self.check(self.L.rt_ctx_device_bytes(self.h, ctypes.byref(out)))
+        return int(out.value)
 
     def set_kernel_timing(self, enable):
         """Bracket each wavefront traversal launch with HIP events (rt_ctx_set_kernel_timing)."""

@@ -249,6 +249,21 @@ typedef struct {
 rt_status rt_ctx_set_kernel_timing(rt_ctx *ctx, int32_t enable);
 rt_status rt_ctx_kernel_stats(rt_ctx *ctx, rt_kernel_stat out[2]);
 
+/* Device memory accounting and allocation fault injection (no reference
+ * counterpart: the reference exits the process when a cudaMalloc fails,
+ * src/scene.cpp:71-81). Every device allocation the library makes for a
+ * context's objects (BVH arenas, render and differentiable-render
+ * workspaces, shading tables, batches, timers) goes through one allocator.
+ * rt_ctx_device_bytes gives the bytes those objects hold now.
+ * rt_ctx_fail_alloc(ctx, k) makes the k-th allocation from now on fail as a
+ * failed hipMalloc would (k = 0 disarms it): the call that makes it returns
+ * RT_ERR_OOM, frees what it had allocated, and leaves the context and its
+ * other objects usable. Under hipGraph capture no call allocates: one that
+ * would have to (a render with a larger image than the context's workspace
+ * holds) returns RT_ERR_INVALID and leaves the capture intact. */
+rt_status rt_ctx_fail_alloc(rt_ctx *ctx, int32_t k);
+rt_status rt_ctx_device_bytes(const rt_ctx *ctx, uint64_t *bytes);
+
 /* Supersampling resolve: AverageSuperSamplingKernel (kernels/scene.cu:127-155).
  * big_dev is a (width*ss) x (height*ss) image as rt_render writes it (3
  * floats per pixel, row-flipped); out_dev receives width x height. Each

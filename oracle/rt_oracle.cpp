@@ -209,6 +209,12 @@ static bool tri_hit(const orc_mesh *m, int32_t tri, V3 p, V3 d, Hit *h) {
   if (beta < 0 || alpha + beta > 1) return false;
   double t = dot(AC, Q) * inv;
   if ((double)fabsf((float)t) < kRayEps) return false;
+  // Deviation: a NaN or infinite t, alpha or beta (only non-finite vertex or
+  // ray coordinates give one) is not a hit. The reference accepts it
+  // (kernels/bvh.cu:330-345 compare against NaN), and keeps a NaN t only if
+  // it is the first hit its DFS finds (:394), so its answer depends on visit
+  // order.
+  if (!(fabs(t) < INFINITY && alpha == alpha && beta == beta)) return false;
   h->t = t;
   h->alpha = alpha;
   h->beta = beta;

@@ -291,6 +291,25 @@ def test_triangle_known_answers():
     assert not O.intersect_triangle(hm, 0, [0.25, 0.25, 5e-9], [0, 0, -1.0])[0]
 
 
+def test_triangle_with_non_finite_vertex_is_not_hit():
+    """A NaN vertex makes alpha, beta and t NaN; the reference's comparisons
+    let such a hit through (kernels/bvh.cu:330-345), the oracle (and the
+    GPU) reject it. A vertex at infinity either fails the reference's own
+    tests or gives a NaN: never a hit with a non-finite t."""
+    z = np.zeros(3, np.float32)
+    for bad in (np.nan, np.inf, -np.inf):
+        for k in range(3):
+            for axis in range(3):
+                v = [np.array([0, 1, 0], np.float32), np.array([0, 0, 1], np.float32), np.zeros(3, np.float32)]
+                v[axis][k] = bad
+                hm = rtamd.HostMesh(v[0], v[1], v[2], z, z, np.ones(3, np.float32), [0, 1, 2])
+                for o, d in (([0.25, 0.25, 1.0], [0, 0, -1.0]), ([0.1, 0.2, -3.0], [0.01, 0.02, 1.0])):
+                    ok, t, a, b, _ = O.intersect_triangle(hm, 0, o, d)
+                    assert not ok or (math.isfinite(t) and math.isfinite(a) and math.isfinite(b)), (bad, k, axis)
+                    if np.isnan(bad):
+                        assert not ok, (k, axis)
+
+
 def test_box_known_answers():
     lo, hi = [0, 0, 0], [1, 1, 1]
     assert O.box_hit(lo, hi, [0.5, 0.5, 5], [0, 0, -1.0])
