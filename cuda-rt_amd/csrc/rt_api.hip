@@ -72,19 +72,26 @@ hipError_t noalloc_scratch_address(int device, void** out) {
 
 }  // namespace
 
+static void destroy_overlap(rt_overlap& o) {
+  for (hipEvent_t e : {o.start, o.filled, o.shaded, o.done})
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : o.flushed)
+    if (e) (void)hipEventDestroy(e);
+  if (o.aux) (void)hipStreamDestroy(o.aux);
+  o = rt_overlap{};
+}
+
 const rt_overlap* rt_ctx_overlap(rt_ctx* ctx) {
   if (ctx->overlap_ready) return ctx->overlap.aux ? &ctx->overlap : nullptr;
   ctx->overlap_ready = true;
   rt_overlap& o = ctx->overlap;
   bool ok = hipStreamCreateWithFlags(&o.aux, hipStreamNonBlocking) == hipSuccess;
-  for (hipEvent_t* e : {&o.start, &o.filled, &o.shaded, &o.flushed[0], &o.flushed[1], &o.done})
+  for (hipEvent_t* e : {&o.start, &o.filled, &o.shaded, &o.done})
     ok = ok && hipEventCreateWithFlags(e, hipEventDisableTiming) == hipSuccess;
+  for (hipEvent_t& e : o.flushed) ok = ok && hipEventCreateWithFlags(&e, hipEventDisableTiming) == hipSuccess;
   if (!ok) {
     (void)hipGetLastError();
-    for (hipEvent_t e : {o.start, o.filled, o.shaded, o.flushed[0], o.flushed[1], o.done})
-      if (e) (void)hipEventDestroy(e);
-    if (o.aux) (void)hipStreamDestroy(o.aux);
-    o = rt_overlap{};
+    destroy_overlap(o);
     return nullptr;
   }
   return &o;
@@ -149,10 +156,7 @@ void rt_ctx_destroy(rt_ctx* ctx) {
   }
   for (hipEvent_t e : ctx->timer.pool) (void)hipEventDestroy(e);
   rt_dev_free(ctx, ctx->timer.dev_work, 6 * sizeof(unsigned long long));
-  rt_overlap& o = ctx->overlap;
-  for (hipEvent_t e : {o.start, o.filled, o.shaded, o.flushed[0], o.flushed[1], o.done})
-    if (e) (void)hipEventDestroy(e);
-  if (o.aux) (void)hipStreamDestroy(o.aux);
+  destroy_overlap(ctx->overlap);
   delete ctx;
 }
 

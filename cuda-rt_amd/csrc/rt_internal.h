@@ -23,11 +23,16 @@ struct rt_kernel_timer {
   unsigned long long* dev_work = nullptr;
 };
 
+// Wavefront renders keep up to this many samples' shadow work in flight
+// (sample slots; the renders use fewer, see rt_trace.hip).
+constexpr int kMaxSampleSlots = 6;
+
 // Second stream and events of a context's wavefront renders: a sample's
-// last any-hit pass runs there while the next sample is traced.
+// last any-hit pass runs there while the next samples are traced.
 struct rt_overlap {
   hipStream_t aux = nullptr;
-  hipEvent_t start = nullptr, filled = nullptr, shaded = nullptr, flushed[2] = {nullptr, nullptr}, done = nullptr;
+  hipEvent_t start = nullptr, filled = nullptr, shaded = nullptr, done = nullptr;
+  hipEvent_t flushed[kMaxSampleSlots] = {};
 };
 
 struct rt_ctx {

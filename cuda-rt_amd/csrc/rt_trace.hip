@@ -17,6 +17,7 @@
 // must return the first hit in DFS order, so it walks left child first.
 #include <hip/hip_runtime.h>
 
+#include "../../include/rt_debug.h"
 #include "rt_adjoint.h"
 #include "rt_internal.h"
 #include "rt_math.h"
@@ -1333,29 +1334,29 @@ struct WfState {
   uint32_t ntiles;
   uint32_t* rng;        // pixels
   double* color;        // pixels*3: sum over finished samples
-  double* acc;          // pixels*3: a sample's path sum (one buffer per sample parity)
+  double* acc;          // pixels*3: a sample's path sum (one buffer per sample slot)
   double* thr;          // pixels*3: current path throughput
   double* ray[2];       // cap*6: ray origin, direction per queue slot
   uint32_t* pix[2];     // cap: global pixel of each queue slot (~0 = unused slot)
   uint16_t* rscene[2];  // cap: scene of each queue slot
   int32_t* hpos;        // cap: closest hit leaf position or -1
   double* hit;          // cap*3: t, alpha, beta
-  // shadow queue of a sample parity: the last bounce's any-hit pass of
-  // sample k may still run while sample k+1 is traced
+  // shadow queue of a sample slot: the last bounce's any-hit pass of
+  // sample k may still run while the next samples are traced
   double* shq;          // cap*16: q, L, I1, ds, thr, fo
   uint32_t* spix;       // cap: global pixel of each shadow entry
   uint16_t* sscene;     // cap: scene of each shadow entry
   int32_t* spos;        // cap: any-hit leaf position or -1
   double* st;           // cap: any-hit t
-  uint32_t* scount;     // number of shadow entries (counts[2] or counts[3])
+  uint32_t* scount;     // number of shadow entries of the sample slot (counts[2 + slot])
   uint16_t* skey;       // cap: sort key of each shadow ray: (scene, direction cell)
   uint32_t* sperm;      // cap: shadow rays in key order
   uint32_t* shist;      // sbins: key counts (zero between sorts)
   uint32_t* sbase;      // sbins: key offsets
   uint32_t sbins;       // number of keys: kShadowCells for one scene, nscenes << scell_bits for a batch
   int scell_bits;       // bits of the cell index in a batch key
-  uint32_t* counts;     // [0],[1] ray queue sizes, [2],[3] shadow entries per parity, [8] stack overflows
-  uint32_t* fetch[2];   // kFetchWords each: closest-hit and any-hit fetch counters (the latter per parity)
+  uint32_t* counts;     // [0],[1] ray queue sizes, [2 + slot] shadow entries per sample slot, [8] stack overflows
+  uint32_t* fetch[2];   // kFetchWords each: closest-hit and any-hit fetch counters (the latter per slot)
   int2* rec;            // optional path record: (hit leaf, occluder code) per pixel, sample, bounce
   int rec_spp, rec_depth;
   double* color64;      // optional FP64 output (pixels*3, row-flipped like rgb, unclamped mean)
@@ -1764,7 +1765,7 @@ __global__ __launch_bounds__(kScanThreads) void k_sort_scan(WfState W) {
   for (int j = 0; j < per; ++j) {
     if (c0 + j < kShadowCells) {
       W.sbase[c0 + j] = run;
-      W.shist[c0 + j] = 0u;  // ready for the next sort of this parity
+      W.shist[c0 + j] = 0u;  // ready for the next sort of this slot
     }
     run += v[j];
   }
@@ -1849,7 +1850,7 @@ __global__ __launch_bounds__(kScanThreads) void k_msort_scan(WfState W) {
   for (uint32_t c = c0; c < c1; ++c) {
     const uint32_t v = W.shist[c];
     W.sbase[c] = run;
-    W.shist[c] = 0u;  // ready for the next sort of this parity
+    W.shist[c] = 0u;  // ready for the next sort of this slot
     run += v;
   }
 }
@@ -2108,90 +2109,142 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
 }
 
 // Workspace layout of a wavefront render over `nscenes` scenes with `npix`
-// pixels and `ntiles` tiles in total.
+// pixels and `ntiles` tiles in total: regions every sample shares, then one
+// block per sample slot. Sample k uses slot k % kSampleSlots; the last
+// bounce's shadow work of sample k (on the auxiliary stream) reads and
+// writes only its slot's block (and the atomic counter replicas), so sample
+// k + 1 .. k + kSampleSlots - 1 may run on the caller's stream meanwhile.
+// Each region is 256-byte aligned and disjoint from every other.
+#ifndef RT_SAMPLE_SLOTS
+#define RT_SAMPLE_SLOTS 2
+#endif
+constexpr int kSampleSlots = RT_SAMPLE_SLOTS;
+// scount = counts + 2 + slot stays below counts[8], the batch overflow count
+static_assert(kSampleSlots >= 2 && kSampleSlots <= kMaxSampleSlots && 2 + kMaxSampleSlots <= 8, "sample slots");
+
+enum WfShared {
+  kRScenes, kRViews, kRTileScene, kRRng, kRColor, kRThr, kRRay0, kRRay1, kRPix0, kRPix1, kRRscene0, kRRscene1,
+  kRHpos, kRHit, kRCounts, kRFetch0, kRRep, kRShared
+};
+enum WfSlot { kRAcc, kRShq, kRSpix, kRSscene, kRSpos, kRSt, kRFetch1, kRSkey, kRSperm, kRShist, kRSbase, kRSlot };
+
 struct WfLayout {
-  size_t off[40];
+  size_t shared[kRShared], shared_bytes[kRShared];
+  size_t slot[kMaxSampleSlots][kRSlot], slot_bytes[kRSlot];
   size_t total;
 };
 
-static WfLayout wf_layout(size_t npix, size_t ntiles, size_t nscenes) {
-  WfLayout L;
-  size_t cap = ntiles * kThreads;
+static WfLayout wf_layout(size_t npix, size_t ntiles, size_t nscenes, int nslots = kSampleSlots) {
+  WfLayout L{};
+  const size_t cap = ntiles * kThreads;
   const size_t bins = sort_bins(nscenes);
-  size_t sizes[] = {nscenes * sizeof(SceneDesc), nscenes * sizeof(BvhView), ntiles * 2, npix * 4, npix * 24,
-                    npix * 24, npix * 24, cap * 48, cap * 48, cap * 4, cap * 4, cap * 2, cap * 2, cap * 4,
-                    cap * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, 256, kFetchWords * 4, kFetchWords * 4,
-                    kCtrReplicas * kCtrStride * 8,
-                    // parity 1: accumulator, shadow queue, any-hit fetch counters
-                    npix * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, kFetchWords * 4,
-                    // shadow-ray sort, per parity: keys, order, cell counts, cell offsets
-                    cap * 2, cap * 4, bins * 4, bins * 4,
-                    cap * 2, cap * 4, bins * 4, bins * 4};
+  const size_t sh[kRShared] = {nscenes * sizeof(SceneDesc), nscenes * sizeof(BvhView), ntiles * 2, npix * 4,
+                               npix * 24, npix * 24, cap * 48, cap * 48, cap * 4, cap * 4, cap * 2, cap * 2,
+                               cap * 4, cap * 24, 256, kFetchWords * 4, kCtrReplicas * kCtrStride * 8};
+  const size_t sl[kRSlot] = {npix * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, kFetchWords * 4,
+                             cap * 2, cap * 4, bins * 4, bins * 4};
   size_t off = 0;
-  for (int k = 0; k < (int)(sizeof(sizes) / sizeof(sizes[0])); ++k) {
-    L.off[k] = off;
-    off += align256(sizes[k]);
+  for (int k = 0; k < kRShared; ++k) {
+    L.shared[k] = off;
+    L.shared_bytes[k] = sh[k];
+    off += align256(sh[k]);
   }
+  for (int k = 0; k < kRSlot; ++k) L.slot_bytes[k] = sl[k];
+  for (int j = 0; j < nslots && j < kMaxSampleSlots; ++j)
+    for (int k = 0; k < kRSlot; ++k) {
+      L.slot[j][k] = off;
+      off += align256(sl[k]);
+    }
   L.total = off;
   return L;
 }
 
-// The state as the passes of a sample of the given parity see it: the
-// accumulator, shadow queue, shadow count and any-hit fetch counters are
-// that parity's; everything else is shared.
-static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes, int parity = 0) {
-  WfLayout L = wf_layout(npix, ntiles, nscenes);
+// The state as the passes of a sample in `slot` see it: the slot's
+// accumulator, shadow queue, shadow count, any-hit fetch counters and
+// shadow-sort buffers; everything else is shared.
+static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes, int slot = 0,
+                        int nslots = kSampleSlots) {
+  const WfLayout L = wf_layout(npix, ntiles, nscenes, nslots);
   char* w = (char*)ws;
   WfState W;
-  int k = 0;
-  W.scenes = (const SceneDesc*)(w + L.off[k++]);
-  W.views = (const BvhView*)(w + L.off[k++]);
-  W.tile_scene = (const uint16_t*)(w + L.off[k++]);
+  W.scenes = (const SceneDesc*)(w + L.shared[kRScenes]);
+  W.views = (const BvhView*)(w + L.shared[kRViews]);
+  W.tile_scene = (const uint16_t*)(w + L.shared[kRTileScene]);
   W.ntiles = (uint32_t)ntiles;
-  W.rng = (uint32_t*)(w + L.off[k++]);
-  W.color = (double*)(w + L.off[k++]);
-  W.acc = (double*)(w + L.off[k++]);
-  W.thr = (double*)(w + L.off[k++]);
-  W.ray[0] = (double*)(w + L.off[k++]);
-  W.ray[1] = (double*)(w + L.off[k++]);
-  W.pix[0] = (uint32_t*)(w + L.off[k++]);
-  W.pix[1] = (uint32_t*)(w + L.off[k++]);
-  W.rscene[0] = (uint16_t*)(w + L.off[k++]);
-  W.rscene[1] = (uint16_t*)(w + L.off[k++]);
-  W.hpos = (int32_t*)(w + L.off[k++]);
-  W.hit = (double*)(w + L.off[k++]);
-  W.shq = (double*)(w + L.off[k++]);
-  W.spix = (uint32_t*)(w + L.off[k++]);
-  W.sscene = (uint16_t*)(w + L.off[k++]);
-  W.spos = (int32_t*)(w + L.off[k++]);
-  W.st = (double*)(w + L.off[k++]);
-  W.counts = (uint32_t*)(w + L.off[k++]);
-  W.fetch[0] = (uint32_t*)(w + L.off[k++]);
-  W.fetch[1] = (uint32_t*)(w + L.off[k++]);
-  W.rep = (unsigned long long*)(w + L.off[k++]);
-  W.scount = W.counts + 2;
-  if (parity) {
-    W.acc = (double*)(w + L.off[k++]);
-    W.shq = (double*)(w + L.off[k++]);
-    W.spix = (uint32_t*)(w + L.off[k++]);
-    W.sscene = (uint16_t*)(w + L.off[k++]);
-    W.spos = (int32_t*)(w + L.off[k++]);
-    W.st = (double*)(w + L.off[k++]);
-    W.fetch[1] = (uint32_t*)(w + L.off[k++]);
-    W.scount = W.counts + 3;
-  } else {
-    k += 7;
-  }
-  k += parity ? 4 : 0;
-  W.skey = (uint16_t*)(w + L.off[k++]);
-  W.sperm = (uint32_t*)(w + L.off[k++]);
-  W.shist = (uint32_t*)(w + L.off[k++]);
-  W.sbase = (uint32_t*)(w + L.off[k++]);
+  W.rng = (uint32_t*)(w + L.shared[kRRng]);
+  W.color = (double*)(w + L.shared[kRColor]);
+  W.thr = (double*)(w + L.shared[kRThr]);
+  W.ray[0] = (double*)(w + L.shared[kRRay0]);
+  W.ray[1] = (double*)(w + L.shared[kRRay1]);
+  W.pix[0] = (uint32_t*)(w + L.shared[kRPix0]);
+  W.pix[1] = (uint32_t*)(w + L.shared[kRPix1]);
+  W.rscene[0] = (uint16_t*)(w + L.shared[kRRscene0]);
+  W.rscene[1] = (uint16_t*)(w + L.shared[kRRscene1]);
+  W.hpos = (int32_t*)(w + L.shared[kRHpos]);
+  W.hit = (double*)(w + L.shared[kRHit]);
+  W.counts = (uint32_t*)(w + L.shared[kRCounts]);
+  W.fetch[0] = (uint32_t*)(w + L.shared[kRFetch0]);
+  W.rep = (unsigned long long*)(w + L.shared[kRRep]);
+  const size_t* S = L.slot[slot];
+  W.acc = (double*)(w + S[kRAcc]);
+  W.shq = (double*)(w + S[kRShq]);
+  W.spix = (uint32_t*)(w + S[kRSpix]);
+  W.sscene = (uint16_t*)(w + S[kRSscene]);
+  W.spos = (int32_t*)(w + S[kRSpos]);
+  W.st = (double*)(w + S[kRSt]);
+  W.fetch[1] = (uint32_t*)(w + S[kRFetch1]);
+  W.skey = (uint16_t*)(w + S[kRSkey]);
+  W.sperm = (uint32_t*)(w + S[kRSperm]);
+  W.shist = (uint32_t*)(w + S[kRShist]);
+  W.sbase = (uint32_t*)(w + S[kRSbase]);
+  W.scount = W.counts + 2 + slot;
   W.sbins = sort_bins(nscenes);
   W.scell_bits = batch_cell_bits(nscenes);
   W.npix = npix;
   W.cap = ntiles * kThreads;
   return W;
+}
+
+// The regions a sample slot's state addresses, for the layout checks of the
+// tests: each pointer of wf_state(slot) as an offset into the workspace,
+// with the size of the buffer it points to.
+extern "C" int32_t rt_wavefront_state_regions(uint64_t npix, uint64_t ntiles, uint64_t nscenes, int32_t nslots,
+                                              int32_t slot, rt_wf_region* out, int32_t max, uint64_t* total) {
+  if (nslots < 2 || nslots > kMaxSampleSlots || slot < 0 || slot >= nslots || nscenes < 1 || ntiles < 1) return -1;
+  const WfLayout L = wf_layout(npix, ntiles, nscenes, nslots);
+  if (total) *total = L.total;
+  // a base address far from 0, so every offset is a plain difference
+  char* base = (char*)(uintptr_t)(1ull << 40);
+  const WfState W = wf_state(base, npix, ntiles, nscenes, slot, nslots);
+  struct F {
+    const char* name;
+    const void* p;
+    size_t bytes;
+    int32_t shared;
+  };
+  const size_t* B = L.shared_bytes;
+  const size_t* Q = L.slot_bytes;
+  const F f[] = {
+      {"scenes", W.scenes, B[kRScenes], 1},    {"views", W.views, B[kRViews], 1},
+      {"tile_scene", W.tile_scene, B[kRTileScene], 1}, {"rng", W.rng, B[kRRng], 1},
+      {"color", W.color, B[kRColor], 1},       {"thr", W.thr, B[kRThr], 1},
+      {"ray0", W.ray[0], B[kRRay0], 1},        {"ray1", W.ray[1], B[kRRay1], 1},
+      {"pix0", W.pix[0], B[kRPix0], 1},        {"pix1", W.pix[1], B[kRPix1], 1},
+      {"rscene0", W.rscene[0], B[kRRscene0], 1}, {"rscene1", W.rscene[1], B[kRRscene1], 1},
+      {"hpos", W.hpos, B[kRHpos], 1},          {"hit", W.hit, B[kRHit], 1},
+      {"counts", W.counts, B[kRCounts], 1},    {"fetch0", W.fetch[0], B[kRFetch0], 1},
+      {"rep", W.rep, B[kRRep], 1},             {"acc", W.acc, Q[kRAcc], 0},
+      {"shq", W.shq, Q[kRShq], 0},             {"spix", W.spix, Q[kRSpix], 0},
+      {"sscene", W.sscene, Q[kRSscene], 0},    {"spos", W.spos, Q[kRSpos], 0},
+      {"st", W.st, Q[kRSt], 0},                {"fetch1", W.fetch[1], Q[kRFetch1], 0},
+      {"skey", W.skey, Q[kRSkey], 0},          {"sperm", W.sperm, Q[kRSperm], 0},
+      {"shist", W.shist, Q[kRShist], 0},       {"sbase", W.sbase, Q[kRSbase], 0},
+      {"scount", W.scount, sizeof(uint32_t), 0},
+  };
+  const int32_t n = (int32_t)(sizeof(f) / sizeof(f[0]));
+  for (int32_t k = 0; k < n && k < max && out; ++k)
+    out[k] = rt_wf_region{f[k].name, (uint64_t)((const char*)f[k].p - base), (uint64_t)f[k].bytes, f[k].shared};
+  return n;
 }
 
 uint32_t* rt_wavefront_counts(void* workspace, size_t npix, size_t ntiles, size_t nscenes) {
@@ -2250,9 +2303,10 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
                                uint32_t* overflow, hipStream_t s, int32_t* records, double* color64,
                                const void* single_scene, rt_kernel_timer* timer, const rt_overlap* ov) {
-  // Wp[k & 1] is the state sample k's passes see (its parity's accumulator
-  // and shadow queue).
-  WfState Wp[2] = {wf_state(workspace, npix, ntiles, nscenes, 0), wf_state(workspace, npix, ntiles, nscenes, 1)};
+  // Wp[k % kSampleSlots] is the state sample k's passes see (its slot's
+  // accumulator, shadow queue and shadow-sort buffers).
+  WfState Wp[kSampleSlots];
+  for (int j = 0; j < kSampleSlots; ++j) Wp[j] = wf_state(workspace, npix, ntiles, nscenes, j);
   for (WfState& W : Wp) {
     W.rec = (int2*)records;
     W.rec_spp = max_spp;
@@ -2281,13 +2335,15 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   const size_t cap = ntiles * kThreads;
   const int tiles = (int)ntiles;
   const bool work = cnt != nullptr;  // count work only when the caller asked for counters
-  // The last bounce's any-hit pass, shadow resolve and flush of sample k run
-  // on the auxiliary stream, overlapping sample k+1's camera, closest-hit
-  // and shading passes on s: they share no buffer but the replicated
-  // counters (atomics). Sample k+2 reuses sample k's parity buffers, so its
-  // camera pass waits for sample k's flush. The shading table is first read
-  // by sample 0's shading pass, so it is filled on the auxiliary stream while
-  // sample 0's camera and closest-hit passes run.
+  // The last bounce's shadow sort, any-hit pass, shadow resolve and flush of
+  // sample k run on the auxiliary stream, overlapping the passes of the
+  // next samples on s: they share no buffer but the replicated counters
+  // (atomics) and the colour sum, which only the flushes (all on the
+  // auxiliary stream, in sample order) write. Sample k + kSampleSlots
+  // reuses sample k's slot, so its camera pass waits for sample k's flush.
+  // The shading table is first read by sample 0's shading pass, so it is
+  // filled on the auxiliary stream while sample 0's camera and closest-hit
+  // passes run.
   hipStream_t sa = ov ? ov->aux : s;
   if (ov) {
     RT_TRY(hipEventRecord(ov->start, s));
@@ -2305,13 +2361,17 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     k_wf_init<true><<<tiles, kThreads, 0, s>>>(W0, D1);
   else
     k_wf_init<false><<<tiles, kThreads, 0, s>>>(W0, D1);
-  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W0.fetch[0], 2 * kFetchWords);
-  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[1].fetch[1], kFetchWords);
-  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[1].shist, Wp[1].sbins);
+  // every slot starts with zero fetch counters and an empty shadow-sort
+  // histogram (k_wf_init clears slot 0's; each sort clears its own after use)
+  k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, W0.fetch[0], kFetchWords);
+  for (int j = 0; j < kSampleSlots; ++j) {
+    k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[j].fetch[1], kFetchWords);
+    if (j > 0) k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, Wp[j].shist, Wp[j].sbins);
+  }
   for (int it = 0; it < max_spp; ++it) {
-    const int par = it & 1;
+    const int par = it % kSampleSlots;
     const WfState& W = Wp[par];
-    if (ov && it >= 2) RT_TRY(hipStreamWaitEvent(s, ov->flushed[par], 0));
+    if (ov && it >= kSampleSlots) RT_TRY(hipStreamWaitEvent(s, ov->flushed[par], 0));
     RT_WF(s, W, k_wf_camera, it, (uint32_t)cap);
     int cur = 0;
     for (int bounce = 0; bounce <= max_depth; ++bounce) {

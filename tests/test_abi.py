@@ -10,7 +10,7 @@ import pytest
 import rtamd
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADERS = ["rt_api.h", "rt_scenes.h", "rt_io.h"]
+HEADERS = ["rt_api.h", "rt_scenes.h", "rt_io.h", "rt_debug.h"]
 
 
 def declared_functions():
