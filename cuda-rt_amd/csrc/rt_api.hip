@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <string.h>
 
 #include <mutex>
 #include <new>
@@ -135,7 +136,11 @@ rt_status rt_ctx_create(int32_t device, rt_ctx** out) {
   if (!c) return RT_ERR_OOM;
   c->device = device;
   hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cus = prop.multiProcessorCount;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+    c->num_cus = prop.multiProcessorCount;
+    c->arch.assign(prop.gcnArchName, strnlen(prop.gcnArchName, sizeof(prop.gcnArchName)));
+    c->arch = c->arch.substr(0, c->arch.find(':'));  // gfx950, without the target features
+  }
   if (noalloc_scratch_address(device, &c->noalloc_scratch) != hipSuccess) {
     (void)hipGetLastError();
     delete c;
@@ -533,6 +538,134 @@ rt_status rt_render(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, co
   RT_HIP(ctx, rt_launch_render(bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, counters_dev, ws, shade,
                                (hipStream_t)stream, nullptr, nullptr, &ctx->timer,
                                shade && !(opts->flags & RT_RENDER_ONE_STREAM) ? rt_ctx_overlap(ctx) : nullptr));
+  return RT_OK;
+}
+
+rt_status rt_render_autotune(rt_ctx* ctx, const rt_bvh* bvh, const rt_mesh_view* mesh, const rt_material* mats_dev,
+                             int32_t nmat, const rt_light* light, const rt_camera* cam, const rt_render_opts* opts,
+                             float* rgb_dev, void* stream, rt_tuning* best, float* best_ms) {
+  if (!ctx) return RT_ERR_INVALID;
+  rt_status st = render_check(ctx, bvh, mesh, mats_dev, nmat, light, cam, opts, rgb_dev, "rt_render_autotune");
+  if (st != RT_OK) return st;
+  hipStream_t s = (hipStream_t)stream;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  RT_HIP(ctx, hipStreamIsCapturing(s, &cap));
+  if (cap != hipStreamCaptureStatusNone)
+    return rt_set_error(ctx, RT_ERR_INVALID, "rt_render_autotune times renders: not under graph capture");
+  rt_render_opts o = *opts;
+  o.flags |= RT_RENDER_WAVEFRONT;
+  const uint32_t key = rt_tuning_key(bvh->n, (int64_t)cam->width * cam->height);
+  const auto had = ctx->tuning.find(key);
+  const bool had_entry = had != ctx->tuning.end();
+  const rt_tuning before = had_entry ? had->second : rt_default_tuning();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  RT_HIP(ctx, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return rt_set_error(ctx, RT_ERR_DEVICE, "rt_render_autotune: event creation failed");
+  }
+  // one untimed render (fills the layout, warms the caches), then the
+  // faster of two timed ones
+  auto time = [&](const rt_tuning& t, float* ms) -> rt_status {
+    ctx->tuning[key] = t;
+    rt_status r = rt_render(ctx, bvh, mesh, mats_dev, nmat, light, cam, &o, rgb_dev, nullptr, stream);
+    *ms = -1.f;
+    for (int rep = 0; r == RT_OK && rep < 2; ++rep) {
+      float m = 0.f;
+      if (hipEventRecord(e0, s) != hipSuccess) return rt_set_error(ctx, RT_ERR_DEVICE, "autotune timing failed");
+      r = rt_render(ctx, bvh, mesh, mats_dev, nmat, light, cam, &o, rgb_dev, nullptr, stream);
+      if (r == RT_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&m, e0, e1) != hipSuccess))
+        return rt_set_error(ctx, RT_ERR_DEVICE, "autotune timing failed");
+      if (r == RT_OK && (*ms < 0.f || m < *ms)) *ms = m;
+    }
+    return r;
+  };
+  rt_tuning win{};
+  float win_ms = 0.f;
+  st = rt_autotune_descent(before, time, &win, &win_ms);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != RT_OK) {
+    if (had_entry)
+      ctx->tuning[key] = before;
+    else
+      ctx->tuning.erase(key);
+    return st;
+  }
+  ctx->tuning[key] = win;
+  // the trials filled every layout: builds and refits keep only the winner's
+  rt_keep_only_layout(bvh, win.node_layout);
+  if (best) *best = win;
+  if (best_ms) *best_ms = win_ms;
+  return RT_OK;
+}
+
+rt_status rt_ctx_get_tuning(const rt_ctx* ctx, int32_t ntris, int64_t nrays, rt_tuning* out, int32_t* found) {
+  if (!ctx || !out) return RT_ERR_INVALID;
+  auto it = ctx->tuning.find(rt_tuning_key(ntris, nrays));
+  if (found) *found = it != ctx->tuning.end();
+  *out = it != ctx->tuning.end() ? it->second : rt_default_tuning();
+  return RT_OK;
+}
+
+rt_status rt_ctx_set_tuning(rt_ctx* ctx, int32_t ntris, int64_t nrays, const rt_tuning* t) {
+  if (!ctx) return RT_ERR_INVALID;
+  const uint32_t key = rt_tuning_key(ntris, nrays);
+  if (!t) {
+    ctx->tuning.erase(key);
+    return RT_OK;
+  }
+  if (!rt_tuning_valid(t)) return rt_set_error(ctx, RT_ERR_INVALID, "invalid tuning");
+  ctx->tuning[key] = *t;
+  return RT_OK;
+}
+
+static const char kTuningHeader[] = "# rtamd tuning table v1: arch log2_tris log2_rays stack_ring fetch_chunk "
+                                    "waves_per_simd node_layout wg_threads";
+
+rt_status rt_ctx_save_tuning(const rt_ctx* ctx, const char* path) {
+  if (!ctx || !path) return RT_ERR_INVALID;
+  FILE* f = fopen(path, "w");
+  if (!f) return rt_set_error(const_cast<rt_ctx*>(ctx), RT_ERR_INVALID, "cannot write %s", path);
+  bool ok = fprintf(f, "%s\n", kTuningHeader) > 0;
+  for (const auto& kv : ctx->tuning) {
+    const rt_tuning& t = kv.second;
+    ok = ok && fprintf(f, "%s %u %u %d %d %d %d %d\n", ctx->arch.empty() ? "unknown" : ctx->arch.c_str(),
+                       kv.first >> 8, kv.first & 255u, t.stack_ring, t.fetch_chunk, t.waves_per_simd,
+                       t.node_layout, t.wg_threads ? t.wg_threads : 64) > 0;
+  }
+  ok = (fclose(f) == 0) && ok;
+  return ok ? RT_OK : rt_set_error(const_cast<rt_ctx*>(ctx), RT_ERR_INVALID, "cannot write %s", path);
+}
+
+rt_status rt_ctx_load_tuning(rt_ctx* ctx, const char* path) {
+  if (!ctx || !path) return RT_ERR_INVALID;
+  FILE* f = fopen(path, "r");
+  if (!f) return rt_set_error(ctx, RT_ERR_INVALID, "cannot open %s", path);
+  std::map<uint32_t, rt_tuning> got;
+  char line[512];
+  int lineno = 0;
+  rt_status st = RT_OK;
+  while (st == RT_OK && fgets(line, sizeof(line), f)) {
+    ++lineno;
+    const char* p = line;
+    while (*p == ' ' || *p == '\t') ++p;
+    if (*p == '#' || *p == '\n' || *p == '\r' || *p == '\0') continue;
+    char arch[64];
+    unsigned a = 0, r = 0;
+    rt_tuning t{};
+    if (sscanf(p, "%63s %u %u %d %d %d %d %d", arch, &a, &r, &t.stack_ring, &t.fetch_chunk, &t.waves_per_simd,
+               &t.node_layout, &t.wg_threads) != 8 ||
+        a > 255 || r > 255 || !rt_tuning_valid(&t)) {
+      st = rt_set_error(ctx, RT_ERR_INVALID, "%s:%d: not a tuning entry", path, lineno);
+      break;
+    }
+    if (ctx->arch == arch) got[a << 8 | r] = t;  // other architectures' entries are skipped
+  }
+  fclose(f);
+  if (st != RT_OK) return st;
+  for (const auto& kv : got) ctx->tuning[kv.first] = kv.second;
   return RT_OK;
 }
 

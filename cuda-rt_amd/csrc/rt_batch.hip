@@ -200,36 +200,36 @@ rt_status rt_batch_get_tuning(const rt_batch* b, rt_tuning* t) {
 rt_status rt_batch_autotune(rt_batch* b, void* stream, rt_tuning* best, float* best_ms) {
   if (!b) return RT_ERR_INVALID;
   hipStream_t s = (hipStream_t)stream;
-  const int layouts[] = {2, 4, 5, 8};
-  const int rings[] = {16, 32};
-  const int chunks[] = {16, 64};
-  const int waves[] = {4, 8};
-  hipEvent_t e0, e1;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  RT_HIP(b->ctx, hipStreamIsCapturing(s, &cap));
+  if (cap != hipStreamCaptureStatusNone)
+    return rt_set_error(b->ctx, RT_ERR_INVALID, "rt_batch_autotune times renders: not under graph capture");
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   RT_HIP(b->ctx, hipEventCreate(&e0));
-  RT_HIP(b->ctx, hipEventCreate(&e1));
-  rt_tuning keep = b->tuning, win = b->tuning;
-  float win_ms = -1.f;
-  rt_status st = RT_OK;
-  for (int l : layouts)
-  for (int r : rings)
-    for (int c : chunks)
-      for (int w : waves) {
-        rt_tuning t{r, c, w, l};
-        b->tuning = t;
-        if ((st = rt_batch_render(b, nullptr, stream)) != RT_OK) goto done;  // warm-up
-        if (hipEventRecord(e0, s) != hipSuccess || (st = rt_batch_render(b, nullptr, stream)) != RT_OK ||
-            hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
-          st = st != RT_OK ? st : rt_set_error(b->ctx, RT_ERR_DEVICE, "autotune timing failed");
-          goto done;
-        }
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (win_ms < 0.f || ms < win_ms) {
-          win_ms = ms;
-          win = t;
-        }
-      }
-done:
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return rt_set_error(b->ctx, RT_ERR_DEVICE, "rt_batch_autotune: event creation failed");
+  }
+  const rt_tuning keep = b->tuning;
+  // one untimed render (fills the layout), then the faster of two timed ones
+  auto time = [&](const rt_tuning& t, float* ms) -> rt_status {
+    b->tuning = t;
+    rt_status r = rt_batch_render(b, nullptr, stream);
+    *ms = -1.f;
+    for (int rep = 0; r == RT_OK && rep < 2; ++rep) {
+      float m = 0.f;
+      if (hipEventRecord(e0, s) != hipSuccess) return rt_set_error(b->ctx, RT_ERR_DEVICE, "autotune timing failed");
+      r = rt_batch_render(b, nullptr, stream);
+      if (r == RT_OK && (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                         hipEventElapsedTime(&m, e0, e1) != hipSuccess))
+        return rt_set_error(b->ctx, RT_ERR_DEVICE, "autotune timing failed");
+      if (r == RT_OK && (*ms < 0.f || m < *ms)) *ms = m;
+    }
+    return r;
+  };
+  rt_tuning win = keep;
+  float win_ms = 0.f;
+  rt_status st = rt_autotune_descent(keep, time, &win, &win_ms);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->tuning = st == RT_OK ? win : keep;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -49,6 +50,8 @@ struct rt_ctx {
   rt_overlap overlap;
   bool overlap_ready = false;
   void* noalloc_scratch = nullptr;  // this device's copy of rt_render_noalloc's static scratch array
+  std::string arch;                 // the device's architecture name (gfx950), the tuning table's key
+  std::map<uint32_t, rt_tuning> tuning;  // single-scene tunings by size bucket (rt_tuning_key)
   uint64_t dev_bytes = 0;           // device memory held by this context's objects (rt_dev_alloc)
   int32_t fail_alloc_in = 0;        // > 0: the fail_alloc_in-th next rt_dev_alloc fails (rt_ctx_fail_alloc)
 };
@@ -292,6 +295,68 @@ hipError_t rt_launch_adjoint(const rt_bvh* b, const rt_mesh_view* m, const doubl
 uint32_t* rt_wavefront_counts(void* workspace, size_t npix, size_t ntiles, size_t nscenes);
 rt_tuning rt_default_tuning();
 bool rt_tuning_valid(const rt_tuning* t);
+// Size bucket of a traversal: floor(log2) of the triangle count and of the
+// rays of one pass (a render's width * height), 8 bits each.
+uint32_t rt_tuning_key(int64_t ntris, int64_t nrays);
+// The context's tuning for that size, or the default.
+rt_tuning rt_ctx_tuning_for(const rt_ctx* ctx, int64_t ntris, int64_t nrays);
+
+// Autotuning by coordinate descent from `start`: each parameter in turn
+// (node layout, workgroup size, waves per SIMD, fetch chunk, stack ring)
+// takes every value with the others fixed, and the fastest is kept.
+// time(t, &ms) runs the work under t and reports its time.
+template <class TimeFn>
+rt_status rt_autotune_descent(rt_tuning start, TimeFn time, rt_tuning* best, float* best_ms) {
+  rt_tuning cur = start;
+  if (cur.wg_threads == 0) cur.wg_threads = 64;
+  float cur_ms = 0.f;
+  rt_status st = time(cur, &cur_ms);
+  if (st != RT_OK) return st;
+  const int layouts[] = {2, 4, 5, 8}, wgs[] = {64, 256}, waves[] = {2, 4, 8}, chunks[] = {16, 32, 64, 128},
+            rings[] = {16, 32};
+  auto try_value = [&](rt_tuning t) -> rt_status {
+    float ms = 0.f;
+    rt_status s = time(t, &ms);
+    if (s == RT_OK && ms < cur_ms) {
+      cur = t;
+      cur_ms = ms;
+    }
+    return s;
+  };
+  for (int v : layouts)
+    if (v != cur.node_layout) {
+      rt_tuning t = cur;
+      t.node_layout = v;
+      if ((st = try_value(t)) != RT_OK) return st;
+    }
+  for (int v : wgs)
+    if (v != cur.wg_threads) {
+      rt_tuning t = cur;
+      t.wg_threads = v;
+      if ((st = try_value(t)) != RT_OK) return st;
+    }
+  for (int v : waves)
+    if (v != cur.waves_per_simd) {
+      rt_tuning t = cur;
+      t.waves_per_simd = v;
+      if ((st = try_value(t)) != RT_OK) return st;
+    }
+  for (int v : chunks)
+    if (v != cur.fetch_chunk) {
+      rt_tuning t = cur;
+      t.fetch_chunk = v;
+      if ((st = try_value(t)) != RT_OK) return st;
+    }
+  for (int v : rings)
+    if (v != cur.stack_ring) {
+      rt_tuning t = cur;
+      t.stack_ring = v;
+      if ((st = try_value(t)) != RT_OK) return st;
+    }
+  *best = cur;
+  *best_ms = cur_ms;
+  return RT_OK;
+}
 
 // Offsets of the per-batch tables inside a wavefront workspace (first three
 // regions: scene table, BVH views, tile -> scene map).

@@ -603,24 +603,21 @@ __device__ __forceinline__ int child_keys(const float (&t0)[NW], const float (&t
 
 // LAYOUT 2: binary nodes, FP64 child tests; 4: 4-wide nodes (128 B);
 // 5: quantized 4-wide nodes (64 B).
-// Workgroups of the persistent traversal: one wave each, so a wave that has
-// finished its rays frees its slot at once (for the kernel that runs
-// concurrently on the other stream) instead of waiting for three others.
-#ifndef RT_TRACE_THREADS
-#define RT_TRACE_THREADS 64
-#endif
-constexpr int kTraceThreads = RT_TRACE_THREADS;
+// Workgroups of the persistent traversal (rt_tuning::wg_threads): one wave
+// by default, so a wave that has finished its rays frees its slot at once
+// (for the kernel that runs concurrently on the other stream) instead of
+// waiting for three others; four waves share one home ray segment.
 
 // LOW = false: a closest-hit launch with tmin 0, whose child test can leave
 // out the lower bound (t1 >= -inf is implied by t0 <= t1).
-template <bool ANY, int RING, int LAYOUT, bool MULTI, bool LOW = true>
-__global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhView* __restrict__ views,
+template <bool ANY, int RING, int LAYOUT, bool MULTI, bool LOW, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __restrict__ views,
                                                                const uint16_t* __restrict__ scene, RayIn R,
                                                                double tmin_phys, HitOut O, uint32_t* fetch_ctr,
                                                                uint32_t fetch_chunk, uint32_t* overflow,
                                                                unsigned long long* stats) {
   constexpr bool WIDE = LAYOUT != 2, QUANT = LAYOUT == 5 || LAYOUT == 8;
-  __shared__ int s_stack[RING * kTraceThreads];
+  __shared__ int s_stack[RING * THREADS];
   const uint32_t nrays = R.count_dev ? *R.count_dev : R.count;
   const uint32_t kFetchChunk = fetch_chunk;
   BvhView B = views[0];
@@ -777,7 +774,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
       }
       if (!done && node != kNoNode && node < 0 && pending == kNoNode) {
         pending = node;
-        node = stack_empty<!ANY>(S) ? kNoNode : stack_pop<RING, kTraceThreads, !ANY>(S, spill);
+        node = stack_empty<!ANY>(S) ? kNoNode : stack_pop<RING, THREADS, !ANY>(S, spill);
       }
       if (!done && node != kNoNode && node >= 0) {
         constexpr int NW = LAYOUT == 8 ? 8 : 4;
@@ -843,20 +840,20 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
               // past the m-1 kept entries are harmless.
               const int a0 = m >= 4 ? ref[3] : (m == 3 ? ref[2] : ref[1]);
               const int a1 = m >= 4 ? ref[2] : ref[1];
-              int* top = S.col + S.d * kTraceThreads;
+              int* top = S.col + S.d * THREADS;
               top[0] = a0;
-              top[kTraceThreads] = a1;
-              top[2 * kTraceThreads] = ref[1];
+              top[THREADS] = a1;
+              top[2 * THREADS] = ref[1];
               S.d += m > 1 ? m - 1 : 0;
             } else {
 #pragma unroll
               for (int k = NW - 1; k >= 1; --k)
-                if (m > k) ovf = ovf || !stack_push<RING, kTraceThreads, !ANY>(S, spill, ref[k]);
+                if (m > k) ovf = ovf || !stack_push<RING, THREADS, !ANY>(S, spill, ref[k]);
             }
           } else {
 #pragma unroll
             for (int k = NW - 1; k >= 1; --k)
-              if (m > k) ovf = ovf || !stack_push<RING, kTraceThreads, !ANY>(S, spill, ref[k]);
+              if (m > k) ovf = ovf || !stack_push<RING, THREADS, !ANY>(S, spill, ref[k]);
           }
           if (m >= 1) next = ref[0];
         } else {
@@ -869,7 +866,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
               const int ref = cref[j];
               if ((ref != kWideEmpty) & (t0[j] <= t1[j])) {
                 if (next != kNoNode) {
-                  S.col[(S.d & (RING - 1)) * kTraceThreads] = next;  // ring stack
+                  S.col[(S.d & (RING - 1)) * THREADS] = next;  // ring stack
                   ++S.d;
                 }
                 next = ref;
@@ -885,7 +882,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
                 p = wide_leaf_slab(r, w, j & 3, l0, l1);  // exact decision for a leaf child
               }
               if (p) {
-                if (next != kNoNode) ovf = ovf || !stack_push<RING, kTraceThreads, !ANY>(S, spill, next);
+                if (next != kNoNode) ovf = ovf || !stack_push<RING, THREADS, !ANY>(S, spill, next);
                 next = ref;
               }
             }
@@ -898,7 +895,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
         } else if (next != kNoNode) {
           node = next;
         } else {
-          node = stack_empty<!ANY>(S) ? kNoNode : stack_pop<RING, kTraceThreads, !ANY>(S, spill);
+          node = stack_empty<!ANY>(S) ? kNoNode : stack_pop<RING, THREADS, !ANY>(S, spill);
         }
       }
       if (!done && node == kNoNode && pending == kNoNode) done = true;
@@ -934,7 +931,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
       bpos = best.pos;
       if (hl && hr) {
         int nearc = (l0 <= r0) ? cl : cr, farc = (l0 <= r0) ? cr : cl;
-        if (!stack_push<RING, kTraceThreads, !ANY>(S, spill, farc)) {
+        if (!stack_push<RING, THREADS, !ANY>(S, spill, farc)) {
           atomicAdd(overflow, 1u);
           bpos = -1;
           done = true;
@@ -947,7 +944,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
       } else if (stack_empty<!ANY>(S)) {
         done = true;
       } else {
-        node = stack_pop<RING, kTraceThreads, !ANY>(S, spill);
+        node = stack_pop<RING, THREADS, !ANY>(S, spill);
       }
     } else {
       // node >= 0: internal node to enter; node < 0: leaf whose box passed.
@@ -960,7 +957,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
         } else if (stack_empty<!ANY>(S)) {
           done = true;
         } else {
-          node = stack_pop<RING, kTraceThreads, !ANY>(S, spill);
+          node = stack_pop<RING, THREADS, !ANY>(S, spill);
         }
       } else {
         float4 q0, q1, q2, q3;
@@ -971,7 +968,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
         bool hr = slab(r, q1.z, q1.w, q2.x, q2.y, q2.z, q2.w, r0, r1);
         int cl = __float_as_int(q3.x), cr = __float_as_int(q3.y);
         if (hl) {
-          if (hr && !stack_push<RING, kTraceThreads, !ANY>(S, spill, cr)) {
+          if (hr && !stack_push<RING, THREADS, !ANY>(S, spill, cr)) {
             atomicAdd(overflow, 1u);
             bpos = -1;
             done = true;
@@ -982,7 +979,7 @@ __global__ __launch_bounds__(kTraceThreads) void k_trace_persistent(const BvhVie
         } else if (stack_empty<!ANY>(S)) {
           done = true;
         } else {
-          node = stack_pop<RING, kTraceThreads, !ANY>(S, spill);
+          node = stack_pop<RING, THREADS, !ANY>(S, spill);
         }
       }
     }
@@ -2031,6 +2028,7 @@ rt_tuning rt_default_tuning() {
 #define RT_DEFAULT_LAYOUT 5
 #endif
   t.node_layout = RT_DEFAULT_LAYOUT;
+  t.wg_threads = 64;
   return t;
 }
 
@@ -2038,14 +2036,37 @@ bool rt_tuning_valid(const rt_tuning* t) {
   return t && (t->stack_ring == 16 || t->stack_ring == 32) &&
          (t->fetch_chunk == 16 || t->fetch_chunk == 32 || t->fetch_chunk == 64 || t->fetch_chunk == 128 ||
           t->fetch_chunk == 256 || t->fetch_chunk == 512) &&
-         t->waves_per_simd >= 1 && t->waves_per_simd <= 8 && (t->node_layout == 2 || t->node_layout == 4 || t->node_layout == 5 ||
-          t->node_layout == 8);
+         t->waves_per_simd >= 1 && t->waves_per_simd <= 8 &&
+         (t->node_layout == 2 || t->node_layout == 4 || t->node_layout == 5 || t->node_layout == 8) &&
+         (t->wg_threads == 0 || t->wg_threads == 64 || t->wg_threads == 256);
 }
 
-// waves_per_simd waves on each of a CU's 4 SIMDs, kTraceThreads per workgroup
+static uint32_t floor_log2(int64_t v) {
+  uint32_t k = 0;
+  while (v > 1 && k < 255) {
+    v >>= 1;
+    ++k;
+  }
+  return k;
+}
+
+uint32_t rt_tuning_key(int64_t ntris, int64_t nrays) { return floor_log2(ntris) << 8 | floor_log2(nrays); }
+
+rt_tuning rt_ctx_tuning_for(const rt_ctx* ctx, int64_t ntris, int64_t nrays) {
+  if (ctx && !ctx->tuning.empty()) {
+    auto it = ctx->tuning.find(rt_tuning_key(ntris, nrays));
+    if (it != ctx->tuning.end()) return it->second;
+  }
+  return rt_default_tuning();
+}
+
+static int trace_threads(const rt_tuning& t) { return t.wg_threads == 256 ? 256 : 64; }
+
+// waves_per_simd waves on each of a CU's 4 SIMDs, trace_threads(t) per workgroup
 static int persistent_grid(int num_cus, const rt_tuning& t, uint32_t max_rays) {
-  int cap = num_cus * t.waves_per_simd * (4 * 64 / kTraceThreads);
-  int need = (int)((max_rays + kTraceThreads - 1) / kTraceThreads);
+  const int threads = trace_threads(t);
+  int cap = num_cus * t.waves_per_simd * (4 * 64 / threads);
+  int need = (int)((max_rays + threads - 1) / threads);
   return need < cap ? (need > 0 ? need : 1) : cap;
 }
 
@@ -2056,23 +2077,30 @@ static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const
   const uint32_t fc = t.fetch_chunk;
   // the default layout has closest-hit kernels without the lower bound
   // for launches with tmin 0
-#define RT_LAUNCH(RINGV, LAYOUTV)                                                                                  \
+#define RT_LAUNCH_T(RINGV, LAYOUTV, TH)                                                                           \
   do {                                                                                                              \
     constexpr bool kNoLow = !ANY && LAYOUTV == 5;                                                                   \
     if (kNoLow && !(tmin > 0)) {                                                                                    \
       if (scene)                                                                                                    \
-        k_trace_persistent<ANY, RINGV, LAYOUTV, true, !kNoLow>                                                      \
-            <<<grid, kTraceThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                 \
+        k_trace_persistent<ANY, RINGV, LAYOUTV, true, !kNoLow, TH>                                                  \
+            <<<grid, TH, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                            \
       else                                                                                                          \
-        k_trace_persistent<ANY, RINGV, LAYOUTV, false, !kNoLow>                                                     \
-            <<<grid, kTraceThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                 \
+        k_trace_persistent<ANY, RINGV, LAYOUTV, false, !kNoLow, TH>                                                 \
+            <<<grid, TH, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                            \
     } else if (scene) {                                                                                             \
-      k_trace_persistent<ANY, RINGV, LAYOUTV, true>                                                                 \
-          <<<grid, kTraceThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                   \
+      k_trace_persistent<ANY, RINGV, LAYOUTV, true, true, TH>                                                       \
+          <<<grid, TH, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                              \
     } else {                                                                                                        \
-      k_trace_persistent<ANY, RINGV, LAYOUTV, false>                                                                \
-          <<<grid, kTraceThreads, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                   \
+      k_trace_persistent<ANY, RINGV, LAYOUTV, false, true, TH>                                                      \
+          <<<grid, TH, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                              \
     }                                                                                                               \
+  } while (0)
+#define RT_LAUNCH(RINGV, LAYOUTV)                   \
+  do {                                              \
+    if (trace_threads(t) == 256)                    \
+      RT_LAUNCH_T(RINGV, LAYOUTV, 256);             \
+    else                                            \
+      RT_LAUNCH_T(RINGV, LAYOUTV, 64);              \
   } while (0)
   if (t.node_layout == 8) {
     if (t.stack_ring == 16) RT_LAUNCH(16, 8); else RT_LAUNCH(32, 8);
@@ -2083,6 +2111,7 @@ static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const
   } else {
     if (t.stack_ring == 16) RT_LAUNCH(16, 2); else RT_LAUNCH(32, 2);
   }
+#undef RT_LAUNCH_T
 #undef RT_LAUNCH
 }
 
@@ -2097,7 +2126,7 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
   k_put_view<<<1, 1, 0, s>>>(view, b->view());
   RayIn R{rays->orig_dev, rays->dir_dev, 3, 1, nullptr, (uint32_t)rays->n, nullptr};
   HitOut O{nullptr, hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev, nullptr, 0};
-  rt_tuning t = rt_default_tuning();
+  const rt_tuning t = rt_ctx_tuning_for(b->ctx, b->n, rays->n);
   hipError_t e = rt_ensure_layout(b, t.node_layout, s);
   if (e != hipSuccess) return e;
   int grid = persistent_grid(b->ctx ? b->ctx->num_cus : 256, t, (uint32_t)rays->n);
@@ -2460,14 +2489,14 @@ hipError_t rt_launch_render(const rt_bvh* b, const rt_mesh_view* m, const rt_mat
   d.pix_off = 0;
   d.tile_off = 0;
   d.tri_off = 0;
-  hipError_t e = rt_ensure_layout(b, rt_default_tuning().node_layout, s);
+  const rt_tuning t = rt_ctx_tuning_for(b->ctx, b->n, (int64_t)P.W * P.H);
+  hipError_t e = rt_ensure_layout(b, t.node_layout, s);
   if (e != hipSuccess) return e;
   WfState W = wf_state(workspace, (size_t)P.W * P.H, (size_t)tiles, 1);
   k_wf_put_scene<<<cdiv(tiles, 256), 256, 0, s>>>((SceneDesc*)W.scenes, (BvhView*)W.views,
                                                   (uint16_t*)W.tile_scene, d, (uint32_t)tiles);
   return rt_launch_wavefront(workspace, (size_t)P.W * P.H, (size_t)tiles, 1, P.spp, P.depth,
-                             b->ctx ? b->ctx->num_cus : 256, nullptr, counters, ovf, s, records, color64, &d,
-                             timer, ov);
+                             b->ctx ? b->ctx->num_cus : 256, &t, counters, ovf, s, records, color64, &d, timer, ov);
 }
 
 // ------------------------------------------------------------ differentiable render

@@ -297,6 +297,22 @@ class Context:
         """Make the k-th device allocation from now on fail (k = 0 disarms; rt_ctx_fail_alloc)."""
         self.check(self.L.rt_ctx_fail_alloc(self.h, int(k)))
 
+    def get_tuning(self, ntris, nrays):
+        """(tuning, found) for that size bucket (rt_ctx_get_tuning)."""
+        t, found = Tuning(), ctypes.c_int32()
+        self.check(self.L.rt_ctx_get_tuning(self.h, int(ntris), int(nrays), ctypes.byref(t), ctypes.byref(found)))
+        return t, bool(found.value)
+
+    def set_tuning(self, ntris, nrays, t):
+        """Set (or with t=None remove) the entry for that size bucket (rt_ctx_set_tuning)."""
+        self.check(self.L.rt_ctx_set_tuning(self.h, int(ntris), int(nrays), None if t is None else ctypes.byref(t)))
+
+    def save_tuning(self, path):
+        self.check(self.L.rt_ctx_save_tuning(self.h, os.fsencode(path)))
+
+    def load_tuning(self, path):
+        self.check(self.L.rt_ctx_load_tuning(self.h, os.fsencode(path)))
+
     def device_bytes(self):
         """Device bytes this context's objects hold (rt_ctx_device_bytes)."""
         out = ctypes.c_uint64()
@@ -464,6 +480,21 @@ def render(ctx, bvh, mesh, mats_dev, nmat, light_, cam, opts, rgb=None, counters
     return rgb
 
 
+def render_autotune(ctx, bvh, mesh, mats_dev, nmat, light_, cam, opts, rgb=None, stream=None):
+    """rt_render_autotune: the fastest tuning for this scene's size, stored in
+    the context's table; returns (tuning, ms, image of the last trial)."""
+    torch = _torch()
+    if rgb is None:
+        rgb = torch.empty(cam.height * cam.width * 3, dtype=torch.float32, device="cuda")
+    _bind_batch(ctx.L)
+    mv = mesh.view()
+    best, ms = Tuning(), ctypes.c_float()
+    ctx.check(ctx.L.rt_render_autotune(ctx.h, bvh.h, ctypes.byref(mv), mats_dev.data_ptr(), nmat,
+                                       ctypes.byref(light_), ctypes.byref(cam), ctypes.byref(opts), rgb.data_ptr(),
+                                       _stream_handle(stream), ctypes.byref(best), ctypes.byref(ms)))
+    return best, float(ms.value), rgb
+
+
 def render_noalloc(ctx, bvh, mesh, mats_dev, nmat, light_, cam, opts, rgb, counters=None, stream=None):
     """rt_render_noalloc: the wavefront render on one stream with the library's
     static scratch (no device allocation; rgb must be given)."""
@@ -478,7 +509,7 @@ def render_noalloc(ctx, bvh, mesh, mats_dev, nmat, light_, cam, opts, rgb, count
 # ------------------------------------------------------------------ batches
 class Tuning(ctypes.Structure):
     _fields_ = [("stack_ring", ctypes.c_int32), ("fetch_chunk", ctypes.c_int32),
-                ("waves_per_simd", ctypes.c_int32), ("node_layout", ctypes.c_int32)]
+                ("waves_per_simd", ctypes.c_int32), ("node_layout", ctypes.c_int32), ("wg_threads", ctypes.c_int32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -498,6 +529,14 @@ def _bind_batch(L):
     L.rt_batch_set_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.rt_batch_get_tuning.argtypes = [vp, ctypes.POINTER(Tuning)]
     L.rt_batch_autotune.argtypes = [vp, vp, ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]
+    L.rt_render_autotune.argtypes = [vp, vp, ctypes.POINTER(MeshView), vp, ctypes.c_int32, ctypes.POINTER(Light),
+                                     ctypes.POINTER(Camera), ctypes.POINTER(RenderOpts), vp, vp,
+                                     ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_float)]
+    L.rt_ctx_get_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(Tuning),
+                                    ctypes.POINTER(ctypes.c_int32)]
+    L.rt_ctx_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(Tuning)]
+    L.rt_ctx_save_tuning.argtypes = [vp, ctypes.c_char_p]
+    L.rt_ctx_load_tuning.argtypes = [vp, ctypes.c_char_p]
     L.rt_batch_diagnostics.argtypes = [vp, vp, c_u32]
     L.rt_batch_destroy.argtypes = [vp]
     L._batch_bound = True
@@ -575,7 +614,8 @@ def write_bmp(path, width, height, rgb):
     if st != RT_OK:
         raise RtError(st, f"cannot write {path}")
 EXPORTED += ["rt_batch_create", "rt_batch_rebuild", "rt_batch_render", "rt_batch_set_tuning", "rt_batch_get_tuning",
-             "rt_batch_autotune", "rt_batch_diagnostics", "rt_batch_destroy"]
+             "rt_batch_autotune", "rt_batch_diagnostics", "rt_batch_destroy", "rt_render_autotune",
+             "rt_ctx_get_tuning", "rt_ctx_set_tuning", "rt_ctx_save_tuning", "rt_ctx_load_tuning"]
 
 
 class Batch:

@@ -232,6 +232,7 @@ rt_status rt_render_noalloc(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *
                             float *rgb_dev, uint64_t *counters_dev, void *stream);
 size_t rt_render_noalloc_max_bytes(void);
 
+
 /* Per-kernel timing for benchmarks (no reference counterpart). While
  * enabled, every closest-hit and any-hit traversal launch of rt_render and
  * rt_batch_render on this context is bracketed by HIP events on its stream,
@@ -289,6 +290,8 @@ typedef struct {
                               5: 4-wide nodes with 8-bit quantized child boxes, 64 B per node (default);
                               8: 8-wide nodes with 8-bit quantized child boxes, 128 B per node.
                               Every layout gives the same hits and images. */
+  int32_t wg_threads;      /* persistent traversal workgroup: 64 (one wave, the default) or 256
+                              (four waves); 0 means 64 */
 } rt_tuning;
 
 typedef struct {
@@ -315,12 +318,45 @@ rt_status rt_batch_rebuild(rt_batch *batch, void *stream);
 rt_status rt_batch_render(rt_batch *batch, uint64_t *counters_dev, void *stream);
 rt_status rt_batch_set_tuning(rt_batch *batch, const rt_tuning *tuning);
 rt_status rt_batch_get_tuning(const rt_batch *batch, rt_tuning *tuning);
-/* Time rt_batch_render under every candidate tuning (synchronises), keep
- * the fastest and return it and its time. Outputs do not depend on tuning. */
+/* Time rt_batch_render under candidate tunings (synchronises; not under
+ * graph capture), keep the fastest and return it and its time (the faster of
+ * two timed renders after an untimed one). The search is a coordinate
+ * descent from the batch's current tuning: node layout {2, 4, 5, 8}, then
+ * workgroup size {64, 256}, waves per SIMD {2, 4, 8}, fetch chunk {16, 32,
+ * 64, 128} and stack ring {16, 32}, each over its values with the others
+ * fixed. Outputs do not depend on tuning. */
 rt_status rt_batch_autotune(rt_batch *batch, void *stream, rt_tuning *best, float *best_ms);
 /* out[0] = traversal stack overflows during batch renders (0 when healthy). */
 rt_status rt_batch_diagnostics(const rt_batch *batch, void *stream, uint32_t out[1]);
 void rt_batch_destroy(rt_batch *batch);
+
+/* Single-scene tuning (no reference counterpart; the reference's launch
+ * shape is fixed, kernels/raytracer.cu:55-70). A context keeps a table of
+ * rt_tuning entries for its device architecture, keyed by the size of the
+ * work: floor(log2(triangles)) and floor(log2(rays per pass)), the rays per
+ * pass of a render being width * height. Wavefront rt_render and
+ * rt_render_noalloc, rt_trace and the differentiable render use the entry
+ * for their size, else the default tuning (layout 5, 64-thread workgroups).
+ * Images and hits do not depend on the tuning.
+ * rt_render_autotune times wavefront renders of this scene under candidate
+ * tunings (the coordinate descent of rt_batch_autotune, from the table's
+ * entry for this size or the default), stores the fastest in the table and
+ * returns it and its time; rgb_dev receives the last trial's image. It
+ * synchronises and is refused under graph capture. */
+rt_status rt_render_autotune(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_material *mats_dev,
+                             int32_t nmat, const rt_light *light, const rt_camera *cam, const rt_render_opts *opts,
+                             float *rgb_dev, void *stream, rt_tuning *best, float *best_ms);
+/* The entry for ntris triangles and nrays rays per pass (*found = 0 and the
+ * default tuning if there is none); set one (NULL removes it). */
+rt_status rt_ctx_get_tuning(const rt_ctx *ctx, int32_t ntris, int64_t nrays, rt_tuning *out, int32_t *found);
+rt_status rt_ctx_set_tuning(rt_ctx *ctx, int32_t ntris, int64_t nrays, const rt_tuning *tuning);
+/* Write the table as text, one line per entry (architecture, the two size
+ * buckets, the five parameters), replacing the file; read such a file and
+ * add its entries for this context's architecture (entries of other
+ * architectures are skipped; a malformed line or invalid tuning rejects the
+ * whole file with RT_ERR_INVALID, leaving the table as it was). */
+rt_status rt_ctx_save_tuning(const rt_ctx *ctx, const char *path);
+rt_status rt_ctx_load_tuning(rt_ctx *ctx, const char *path);
 
 /* ---- differentiable render (no reference counterpart: the reference
  * renders forward only, kernels/raytracer.cu:94-249). The scene is the

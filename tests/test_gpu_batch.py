@@ -169,8 +169,8 @@ def test_batch_outputs_do_not_depend_on_tuning(ctx):
     sc = _make(ctx, specs)
     batch = rtamd.Batch(ctx, sc)
     ref = None
-    for layout, ring, chunk, waves in itertools.product((2, 4, 5, 8), (16, 32), (16, 256), (1, 4, 8)):
-        t = rtamd.Tuning(ring, chunk, waves, layout)
+    for layout, ring, chunk, waves, wg in itertools.product((2, 4, 5, 8), (16, 32), (16, 256), (1, 4, 8), (64, 256)):
+        t = rtamd.Tuning(ring, chunk, waves, layout, wg)
         batch.set_tuning(t)
         for s in sc:
             s["rgb"].zero_()
