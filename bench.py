@@ -63,6 +63,9 @@ def parse():
     ap.add_argument("--one-stream", action="store_true",
                     help="soup1M: every render pass on one stream (for per-kernel profiles; same image)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="headline: keep the default tuning (by default rt_render_autotune picks the single-scene "
+                         "tuning before the timed steps)")
     ap.add_argument("--headline-only", action="store_true",
                     help="soup1M: skip the refit, batch and differentiable legs of the metric")
     ap.add_argument("--cpu-pixel-stride", type=int, default=1,
@@ -499,6 +502,11 @@ def main():
     stream = torch.cuda.current_stream()
     bvh = rtamd.BVH(ctx, dm, 63)
     rgb = torch.empty(args.width * args.height * 3, dtype=torch.float32, device=dev)
+    tuned = None
+    if args.autotune and args.mode == "wavefront":
+        # setup, not timed: the context keeps the pick for this scene's size
+        best, best_ms, _ = rtamd.render_autotune(ctx, bvh, dm, md, len(mats), light, cam, opts, rgb=rgb)
+        tuned = {"tuning": best.as_dict(), "autotune_render_ms": round(best_ms, 3)}
 
     def step(events=None, cnt=None):
         if events:
@@ -605,6 +613,7 @@ def main():
                    "tris": args.tris, "width": args.width, "height": args.height, "spp": args.spp,
                    "morton_bits": 63, "aperture": 0.05, "focus": 1.5},
         "render_mode": args.mode,
+        "render_tuning": tuned or {"tuning": "default"},
         "roofline": roof,
         "roofline_any_hit": roof_any,
         "roofline_isolated": roof_iso,
