@@ -251,7 +251,7 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   size_t o_k0 = take((size_t)n * ks), o_k1 = take((size_t)n * ks);
   size_t o_i0 = take((size_t)n * 4), o_i1 = take((size_t)n * 4);
   size_t o_par = take(nn * 4), o_l = take(ni * 4), o_r = take(ni * 4), o_f = take(ni * 4);
-  size_t o_nb = take(nn * 6 * 4), o_tn = take(ni * 64), o_lt = take((size_t)n * 16 * kLeafStride);
+  size_t o_nb = take(nn * 6 * 4), o_tn = take(ni * 64), o_lt = take(((size_t)n + 1) * 16 * kLeafStride);
   size_t o_w = take(ni * 128), o_wq = take(ni * 64), o_w8 = take(ni * 128), o_wf = take(ni);
   size_t o_lo = take((size_t)n * 4), o_def2 = take((size_t)n * 4), o_tr = take((size_t)n * 48);
   size_t o_wc = take(ni * 4), o_ws = take(((ni + 1023) / 1024 + 1) * 4), o_wsrc = take(ni * 16);
@@ -293,8 +293,11 @@ static rt_status alloc_bvh(rt_ctx* ctx, int32_t n, int32_t bits, int32_t flags, 
   b->wide8 = (float4*)(a + o_w8);
   b->wide_flag = (uint8_t*)(a + o_wf);
   b->leaf_of = (int32_t*)(a + o_lo);
-  // The diagnostic counters in misc[] accumulate over the BVH's lifetime.
+  // The diagnostic counters in misc[] accumulate over the BVH's lifetime;
+  // the null leaf (record n, the child of every unused wide-node slot) is a
+  // zero triangle that no build or refit writes.
   e = hipMemset(b->misc, 0, kMiscWords * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemset(b->ltri + kLeafStride * (size_t)n, 0, 16 * kLeafStride);
   if (e != hipSuccess) {
     rt_dev_free(ctx, p, off);
     delete b;

@@ -1127,7 +1127,7 @@ __device__ __forceinline__ void wide_fill_body(uint32_t bid, uint32_t nblk, int 
         lo[a][j] = __int_as_float(0x7f800000);   // +inf
         hi[a][j] = __int_as_float(0xff800000);   // -inf
       }
-      ref[j] = kWideEmpty;
+      ref[j] = kNullLeaf(n);
     }
   }
   const float4 refs =
@@ -1149,7 +1149,7 @@ __device__ __forceinline__ void wide_fill_body(uint32_t bid, uint32_t nblk, int 
   float4* q = wideq + 4 * (size_t)w;
   quantize4(nb, lo, hi, k, q);
   int cref[4];
-  for (int j = 0; j < 4; ++j) cref[j] = ref[j] >= 0 && ref[j] != kWideEmpty ? wcomp[ref[j]] : ref[j];
+  for (int j = 0; j < 4; ++j) cref[j] = ref[j] >= 0 ? wcomp[ref[j]] : ref[j];
   q[3] = make_float4(__int_as_float(cref[0]), __int_as_float(cref[1]), __int_as_float(cref[2]),
                      __int_as_float(cref[3]));
   if (wsrc) wsrc[w] = make_int4(k > 0 ? slot[0] : -1, k > 1 ? slot[1] : -1, k > 2 ? slot[2] : -1, k > 3 ? slot[3] : -1);
@@ -1293,7 +1293,7 @@ __global__ __launch_bounds__(256) void k_wide8_fill(int n, const int32_t* __rest
         lo[a][j] = 0.f;
         hi[a][j] = 0.f;
       }
-      ref[j] = kWideEmpty;
+      ref[j] = kNullLeaf(n);
     }
   }
   uint32_t ebits, qlo[3][2], qhi[3][2];

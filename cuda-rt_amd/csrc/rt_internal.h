@@ -74,7 +74,8 @@ struct BvhView {
   int32_t n;
   const float* node_box;   // (2n-1)*6: lo xyz, hi xyz
   const float4* tnode;     // (n-1)*4: child boxes and child refs (64 B per internal node)
-  const float4* ltri;      // n*kLeafStride: vertices a, b, c of the leaf's triangle + triangle id (48 B used)
+  const float4* ltri;      // (n+1)*kLeafStride: vertices a, b, c of the leaf's triangle + triangle id (48 B
+                           // used); record n is the null leaf (all zero, a degenerate triangle no ray hits)
   const float4* wide;      // (n-1)*8: 4-wide nodes (128 B), valid at even-depth internal nodes
   const float4* wideq;     // 8-bit quantized 4-wide nodes (64 B), packed: node k is the k-th even-depth node
   const float4* wide8;     // (n-1)*8: 8-wide quantized nodes (128 B), valid at depths divisible by 3
@@ -88,8 +89,11 @@ struct BvhView {
 // exact refit boxes:
 //   [0] lo.x of children 0..3   [1] lo.y   [2] lo.z
 //   [3] hi.x of children 0..3   [4] hi.y   [5] hi.z
-//   [6] child refs (int bits): >= 0 wide node, < 0 ~(sorted leaf position),
-//       kWideEmpty for an unused slot (its box is empty: lo = +inf, hi = -inf)
+//   [6] child refs (int bits): >= 0 wide node, < 0 ~(sorted leaf position);
+//       an unused slot names the null leaf ~n (kNullLeaf) and has an empty
+//       box (lo = +inf, hi = -inf), so the traversal needs no test for it:
+//       the box is never entered, and if a child test let it through, the
+//       null leaf's degenerate triangle fails the triangle test
 //   [7] unused
 // The quantized form (64 B, four float4) stores the node's box corner and,
 // per axis, a power-of-two step; child planes are lo = corner + qlo*step
@@ -98,6 +102,10 @@ struct BvhView {
 //   [0] corner.xyz, step.x (a float: a power of two, or 0 for a flat axis)
 //   [1] qlo of children 0..3 (byte j = child j) for x, y, z; step.y
 //   [2] qhi likewise; step.z
+//   An unused slot has qlo = 255 and qhi = 0 on every axis: an inverted box,
+//   which the conservative child test rejects whenever the widening E of an
+//   axis is below 255 steps (always, except for rays with coordinates of
+//   order 2^24 steps; the null leaf covers those)
 //   [3] child refs as in the 128-B form, except that a wide node is named
 //       by its packed slot: the quantized nodes are stored densely, the k-th
 //       even-depth internal node (in index order) at slot k
@@ -110,7 +118,9 @@ struct BvhView {
 //   [2] qlo.z of 0-3, 4-7, qhi.x of 0-3, 4-7
 //   [3] qhi.y of 0-3, 4-7, qhi.z of 0-3, 4-7
 //   [4] refs of children 0-3   [5] refs of children 4-7   [6], [7] unused
-constexpr int32_t kWideEmpty = 0x7fffffff;
+// The child ref of an unused slot of a BVH with n leaves: the null leaf, at
+// sorted position n (one record past the last leaf).
+__host__ __device__ constexpr int32_t kNullLeaf(int32_t n) { return ~n; }
 
 // Traversal record of internal node i (64 B, four float4):
 //   [0] = left.lo.xyz,  left.hi.x

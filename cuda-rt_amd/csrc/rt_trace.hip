@@ -585,7 +585,9 @@ __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uin
 // views[scene[i]] is the BVH of ray i (scene == nullptr: views[0] for all).
 // Closest-hit child keys: key = entry distance of a child the ray enters
 // within [low, lim], +inf otherwise; returns the number of such children.
-// LOW = false leaves out the test against low (for low = -inf).
+// LOW = false leaves out the test against low (for low = -inf). An unused
+// slot needs no test of its own: its box is inverted, and it names the
+// null leaf, which no ray hits.
 template <bool LOW, int NW>
 __device__ __forceinline__ int child_keys(const float (&t0)[NW], const float (&t1)[NW], const int (&cref)[NW],
                                           float lim, float low, float (&key)[NW], int (&ref)[NW]) {
@@ -593,7 +595,7 @@ __device__ __forceinline__ int child_keys(const float (&t0)[NW], const float (&t
 #pragma unroll
   for (int j = 0; j < NW; ++j) {
     ref[j] = cref[j];
-    bool h = (ref[j] != kWideEmpty) & (t0[j] <= t1[j]) & (t0[j] <= lim);
+    bool h = (t0[j] <= t1[j]) & (t0[j] <= lim);
     if (LOW) h = h & (t1[j] >= low);
     key[j] = h ? t0[j] : __int_as_float(0x7f800000);
     m += h ? 1 : 0;
@@ -803,11 +805,11 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
 #pragma unroll
           for (int j = 0; j < NW; ++j) cref[j] = __float_as_int(f4c(w[6], j & 3));
         }
-        // child-box tests of the filled slots only (an empty slot's box is
-        // empty and never entered); counted only when a caller asked
+        // child-box tests of the filled slots only (an unused slot names
+        // the null leaf); counted only when a caller asked
         if (stats) {
 #pragma unroll
-          for (int j = 0; j < NW; ++j) st.visits += cref[j] != kWideEmpty ? 1u : 0u;
+          for (int j = 0; j < NW; ++j) st.visits += cref[j] != kNullLeaf(n) ? 1u : 0u;
         }
         bool ovf = false;
         int next = kNoNode;
@@ -864,7 +866,7 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
 #pragma unroll
             for (int j = NW - 1; j >= 0; --j) {
               const int ref = cref[j];
-              if ((ref != kWideEmpty) & (t0[j] <= t1[j])) {
+              if (t0[j] <= t1[j]) {
                 if (next != kNoNode) {
                   S.col[(S.d & (RING - 1)) * THREADS] = next;  // ring stack
                   ++S.d;
@@ -876,7 +878,7 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
 #pragma unroll
             for (int j = NW - 1; j >= 0; --j) {
               const int ref = cref[j];
-              bool p = ref != kWideEmpty && t0[j] <= t1[j];
+              bool p = t0[j] <= t1[j];
               if (LAYOUT == 4 && p && ref < 0) {
                 double l0, l1;
                 p = wide_leaf_slab(r, w, j & 3, l0, l1);  // exact decision for a leaf child
