@@ -560,25 +560,52 @@ struct RayIn {
   const uint32_t* perm;       // optional fetch order: the k-th ray fetched is ray perm[k]
 };
 
+// The wavefront passes keep each traversal result in one record, written
+// with one or two 16-byte stores into a single 32-byte sector: the
+// closest-hit record of a queue slot holds the sorted leaf position (-1: no
+// hit) and t, alpha, beta; the any-hit record the leaf position and t. Split
+// over separate arrays, the results of rays that finish at different times
+// were written back as several partial sectors each.
+struct alignas(32) HitRec32 {
+  int32_t pos, pad;
+  double t, al, be;
+};
+struct alignas(16) HitRec16 {
+  int32_t pos, pad;
+  double t;
+};
+
 struct HitOut {
   int32_t* pos;  // sorted leaf position or -1 (nullable)
   int32_t* tri;  // triangle id or -1 (nullable)
   double *t, *al, *be;  // nullable
-  double* tab;   // nullable: t, alpha, beta of ray i at tab[i], tab[i + tab_comp], tab[i + 2*tab_comp]
-  size_t tab_comp;
+  HitRec32* rec32;      // nullable: the record of ray i at rec32[i]
+  HitRec16* rec16;      // nullable: the record of the k-th ray fetched at rec16[k]
 };
 
-__device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uint32_t i, int pos, double t, double al,
-                                          double be) {
+// Results of ray i, the k-th ray fetched (k = i without a fetch order).
+__device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uint32_t i, uint32_t k, int pos,
+                                          double t, double al, double be) {
   if (O.pos) O.pos[i] = pos;
   if (O.tri) O.tri[i] = pos >= 0 ? leaf_tri_of(B, pos) : -1;
   if (O.t) O.t[i] = pos >= 0 ? t : 0.0;
   if (O.al) O.al[i] = pos >= 0 ? al : 0.0;
   if (O.be) O.be[i] = pos >= 0 ? be : 0.0;
-  if (O.tab) {
-    O.tab[i] = pos >= 0 ? t : 0.0;
-    O.tab[i + O.tab_comp] = pos >= 0 ? al : 0.0;
-    O.tab[i + 2 * O.tab_comp] = pos >= 0 ? be : 0.0;
+  if (O.rec32) {
+    HitRec32 h;
+    h.pos = pos;
+    h.pad = 0;
+    h.t = pos >= 0 ? t : 0.0;
+    h.al = pos >= 0 ? al : 0.0;
+    h.be = pos >= 0 ? be : 0.0;
+    O.rec32[i] = h;
+  }
+  if (O.rec16) {
+    HitRec16 h;
+    h.pos = pos;
+    h.pad = 0;
+    h.t = pos >= 0 ? t : 0.0;
+    O.rec16[k] = h;
   }
 }
 
@@ -633,7 +660,7 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
   RayD r;
   RayF rf;
   bool active = false;
-  uint32_t rid = 0;
+  uint32_t rid = 0, kid = 0;  // the lane's ray and its fetch index
   int node = 0;
   int pending = kNoNode;  // 4-wide traversal: a leaf waiting for the wave's next batch of leaf tests
   double bt = 0, bal = 0, bbe = 0;
@@ -687,6 +714,7 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
         }
       }
       if (myid != 0xffffffffu) {
+        kid = myid;
         rid = R.perm ? R.perm[myid] : myid;
         if (MULTI) {
           B = views[scene[rid]];
@@ -703,13 +731,13 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
         double t0, t1;
         const float* rb = B.node_box;
         if (n <= 0 || !slab(r, ldgf(rb), ldgf(rb + 1), ldgf(rb + 2), ldgf(rb + 3), ldgf(rb + 4), ldgf(rb + 5), t0, t1)) {
-          write_hit(B, O, rid, -1, 0, 0, 0);
+          write_hit(B, O, rid, kid, -1, 0, 0, 0);
         } else if (n == 1) {
           double t, al, be;
           ++st.tests;
           bool ok = tri_test(r, B.ltri, t, al, be);
           if (!ANY && tmin_phys > 0 && !(t >= tmin_phys)) ok = false;
-          write_hit(B, O, rid, ok ? 0 : -1, t, al, be);
+          write_hit(B, O, rid, kid, ok ? 0 : -1, t, al, be);
         } else {
           active = true;
           node = 0;
@@ -986,7 +1014,7 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
       }
     }
     if (done) {
-      write_hit(B, O, rid, bpos, bt, bal, bbe);
+      write_hit(B, O, rid, kid, bpos, bt, bal, bbe);
       active = false;
     }
   }
@@ -1338,15 +1366,14 @@ struct WfState {
   double* ray[2];       // cap*6: ray origin, direction per queue slot
   uint32_t* pix[2];     // cap: global pixel of each queue slot (~0 = unused slot)
   uint16_t* rscene[2];  // cap: scene of each queue slot
-  int32_t* hpos;        // cap: closest hit leaf position or -1
-  double* hit;          // cap*3: t, alpha, beta
+  HitRec32* hrec;       // cap: closest-hit record of each queue slot
   // shadow queue of a sample slot: the last bounce's any-hit pass of
   // sample k may still run while the next samples are traced
   double* shq;          // cap*16: q, L, I1, ds, thr, fo
   uint32_t* spix;       // cap: global pixel of each shadow entry
   uint16_t* sscene;     // cap: scene of each shadow entry
-  int32_t* spos;        // cap: any-hit leaf position or -1
-  double* st;           // cap: any-hit t
+  HitRec16* srec;       // cap: any-hit records in fetch order (rank in key order when sorted)
+  uint32_t* srank;      // cap: rank of each shadow entry in key order (when sorted)
   uint32_t* scount;     // number of shadow entries of the sample slot (counts[2 + slot])
   uint16_t* skey;       // cap: sort key of each shadow ray: (scene, direction cell)
   uint32_t* sperm;      // cap: shadow rays in key order
@@ -1646,13 +1673,14 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
     sc = MULTI ? W.rscene[cur][i] : 0;
     const SceneDesc& D = scene_of<MULTI>(W, D1, sc);
     thr = ldv(W.thr, p, W.npix);
-    const int pos = W.hpos[i];
+    const HitRec32 hr = W.hrec[i];
+    const int pos = hr.pos;
     if (W.rec) W.rec[rec_index(W, p, sample, bounce)] = make_int2(pos, -1);
     if (pos < 0) {
       stv(W.acc, p, W.npix, ldv(W.acc, p, W.npix) + rt::hadamard(thr, mk(0.9, 0.9, 0.9)));
     } else {
       d3 o = ldv(W.ray[cur], i, W.cap), d = ldv(W.ray[cur] + 3 * W.cap, i, W.cap);
-      const double hh[3] = {W.hit[i], W.hit[W.cap + i], W.hit[2 * W.cap + i]};
+      const double hh[3] = {hr.t, hr.al, hr.be};
       uint32_t rng = W.rng[p];
       const LeafShade LS = leaf_shade(D.mats, D.P.nmat, D.M, D.B, pos);
       const rt_material& m = *LS.m;
@@ -1791,7 +1819,11 @@ __global__ __launch_bounds__(kThreads) void k_sort_scatter(WfState W) {
 #pragma unroll
   for (int j = 0; j < kSortPer; ++j) {
     const uint32_t i = i0 + j * kThreads;
-    if (i < n) W.sperm[h[key[j]] + rank[j]] = i;
+    if (i < n) {
+      const uint32_t r = h[key[j]] + rank[j];
+      W.sperm[r] = i;
+      W.srank[i] = r;
+    }
   }
 }
 
@@ -1863,7 +1895,11 @@ __global__ __launch_bounds__(kThreads) void k_msort_scatter(WfState W) {
     uint32_t base = 0;
     if (r.head) base = atomicAdd(&W.sbase[key], r.len);
     base = __shfl(base, (int)r.first, 64);
-    if (i < n) W.sperm[base + (__lane_id() - r.first)] = i;
+    if (i < n) {
+      const uint32_t rk = base + (__lane_id() - r.first);
+      W.sperm[rk] = i;
+      W.srank[i] = rk;
+    }
   }
 }
 
@@ -1889,9 +1925,10 @@ __global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, SceneD
   const double* e = W.shq + i;
   const size_t c = W.cap;
   d3 q = ldv(e, 0, c);
+  const HitRec16 r = W.srec[kShadowSort ? W.srank[i] : i];
   HitRec s;
-  s.pos = W.spos[i];
-  s.t = W.st[i];
+  s.pos = r.pos;
+  s.t = r.t;
   d3 I = shade_finish(D.P, ldv(e + 6 * c, 0, c), e[15 * c], ldv(e + 9 * c, 0, c),
                       shadow_factor(D.P, D.B, D.mats, D.M, q, s.pos >= 0, s));
   uint32_t p = W.spix[i];
@@ -2127,7 +2164,7 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
   k_wf_reset<<<1, 256, 0, s>>>(nullptr, 0u, fetch, kFetchWords);
   k_put_view<<<1, 1, 0, s>>>(view, b->view());
   RayIn R{rays->orig_dev, rays->dir_dev, 3, 1, nullptr, (uint32_t)rays->n, nullptr};
-  HitOut O{nullptr, hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev, nullptr, 0};
+  HitOut O{nullptr, hits->tri_dev, hits->t_dev, hits->alpha_dev, hits->beta_dev, nullptr, nullptr};
   const rt_tuning t = rt_ctx_tuning_for(b->ctx, b->n, rays->n);
   hipError_t e = rt_ensure_layout(b, t.node_layout, s);
   if (e != hipSuccess) return e;
@@ -2155,9 +2192,9 @@ static_assert(kSampleSlots >= 2 && kSampleSlots <= kMaxSampleSlots && 2 + kMaxSa
 
 enum WfShared {
   kRScenes, kRViews, kRTileScene, kRRng, kRColor, kRThr, kRRay0, kRRay1, kRPix0, kRPix1, kRRscene0, kRRscene1,
-  kRHpos, kRHit, kRCounts, kRFetch0, kRRep, kRShared
+  kRHrec, kRCounts, kRFetch0, kRRep, kRShared
 };
-enum WfSlot { kRAcc, kRShq, kRSpix, kRSscene, kRSpos, kRSt, kRFetch1, kRSkey, kRSperm, kRShist, kRSbase, kRSlot };
+enum WfSlot { kRAcc, kRShq, kRSpix, kRSscene, kRSrec, kRSrank, kRFetch1, kRSkey, kRSperm, kRShist, kRSbase, kRSlot };
 
 struct WfLayout {
   size_t shared[kRShared], shared_bytes[kRShared];
@@ -2171,9 +2208,9 @@ static WfLayout wf_layout(size_t npix, size_t ntiles, size_t nscenes, int nslots
   const size_t bins = sort_bins(nscenes);
   const size_t sh[kRShared] = {nscenes * sizeof(SceneDesc), nscenes * sizeof(BvhView), ntiles * 2, npix * 4,
                                npix * 24, npix * 24, cap * 48, cap * 48, cap * 4, cap * 4, cap * 2, cap * 2,
-                               cap * 4, cap * 24, 256, kFetchWords * 4, kCtrReplicas * kCtrStride * 8};
-  const size_t sl[kRSlot] = {npix * 24, cap * 128, cap * 4, cap * 2, cap * 4, cap * 8, kFetchWords * 4,
-                             cap * 2, cap * 4, bins * 4, bins * 4};
+                               cap * sizeof(HitRec32), 256, kFetchWords * 4, kCtrReplicas * kCtrStride * 8};
+  const size_t sl[kRSlot] = {npix * 24, cap * 128, cap * 4, cap * 2, cap * sizeof(HitRec16), cap * 4,
+                             kFetchWords * 4, cap * 2, cap * 4, bins * 4, bins * 4};
   size_t off = 0;
   for (int k = 0; k < kRShared; ++k) {
     L.shared[k] = off;
@@ -2211,8 +2248,7 @@ static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes, in
   W.pix[1] = (uint32_t*)(w + L.shared[kRPix1]);
   W.rscene[0] = (uint16_t*)(w + L.shared[kRRscene0]);
   W.rscene[1] = (uint16_t*)(w + L.shared[kRRscene1]);
-  W.hpos = (int32_t*)(w + L.shared[kRHpos]);
-  W.hit = (double*)(w + L.shared[kRHit]);
+  W.hrec = (HitRec32*)(w + L.shared[kRHrec]);
   W.counts = (uint32_t*)(w + L.shared[kRCounts]);
   W.fetch[0] = (uint32_t*)(w + L.shared[kRFetch0]);
   W.rep = (unsigned long long*)(w + L.shared[kRRep]);
@@ -2221,8 +2257,8 @@ static WfState wf_state(void* ws, size_t npix, size_t ntiles, size_t nscenes, in
   W.shq = (double*)(w + S[kRShq]);
   W.spix = (uint32_t*)(w + S[kRSpix]);
   W.sscene = (uint16_t*)(w + S[kRSscene]);
-  W.spos = (int32_t*)(w + S[kRSpos]);
-  W.st = (double*)(w + S[kRSt]);
+  W.srec = (HitRec16*)(w + S[kRSrec]);
+  W.srank = (uint32_t*)(w + S[kRSrank]);
   W.fetch[1] = (uint32_t*)(w + S[kRFetch1]);
   W.skey = (uint16_t*)(w + S[kRSkey]);
   W.sperm = (uint32_t*)(w + S[kRSperm]);
@@ -2262,12 +2298,12 @@ extern "C" int32_t rt_wavefront_state_regions(uint64_t npix, uint64_t ntiles, ui
       {"ray0", W.ray[0], B[kRRay0], 1},        {"ray1", W.ray[1], B[kRRay1], 1},
       {"pix0", W.pix[0], B[kRPix0], 1},        {"pix1", W.pix[1], B[kRPix1], 1},
       {"rscene0", W.rscene[0], B[kRRscene0], 1}, {"rscene1", W.rscene[1], B[kRRscene1], 1},
-      {"hpos", W.hpos, B[kRHpos], 1},          {"hit", W.hit, B[kRHit], 1},
+      {"hrec", W.hrec, B[kRHrec], 1},
       {"counts", W.counts, B[kRCounts], 1},    {"fetch0", W.fetch[0], B[kRFetch0], 1},
       {"rep", W.rep, B[kRRep], 1},             {"acc", W.acc, Q[kRAcc], 0},
       {"shq", W.shq, Q[kRShq], 0},             {"spix", W.spix, Q[kRSpix], 0},
-      {"sscene", W.sscene, Q[kRSscene], 0},    {"spos", W.spos, Q[kRSpos], 0},
-      {"st", W.st, Q[kRSt], 0},                {"fetch1", W.fetch[1], Q[kRFetch1], 0},
+      {"sscene", W.sscene, Q[kRSscene], 0},    {"srec", W.srec, Q[kRSrec], 0},
+      {"srank", W.srank, Q[kRSrank], 0},       {"fetch1", W.fetch[1], Q[kRFetch1], 0},
       {"skey", W.skey, Q[kRSkey], 0},          {"sperm", W.sperm, Q[kRSperm], 0},
       {"shist", W.shist, Q[kRShist], 0},       {"sbase", W.sbase, Q[kRSbase], 0},
       {"scount", W.scount, sizeof(uint32_t), 0},
@@ -2407,7 +2443,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     int cur = 0;
     for (int bounce = 0; bounce <= max_depth; ++bounce) {
       RayIn R{W.ray[cur], W.ray[cur] + 3 * cap, 1, cap, &W.counts[cur], 0, nullptr};
-      HitOut O{W.hpos, nullptr, nullptr, nullptr, nullptr, W.hit, cap};
+      HitOut O{nullptr, nullptr, nullptr, nullptr, nullptr, W.hrec, nullptr};
       // one scene: every ray uses views[0] (no per-ray scene lookup)
       if (T) RT_TRY(timer_mark(T, 0, true, s));
       launch_persistent<false>(t, grid, s, W.views, nscenes > 1 ? W.rscene[cur] : nullptr, R,
@@ -2436,7 +2472,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
         k_msort_scan<<<1, kScanThreads, 0, sb>>>(W);
         k_msort_scatter<<<sort_blocks, kThreads, 0, sb>>>(W);
       }
-      HitOut SO{W.spos, nullptr, W.st, nullptr, nullptr, nullptr, 0};
+      HitOut SO{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, W.srec};
       if (T) RT_TRY(timer_mark(T, 1, true, sb));
       launch_persistent<true>(t, grid, sb, W.views, nscenes > 1 ? W.sscene : nullptr, SR, 0.0, SO, W.fetch[1],
                               ovf, work ? W.rep + 5 : nullptr);

@@ -126,10 +126,10 @@ def schedule(spp, depth, nslots, slot_buffers=None):
         cur = 0
         for b in range(depth + 1):
             ray, nxt = f"ray{cur}", f"ray{cur ^ 1}"
-            op("s", f"closest {it}.{b}", [ray, "counts01", "views"], ["hpos", "hit", "fetch0", "rep", "ovf"])
+            op("s", f"closest {it}.{b}", [ray, "counts01", "views"], ["hrec", "fetch0", "rep", "ovf"])
             if it == 0 and b == 0:
                 ops.append(("wait", "s", "filled"))
-            op("s", f"shade {it}.{b}", ["hpos", "hit", ray, f"pix{cur}", f"rscene{cur}", "thr", "rng", "shade_table",
+            op("s", f"shade {it}.{b}", ["hrec", ray, f"pix{cur}", f"rscene{cur}", "thr", "rng", "shade_table",
                                          own("scount", it)],
                ["rng", "thr", own("acc", it), nxt, f"pix{cur ^ 1}", f"rscene{cur ^ 1}", "counts01", own("shq", it),
                 own("spix", it), own("sscene", it), own("scount", it), own("skey", it), "rep", f"rec{it}"])
@@ -140,10 +140,10 @@ def schedule(spp, depth, nslots, slot_buffers=None):
                 ops.append(("wait", "a", ("shaded", it)))
                 sb = "a"
             op(sb, f"sort {it}.{b}", [own("shq", it), own("scount", it), own("skey", it)],
-               [own("skey", it), own("shist", it), own("sbase", it), own("sperm", it)])
+               [own("skey", it), own("shist", it), own("sbase", it), own("sperm", it), own("srank", it)])
             op(sb, f"any {it}.{b}", [own("shq", it), own("sperm", it), own("scount", it), own("sscene", it), "views"],
-               [own("spos", it), own("st", it), own("fetch1", it), "rep", "ovf"])
-            op(sb, f"shadow finish {it}.{b}", [own("spos", it), own("st", it), own("shq", it), own("spix", it),
+               [own("srec", it), own("fetch1", it), "rep", "ovf"])
+            op(sb, f"shadow finish {it}.{b}", [own("srec", it), own("srank", it), own("shq", it), own("spix", it),
                                               own("sscene", it), own("scount", it)],
                [own("acc", it), "rep", f"rec{it}"])
             op(sb, f"reset shadow {it}.{b}", [], [own("scount", it), own("fetch1", it)])
