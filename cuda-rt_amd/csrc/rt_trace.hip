@@ -584,8 +584,29 @@ struct HitOut {
 };
 
 // Results of ray i, the k-th ray fetched (k = i without a fetch order).
+// WFO (the wavefront passes): only the record of the kernel's kind; the
+// other outputs are compiled out, so their pointers take no registers.
+template <bool ANY, bool WFO>
 __device__ __forceinline__ void write_hit(const BvhView& B, const HitOut& O, uint32_t i, uint32_t k, int pos,
                                           double t, double al, double be) {
+  if constexpr (WFO) {
+    if constexpr (ANY) {
+      HitRec16 h;
+      h.pos = pos;
+      h.pad = 0;
+      h.t = pos >= 0 ? t : 0.0;
+      O.rec16[k] = h;
+    } else {
+      HitRec32 h;
+      h.pos = pos;
+      h.pad = 0;
+      h.t = pos >= 0 ? t : 0.0;
+      h.al = pos >= 0 ? al : 0.0;
+      h.be = pos >= 0 ? be : 0.0;
+      O.rec32[i] = h;
+    }
+    return;
+  }
   if (O.pos) O.pos[i] = pos;
   if (O.tri) O.tri[i] = pos >= 0 ? leaf_tri_of(B, pos) : -1;
   if (O.t) O.t[i] = pos >= 0 ? t : 0.0;
@@ -639,7 +660,7 @@ __device__ __forceinline__ int child_keys(const float (&t0)[NW], const float (&t
 
 // LOW = false: a closest-hit launch with tmin 0, whose child test can leave
 // out the lower bound (t1 >= -inf is implied by t0 <= t1).
-template <bool ANY, int RING, int LAYOUT, bool MULTI, bool LOW, int THREADS>
+template <bool ANY, int RING, int LAYOUT, bool MULTI, bool LOW, int THREADS, bool WFO>
 __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __restrict__ views,
                                                                const uint16_t* __restrict__ scene, RayIn R,
                                                                double tmin_phys, HitOut O, uint32_t* fetch_ctr,
@@ -731,13 +752,13 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
         double t0, t1;
         const float* rb = B.node_box;
         if (n <= 0 || !slab(r, ldgf(rb), ldgf(rb + 1), ldgf(rb + 2), ldgf(rb + 3), ldgf(rb + 4), ldgf(rb + 5), t0, t1)) {
-          write_hit(B, O, rid, kid, -1, 0, 0, 0);
+          write_hit<ANY, WFO>(B, O, rid, kid, -1, 0, 0, 0);
         } else if (n == 1) {
           double t, al, be;
           ++st.tests;
           bool ok = tri_test(r, B.ltri, t, al, be);
           if (!ANY && tmin_phys > 0 && !(t >= tmin_phys)) ok = false;
-          write_hit(B, O, rid, kid, ok ? 0 : -1, t, al, be);
+          write_hit<ANY, WFO>(B, O, rid, kid, ok ? 0 : -1, t, al, be);
         } else {
           active = true;
           node = 0;
@@ -1014,7 +1035,7 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
       }
     }
     if (done) {
-      write_hit(B, O, rid, kid, bpos, bt, bal, bbe);
+      write_hit<ANY, WFO>(B, O, rid, kid, bpos, bt, bal, bbe);
       active = false;
     }
   }
@@ -2109,7 +2130,7 @@ static int persistent_grid(int num_cus, const rt_tuning& t, uint32_t max_rays) {
   return need < cap ? (need > 0 ? need : 1) : cap;
 }
 
-template <bool ANY>
+template <bool ANY, bool WFO>
 static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const BvhView* views,
                               const uint16_t* scene, RayIn R, double tmin, HitOut O, uint32_t* fetch,
                               uint32_t* overflow, unsigned long long* stats) {
@@ -2121,16 +2142,16 @@ static void launch_persistent(const rt_tuning& t, int grid, hipStream_t s, const
     constexpr bool kNoLow = !ANY && LAYOUTV == 5;                                                                   \
     if (kNoLow && !(tmin > 0)) {                                                                                    \
       if (scene)                                                                                                    \
-        k_trace_persistent<ANY, RINGV, LAYOUTV, true, !kNoLow, TH>                                                  \
+        k_trace_persistent<ANY, RINGV, LAYOUTV, true, !kNoLow, TH, WFO>                                                  \
             <<<grid, TH, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                            \
       else                                                                                                          \
-        k_trace_persistent<ANY, RINGV, LAYOUTV, false, !kNoLow, TH>                                                 \
+        k_trace_persistent<ANY, RINGV, LAYOUTV, false, !kNoLow, TH, WFO>                                                 \
             <<<grid, TH, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                            \
     } else if (scene) {                                                                                             \
-      k_trace_persistent<ANY, RINGV, LAYOUTV, true, true, TH>                                                       \
+      k_trace_persistent<ANY, RINGV, LAYOUTV, true, true, TH, WFO>                                                       \
           <<<grid, TH, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                              \
     } else {                                                                                                        \
-      k_trace_persistent<ANY, RINGV, LAYOUTV, false, true, TH>                                                      \
+      k_trace_persistent<ANY, RINGV, LAYOUTV, false, true, TH, WFO>                                                      \
           <<<grid, TH, 0, s>>>(views, scene, R, tmin, O, fetch, fc, overflow, stats);                              \
     }                                                                                                               \
   } while (0)
@@ -2170,9 +2191,9 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
   if (e != hipSuccess) return e;
   int grid = persistent_grid(b->ctx ? b->ctx->num_cus : 256, t, (uint32_t)rays->n);
   if (mode == RT_TRACE_ANY)
-    launch_persistent<true>(t, grid, s, view, nullptr, R, 0.0, O, fetch, b->misc + kMiscOverflow, nullptr);
+    launch_persistent<true, false>(t, grid, s, view, nullptr, R, 0.0, O, fetch, b->misc + kMiscOverflow, nullptr);
   else
-    launch_persistent<false>(t, grid, s, view, nullptr, R, tmin, O, fetch, b->misc + kMiscOverflow, nullptr);
+    launch_persistent<false, false>(t, grid, s, view, nullptr, R, tmin, O, fetch, b->misc + kMiscOverflow, nullptr);
   return hipGetLastError();
 }
 
@@ -2446,7 +2467,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
       HitOut O{nullptr, nullptr, nullptr, nullptr, nullptr, W.hrec, nullptr};
       // one scene: every ray uses views[0] (no per-ray scene lookup)
       if (T) RT_TRY(timer_mark(T, 0, true, s));
-      launch_persistent<false>(t, grid, s, W.views, nscenes > 1 ? W.rscene[cur] : nullptr, R,
+      launch_persistent<false, true>(t, grid, s, W.views, nscenes > 1 ? W.rscene[cur] : nullptr, R,
                                bounce == 0 ? 0.0 : kBounceTmin, O, W.fetch[0], ovf, work ? W.rep + 3 : nullptr);
       if (T) RT_TRY(timer_mark(T, 0, false, s));
       if (ov && it == 0 && bounce == 0) RT_TRY(hipStreamWaitEvent(s, ov->filled, 0));
@@ -2474,7 +2495,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
       }
       HitOut SO{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, W.srec};
       if (T) RT_TRY(timer_mark(T, 1, true, sb));
-      launch_persistent<true>(t, grid, sb, W.views, nscenes > 1 ? W.sscene : nullptr, SR, 0.0, SO, W.fetch[1],
+      launch_persistent<true, true>(t, grid, sb, W.views, nscenes > 1 ? W.sscene : nullptr, SR, 0.0, SO, W.fetch[1],
                               ovf, work ? W.rep + 5 : nullptr);
       if (T) RT_TRY(timer_mark(T, 1, false, sb));
       RT_WF(sb, W, k_wf_shadow_finish, it, bounce, cnt);
