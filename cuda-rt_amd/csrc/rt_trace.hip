@@ -434,6 +434,25 @@ __device__ __forceinline__ RayF make_rayf(const RayD& r, const float* root_box) 
 
 __device__ __forceinline__ float f4c(const float4& v, int j) { return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w; }
 
+// The mirror image of the conservative child test: with the near and far
+// offsets swapped, each fp32 entry distance is >= the reference's FP64 one
+// and each exit distance <= it (the same bound E on the same roundings, now
+// on the other side), so a box this test accepts is accepted by the FP64
+// slab test (inc/bbox.h:82-102) too. Used for a leaf about to be taken: it
+// is rejected only when the line passes within about E of the box's
+// boundary (for an axis with a zero direction component, when the origin
+// lies within 6 * 2^-24 * (B + |o|) of that slab's planes); the FP64 test
+// decides those.
+__device__ __forceinline__ bool inner_slab(const RayD& r, const RayF& f, float lx, float ly, float lz, float hx,
+                                           float hy, float hz) {
+  const float nx = r.sx ? lx : hx, fx = r.sx ? hx : lx;
+  const float ny = r.sy ? ly : hy, fy = r.sy ? hy : ly;
+  const float nz = r.sz ? lz : hz, fz = r.sz ? hz : lz;
+  const float t0 = fmaxf(fmaxf(fmaf(nx, f.ix, -f.fx), fmaf(ny, f.iy, -f.fy)), fmaf(nz, f.iz, -f.fz));
+  const float t1 = fminf(fminf(fmaf(fx, f.ix, -f.nx), fmaf(fy, f.iy, -f.ny)), fmaf(fz, f.iz, -f.nz));
+  return t0 <= t1;
+}
+
 __device__ __forceinline__ void load_wide(const float4* __restrict__ wide, int node, float4 (&w)[7]) {
   const float4* p = wide + 8 * (size_t)node;
 #pragma unroll
@@ -809,9 +828,13 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
             const float4 q0 = ldg4(L), q1 = ldg4(L + 1), q2 = ldg4(L + 2);
             const float ax = q0.x, ay = q0.y, az = q0.z, bx = q0.w, by = q1.x, bz = q1.y, cx = q1.z, cy = q1.w,
                         cz = q2.x;
-            double l0, l1;
-            take = slab(r, fminf(cx, fminf(ax, bx)), fminf(cy, fminf(ay, by)), fminf(cz, fminf(az, bz)),
-                        fmaxf(cx, fmaxf(ax, bx)), fmaxf(cy, fmaxf(ay, by)), fmaxf(cz, fmaxf(az, bz)), l0, l1);
+            const float lx = fminf(cx, fminf(ax, bx)), ly = fminf(cy, fminf(ay, by)), lz = fminf(cz, fminf(az, bz));
+            const float hx = fmaxf(cx, fmaxf(ax, bx)), hy = fmaxf(cy, fmaxf(ay, by)), hz = fmaxf(cz, fmaxf(az, bz));
+            // the FP64 test only where the fp32 inner test cannot decide
+            if (!inner_slab(r, rf, lx, ly, lz, hx, hy, hz)) {
+              double l0, l1;
+              take = slab(r, lx, ly, lz, hx, hy, hz, l0, l1);
+            }
           }
           if (take) {
             bt = t;
