@@ -223,8 +223,8 @@ rt_status rt_render(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, co
  * in a library-owned static device array of rt_render_noalloc_max_bytes()
  * (48 MiB) per GPU, found when the context is created. It calls no
  * hipMalloc/hipFree and creates no stream or event. A call whose workspace
- * (about 540 B per pixel) plus shading table (64 B per triangle) exceeds
- * the array returns RT_ERR_INVALID; 256 x 256 fits with up to 230,000
+ * (about 560 B per pixel) plus shading table (64 B per triangle) exceeds
+ * the array returns RT_ERR_INVALID; 256 x 256 fits with up to 210,000
  * triangles. Every call on one GPU shares that GPU's array: calls that
  * overlap on one GPU (other streams, other contexts, other threads) race. */
 rt_status rt_render_noalloc(rt_ctx *ctx, const rt_bvh *bvh, const rt_mesh_view *mesh, const rt_material *mats_dev,
