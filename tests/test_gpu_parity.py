@@ -273,6 +273,27 @@ def test_trace_from_inside(ctx):
     _trace_compare(ctx, hm, 63, orig, dirs)
 
 
+def test_trace_far_origins(ctx):
+    """Origins 10^3 .. 10^7 away from a small soup: the conservative fp32
+    child test then widens every box by E ~ 6 * 2^-24 * |o| / |d|, more than
+    the small nodes' extent, so the inverted boxes of unused wide-node slots
+    can pass and their null leaf (a zero triangle) is tested; the hits must
+    still equal the oracle's."""
+    _say("trace far origins")
+    hm = rtamd.gen_soup(3000, 13, edge_median=0.05)  # larger triangles: most rays hit one
+    rng = np.random.default_rng(41)
+    n = 4000
+    dist = 10.0 ** rng.uniform(3, 7, n)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    target = np.column_stack([rng.uniform(-.5, .5, n), rng.uniform(-.5, .5, n), rng.uniform(-2, -1, n)])
+    orig = target + u * dist[:, None]
+    dirs = target - orig
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    hits = _trace_compare(ctx, hm, 63, orig, dirs)
+    assert hits > 1000
+
+
 def test_trace_axis_aligned_and_tiny_directions(ctx):
     """Directions with exactly zero, tiny (fp32-denormal, FP64-denormal) and
     huge-ratio components, from origins on the ground quad's plane, on box
