@@ -404,16 +404,25 @@ __device__ __forceinline__ void flush_stats(unsigned long long* stats, const Tra
 // multiply-adds; a leaf child is decided by the FP64 test on its exact fp32
 // box, as in the binary traversal.
 struct RayF {
-  float ix, iy, iz;  // 1/d (+-1e30 for a zero component)
-  float nx, ny, nz;  // o/d + E: near-plane offsets
-  float fx, fy, fz;  // o/d - E: far-plane offsets
+  float ix, iy, iz;  // sc / d (sc * +-1e30 for a zero component)
+  float nx, ny, nz;  // sc * (o/d + E): near-plane offsets
+  float fx, fy, fz;  // sc * (o/d - E): far-plane offsets
+  float sx, sy, sz;  // 2^24 * sc / d: the plane scale of quantized nodes decoded as fp16 subnormals
+  float sc;          // 1, or the power of two 2^-k that keeps every step * 2^24 * sc / d finite
 };
+// Every fp32 distance of a ray is its unscaled value times sc, so the child
+// tests and the child order do not change: sc is a power of two, which
+// scales exactly in the normal range, and the widening E, scaled with the
+// rest, stays far above the subnormal range for any direction whose
+// nonzero components are within a factor 2^180 of each other. The pruning bounds
+// compared with these distances are scaled too (scaled_up / scaled_down).
 
-// a = 1/d of the axis (RayD's ax, ay or az)
-__device__ __forceinline__ void rayf_axis(double o, double d, double a, double bmax, float& iv, float& nn,
-                                          float& ff) {
-  double inv = a;
-  if (!(fabs(inv) <= 1e30)) inv = copysign(1e30, d);
+// a = 1/d of the axis (RayD's ax, ay or az), clamped to +-1e30
+__device__ __forceinline__ double clamped_inv(double a, double d) {
+  return fabs(a) <= 1e30 ? a : copysign(1e30, d);
+}
+
+__device__ __forceinline__ void rayf_axis(double o, double inv, double bmax, float& iv, float& nn, float& ff) {
   const double oi = o * inv;
   const double E = 6.0 * 5.9604644775390625e-08 * (bmax + fabs(o)) * fabs(inv);
   iv = (float)inv;
@@ -423,14 +432,53 @@ __device__ __forceinline__ void rayf_axis(double o, double d, double a, double b
 
 __device__ __forceinline__ RayF make_rayf(const RayD& r, const float* root_box) {
   double bmax = 0;
+  float b[6];
 #pragma unroll
-  for (int k = 0; k < 6; ++k) bmax = fmax(bmax, fabs((double)ldgf(root_box + k)));
+  for (int k = 0; k < 6; ++k) {
+    b[k] = ldgf(root_box + k);
+    bmax = fmax(bmax, fabs((double)b[k]));
+  }
+  const double ix = clamped_inv(r.ax, r.d.x), iy = clamped_inv(r.ay, r.d.y), iz = clamped_inv(r.az, r.d.z);
   RayF f;
-  rayf_axis(r.o.x, r.d.x, r.ax, bmax, f.ix, f.nx, f.fx);
-  rayf_axis(r.o.y, r.d.y, r.ay, bmax, f.iy, f.ny, f.fy);
-  rayf_axis(r.o.z, r.d.z, r.az, bmax, f.iz, f.nz, f.fz);
+  rayf_axis(r.o.x, ix, bmax, f.ix, f.nx, f.fx);
+  rayf_axis(r.o.y, iy, bmax, f.iy, f.ny, f.fy);
+  rayf_axis(r.o.z, iz, bmax, f.iz, f.nz, f.fz);
+  f.sc = 1.0f;
+  // A quantized node's step is the smallest power of two >= its extent /
+  // 255 (2^-126 at least), so at most 2 * extent / 255 above that, and no
+  // node is wider than the root: while max(extent * |1/d|) < 2^109, every
+  // step * 2^24 / d is below 2^127. Beyond it (a zero direction component
+  // in a scene a few thousand units wide) the ray's fp32 values are scaled
+  // down by sc (exactly: a power of two); a non-finite root box takes the
+  // smallest scale.
+  const double sd = fmax(fmax(((double)b[3] - b[0]) * fabs(ix), ((double)b[4] - b[1]) * fabs(iy)),
+                         ((double)b[5] - b[2]) * fabs(iz));
+  if (!(sd < 0x1p109)) {
+    // finite boxes give sd < 2^230 (extent < 2^129, |1/d| <= 1e30), so sc >= 2^-121
+    const float sc = sd < 0x1p230 ? ldexpf(1.0f, 108 - ilogb(sd)) : 0x1p-126f;
+    f.ix *= sc; f.iy *= sc; f.iz *= sc;
+    f.nx *= sc; f.ny *= sc; f.nz *= sc;
+    f.fx *= sc; f.fy *= sc; f.fz *= sc;
+    f.sc = sc;
+  }
+  f.sx = f.ix * 16777216.0f;  // exact: |sc / d| <= 1e30
+  f.sy = f.iy * 16777216.0f;
+  f.sz = f.iz * 16777216.0f;
   return f;
 }
+
+// Float bounds of a double, rounded outward.
+__device__ __forceinline__ float f_up(double x) {
+  float f = (float)x;
+  return (double)f < x ? __int_as_float(__float_as_int(f) + (f >= 0 ? 1 : -1)) : f;
+}
+__device__ __forceinline__ float f_down(double x) {
+  float f = (float)x;
+  return (double)f > x ? __int_as_float(__float_as_int(f) + (f > 0 ? -1 : 1)) : f;
+}
+// A distance bound in a ray's scaled fp32 units, rounded outward.
+__device__ __forceinline__ float scaled_up(const RayF& f, double x) { return f_up(x * (double)f.sc); }
+__device__ __forceinline__ float scaled_down(const RayF& f, double x) { return f_down(x * (double)f.sc); }
 
 __device__ __forceinline__ float f4c(const float4& v, int j) { return j == 0 ? v.x : j == 1 ? v.y : j == 2 ? v.z : v.w; }
 
@@ -474,11 +522,24 @@ __device__ __forceinline__ void wide_slabs(const RayD& r, const RayF& f, const f
   }
 }
 
+// fma((float)h, a, c) for the fp16 value h in the low (hi = 0) or high half
+// of h2: one v_fma_mix_f32.
+__device__ __forceinline__ float fma_h16(uint32_t h2, int hi, float a, float c) {
+  const _Float16 h = __builtin_bit_cast(_Float16, (uint16_t)(hi ? h2 >> 16 : h2 & 0xffffu));
+  return fmaf((float)h, a, c);
+}
+
 // The same distances from a quantized node: plane = corner + q * step,
-// folded into t = q * (step / d) + (corner / d - offset).
+// folded into t = q * (step / d) + (corner / d - offset). A v_perm_b32
+// spreads two plane bytes into two fp16 halves with a zero byte above each,
+// i.e. the fp16 subnormals q * 2^-24 (exact), which v_fma_mix_f32 takes
+// directly: fma(q * 2^-24, step * 2^24 / d, c) is fma(q, step / d, c) bit
+// for bit, since step * 2^24 / d is exact (step is a power of two) and the
+// product is the same real number -- one instruction per plane instead of a
+// byte conversion and an fma. (With the ray's scale sc, step * 2^24 * sc / d
+// is below 2^127, so it is finite.)
 __device__ __forceinline__ void wideq_slabs(const RayD& r, const RayF& f, const float4 (&q)[4], float (&t0)[4],
                                             float (&t1)[4]) {
-  const float ax = q[0].w * f.ix, ay = q[1].w * f.iy, az = q[2].w * f.iz;  // step / d
   const float cnx = fmaf(q[0].x, f.ix, -f.nx), cfx = fmaf(q[0].x, f.ix, -f.fx);
   const float cny = fmaf(q[0].y, f.iy, -f.ny), cfy = fmaf(q[0].y, f.iy, -f.fy);
   const float cnz = fmaf(q[0].z, f.iz, -f.nz), cfz = fmaf(q[0].z, f.iz, -f.fz);
@@ -487,13 +548,19 @@ __device__ __forceinline__ void wideq_slabs(const RayD& r, const RayF& f, const 
   const uint32_t nx = r.sx ? lx : hx, fx = r.sx ? hx : lx;
   const uint32_t ny = r.sy ? ly : hy, fy = r.sy ? hy : ly;
   const uint32_t nz = r.sz ? lz : hz, fz = r.sz ? hz : lz;
+  const float ax = q[0].w * f.sx, ay = q[1].w * f.sy, az = q[2].w * f.sz;  // step * 2^24 / d
+  const uint32_t h[6][2] = {
+      {__builtin_amdgcn_perm(nx, nx, 0x0C010C00u), __builtin_amdgcn_perm(nx, nx, 0x0C030C02u)},
+      {__builtin_amdgcn_perm(ny, ny, 0x0C010C00u), __builtin_amdgcn_perm(ny, ny, 0x0C030C02u)},
+      {__builtin_amdgcn_perm(nz, nz, 0x0C010C00u), __builtin_amdgcn_perm(nz, nz, 0x0C030C02u)},
+      {__builtin_amdgcn_perm(fx, fx, 0x0C010C00u), __builtin_amdgcn_perm(fx, fx, 0x0C030C02u)},
+      {__builtin_amdgcn_perm(fy, fy, 0x0C010C00u), __builtin_amdgcn_perm(fy, fy, 0x0C030C02u)},
+      {__builtin_amdgcn_perm(fz, fz, 0x0C010C00u), __builtin_amdgcn_perm(fz, fz, 0x0C030C02u)}};
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int sh = 8 * j;
-    const float a = fmaf((float)((nx >> sh) & 0xffu), ax, cnx), b = fmaf((float)((ny >> sh) & 0xffu), ay, cny),
-                c = fmaf((float)((nz >> sh) & 0xffu), az, cnz);
-    const float d = fmaf((float)((fx >> sh) & 0xffu), ax, cfx), e = fmaf((float)((fy >> sh) & 0xffu), ay, cfy),
-                g = fmaf((float)((fz >> sh) & 0xffu), az, cfz);
+    const int w = j >> 1, s = j & 1;
+    const float a = fma_h16(h[0][w], s, ax, cnx), b = fma_h16(h[1][w], s, ay, cny), c = fma_h16(h[2][w], s, az, cnz);
+    const float d = fma_h16(h[3][w], s, ax, cfx), e = fma_h16(h[4][w], s, ay, cfy), g = fma_h16(h[5][w], s, az, cfz);
     t0[j] = fmaxf(fmaxf(a, b), c);
     t1[j] = fminf(fminf(d, e), g);
   }
@@ -536,16 +603,6 @@ __device__ __forceinline__ void wide8_slabs(const RayF& f, const float4 (&q)[6],
 // The FP64 slab test of child j's exact box.
 __device__ __forceinline__ bool wide_leaf_slab(const RayD& r, const float4 (&w)[7], int j, double& t0, double& t1) {
   return slab(r, f4c(w[0], j), f4c(w[1], j), f4c(w[2], j), f4c(w[3], j), f4c(w[4], j), f4c(w[5], j), t0, t1);
-}
-
-// Float bounds of a double, rounded outward.
-__device__ __forceinline__ float f_up(double x) {
-  float f = (float)x;
-  return (double)f < x ? __int_as_float(__float_as_int(f) + (f >= 0 ? 1 : -1)) : f;
-}
-__device__ __forceinline__ float f_down(double x) {
-  float f = (float)x;
-  return (double)f > x ? __int_as_float(__float_as_int(f) + (f > 0 ? -1 : 1)) : f;
 }
 
 // ------------------------------------------------------------ persistent traversal
@@ -831,7 +888,8 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
             const float lx = fminf(cx, fminf(ax, bx)), ly = fminf(cy, fminf(ay, by)), lz = fminf(cz, fminf(az, bz));
             const float hx = fmaxf(cx, fmaxf(ax, bx)), hy = fmaxf(cy, fmaxf(ay, by)), hz = fmaxf(cz, fmaxf(az, bz));
             // the FP64 test only where the fp32 inner test cannot decide
-            if (!inner_slab(r, rf, lx, ly, lz, hx, hy, hz)) {
+            // (and for every ray whose fp32 distances are scaled)
+            if (rf.sc != 1.0f || !inner_slab(r, rf, lx, ly, lz, hx, hy, hz)) {
               double l0, l1;
               take = slab(r, lx, ly, lz, hx, hy, hz, l0, l1);
             }
@@ -842,7 +900,7 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
             bbe = be;
             bpos = pos;
             if (ANY) done = true;
-            else flim = f_up(bt + prune_margin(bt));
+            else flim = scaled_up(rf, bt + prune_margin(bt));
           }
         }
       }
@@ -888,7 +946,7 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
         if (!ANY) {
           const float lim = flim;
           const float low =
-              tmin_phys > 0 ? f_down(tmin_phys - prune_margin(tmin_phys)) : __int_as_float(0xff800000);
+              tmin_phys > 0 ? scaled_down(rf, tmin_phys - prune_margin(tmin_phys)) : __int_as_float(0xff800000);
           float key[NW];
           int ref[NW];
           const int m = child_keys<LOW>(t0, t1, cref, lim, low, key, ref);

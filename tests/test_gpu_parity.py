@@ -294,14 +294,20 @@ def test_trace_far_origins(ctx):
     assert hits > 1000
 
 
-def test_trace_axis_aligned_and_tiny_directions(ctx):
+@pytest.mark.parametrize("scale", [1.0, 5e4])
+def test_trace_axis_aligned_and_tiny_directions(ctx, scale):
     """Directions with exactly zero, tiny (fp32-denormal, FP64-denormal) and
     huge-ratio components, from origins on the ground quad's plane, on box
     planes of the sphere and inside the scene: the reference's 1/d is then
     +-inf or enormous and its slab terms +-inf or NaN, which the traversal's
-    fp32 child tests must never reject where the FP64 test accepts."""
-    _say("trace axis-aligned and tiny directions")
+    fp32 child tests must never reject where the FP64 test accepts. Scaled
+    by 5e4, the scene is wide enough that a ray with a zero direction
+    component decodes the quantized planes by byte conversions while the
+    other rays of its wave take the fp16 decode."""
+    _say(f"trace axis-aligned and tiny directions scale={scale}")
     hm = scenes.c1_mesh(16)
+    if scale != 1.0:
+        hm = rtamd.HostMesh(hm.vx * scale, hm.vy * scale, hm.vz * scale, hm.nx, hm.ny, hm.nz, hm.idx, hm.mat_id)
     rng = np.random.default_rng(21)
     base = []
     for ax in range(3):
@@ -323,7 +329,7 @@ def test_trace_axis_aligned_and_tiny_directions(ctx):
                np.array([0.0, 0.0, -5.0]), np.array([1.0, 0.0, -4.0]), np.array([-4.0, -1.0, -8.0])]
     for _ in range(30):
         origins.append(np.array([rng.uniform(-3, 3), rng.choice([-1.0, rng.uniform(-1, 1)]), rng.uniform(-7, -1)]))
-    orig = np.repeat(np.array(origins), len(base), axis=0)
+    orig = np.repeat(np.array(origins), len(base), axis=0) * scale
     dirs = np.tile(base, (len(origins), 1))
     hits = _trace_compare(ctx, hm, 30, orig, dirs)
     assert hits > 100
