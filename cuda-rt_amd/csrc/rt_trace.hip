@@ -1686,15 +1686,28 @@ __global__ __launch_bounds__(kThreads) void k_wf_init(WfState W, SceneDesc D1) {
 
 // Start sample k: slot = tile*256 + lane holds the pixel's primary ray.
 // Slots outside an image, or of a scene that has taken all its samples,
-// get a NaN ray, which misses the root box.
+// get a NaN ray, which misses the root box. flush_k >= 0: sample flush_k,
+// the last one that used this sample slot, has finished its shading; its
+// path sums go into the image here (color += acc) before the accumulator
+// is cleared for sample k, and the slot's shadow count and any-hit fetch
+// counters are cleared for this sample's shadow rays.
 template <bool MULTI>
-__global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, SceneDesc D1, int k, uint32_t nslots) {
+__global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, SceneDesc D1, int k, uint32_t nslots,
+                                                        int flush_k) {
   const size_t slot = (size_t)blockIdx.x * kThreads + threadIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0) W.counts[0] = nslots;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) W.counts[0] = nslots;
+    if (flush_k >= 0) {
+      if (threadIdx.x == 0) *W.scount = 0u;
+      for (uint32_t j = threadIdx.x; j < kFetchWords; j += kThreads) W.fetch[1][j] = 0u;
+    }
+  }
   int s, px, py;
   uint32_t gp;
   bool ok = tile_pixel<MULTI>(W, D1, blockIdx.x, threadIdx.x, s, px, py, gp);
   const RenderParams& P = scene_of<MULTI>(W, D1, s).P;
+  if (ok && flush_k >= 0 && flush_k < P.spp)
+    stv(W.color, gp, W.npix, ldv(W.color, gp, W.npix) + ldv(W.acc, gp, W.npix));
   double* rr = W.ray[0] + slot;
   W.rscene[0][slot] = (uint16_t)s;
   if (!ok || k >= P.spp) {
@@ -2079,7 +2092,8 @@ __global__ void k_wf_reset(uint32_t* counts, uint32_t mask, uint32_t* fetch, uin
     for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) fetch[k] = 0u;
 }
 
-// End of sample k: color += path sum for scenes still sampling.
+// End of one of the last samples (the others are added by the camera pass
+// that reuses their slot): color += path sum for scenes that took sample k.
 template <bool MULTI>
 __global__ __launch_bounds__(kThreads) void k_wf_flush(WfState W, SceneDesc D1, int k) {
   int s, px, py;
@@ -2541,7 +2555,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     const int par = it % kSampleSlots;
     const WfState& W = Wp[par];
     if (ov && it >= kSampleSlots) RT_TRY(hipStreamWaitEvent(s, ov->flushed[par], 0));
-    RT_WF(s, W, k_wf_camera, it, (uint32_t)cap);
+    RT_WF(s, W, k_wf_camera, it, (uint32_t)cap, it >= kSampleSlots ? it - kSampleSlots : -1);
     int cur = 0;
     for (int bounce = 0; bounce <= max_depth; ++bounce) {
       RayIn R{W.ray[cur], W.ray[cur] + 3 * cap, 1, cap, &W.counts[cur], 0, nullptr};
@@ -2580,16 +2594,21 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
                               ovf, work ? W.rep + 5 : nullptr);
       if (T) RT_TRY(timer_mark(T, 1, false, sb));
       RT_WF(sb, W, k_wf_shadow_finish, it, bounce, cnt);
-      k_wf_reset<<<1, 256, 0, sb>>>(W.counts, 1u << (2 + par), W.fetch[1], kFetchWords);
+      // the last bounce's shadow count and fetch counters are cleared by the
+      // camera pass of the next sample on this slot
+      if (bounce < max_depth) k_wf_reset<<<1, 256, 0, sb>>>(W.counts, 1u << (2 + par), W.fetch[1], kFetchWords);
       cur ^= 1;
     }
-    RT_WF(sa, W, k_wf_flush, it);
+    // sample it's path sums are complete: the camera pass of sample
+    // it + kSampleSlots adds them to the image
     if (ov) RT_TRY(hipEventRecord(ov->flushed[par], sa));
   }
   if (ov) {
     RT_TRY(hipEventRecord(ov->done, sa));
     RT_TRY(hipStreamWaitEvent(s, ov->done, 0));
   }
+  // the last samples' path sums (no later camera pass took them)
+  for (int it = std::max(0, max_spp - kSampleSlots); it < max_spp; ++it) RT_WF(s, Wp[it % kSampleSlots], k_wf_flush, it);
   if (multi)
     k_wf_resolve<true><<<tiles, kThreads, 0, s>>>(W0, D1);
   else

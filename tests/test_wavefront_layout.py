@@ -98,7 +98,10 @@ def schedule(spp, depth, nslots, slot_buffers=None):
     writes) plus ('record', event) / ('wait', event) markers, in enqueue
     order. slot_buffers(it) names the buffer set sample `it` uses (default
     it % nslots, as the code does); the wait before sample it's camera pass
-    is on the flush of sample it - nslots."""
+    is on the shadow finish of sample it - nslots, whose path sums that
+    camera pass adds to the image (and whose shadow count and any-hit fetch
+    counters it clears); the last nslots samples are added after the
+    auxiliary stream is done."""
     slot_of = slot_buffers or (lambda it: it % nslots)
     ops = []
 
@@ -122,7 +125,11 @@ def schedule(spp, depth, nslots, slot_buffers=None):
     for it in range(spp):
         if it >= nslots:
             ops.append(("wait", "s", ("flushed", it - nslots)))
-        op("s", f"camera {it}", ["rng"], ["rng", "ray0", "pix0", "rscene0", "thr", own("acc", it), "counts01"])
+            op("s", f"camera {it}", ["rng", own("acc", it), "color"],
+               ["rng", "ray0", "pix0", "rscene0", "thr", own("acc", it), "counts01", "color", own("scount", it),
+                own("fetch1", it)])
+        else:
+            op("s", f"camera {it}", ["rng"], ["rng", "ray0", "pix0", "rscene0", "thr", own("acc", it), "counts01"])
         cur = 0
         for b in range(depth + 1):
             ray, nxt = f"ray{cur}", f"ray{cur ^ 1}"
@@ -146,12 +153,14 @@ def schedule(spp, depth, nslots, slot_buffers=None):
             op(sb, f"shadow finish {it}.{b}", [own("srec", it), own("srank", it), own("shq", it), own("spix", it),
                                               own("sscene", it), own("scount", it)],
                [own("acc", it), "rep", f"rec{it}"])
-            op(sb, f"reset shadow {it}.{b}", [], [own("scount", it), own("fetch1", it)])
+            if b < depth:
+                op(sb, f"reset shadow {it}.{b}", [], [own("scount", it), own("fetch1", it)])
             cur ^= 1
-        op("a", f"flush {it}", [own("acc", it)], ["color"])
         ops.append(("record", "a", ("flushed", it)))
     ops.append(("record", "a", "done"))
     ops.append(("wait", "s", "done"))
+    for it in range(max(0, spp - nslots), spp):
+        op("s", f"flush {it}", [own("acc", it), "color"], ["color"])
     op("s", "resolve", ["color"], ["out"])
     return ops
 
