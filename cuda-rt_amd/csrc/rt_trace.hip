@@ -1686,28 +1686,16 @@ __global__ __launch_bounds__(kThreads) void k_wf_init(WfState W, SceneDesc D1) {
 
 // Start sample k: slot = tile*256 + lane holds the pixel's primary ray.
 // Slots outside an image, or of a scene that has taken all its samples,
-// get a NaN ray, which misses the root box. flush_k >= 0: sample flush_k,
-// the last one that used this sample slot, has finished its shading; its
-// path sums go into the image here (color += acc) before the accumulator
-// is cleared for sample k, and the slot's shadow count and any-hit fetch
-// counters are cleared for this sample's shadow rays.
+// get a NaN ray, which misses the root box. The sample's accumulator is
+// set by its first shading pass.
 template <bool MULTI>
-__global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, SceneDesc D1, int k, uint32_t nslots,
-                                                        int flush_k) {
+__global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, SceneDesc D1, int k, uint32_t nslots) {
   const size_t slot = (size_t)blockIdx.x * kThreads + threadIdx.x;
-  if (blockIdx.x == 0) {
-    if (threadIdx.x == 0) W.counts[0] = nslots;
-    if (flush_k >= 0) {
-      if (threadIdx.x == 0) *W.scount = 0u;
-      for (uint32_t j = threadIdx.x; j < kFetchWords; j += kThreads) W.fetch[1][j] = 0u;
-    }
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) W.counts[0] = nslots;
   int s, px, py;
   uint32_t gp;
   bool ok = tile_pixel<MULTI>(W, D1, blockIdx.x, threadIdx.x, s, px, py, gp);
   const RenderParams& P = scene_of<MULTI>(W, D1, s).P;
-  if (ok && flush_k >= 0 && flush_k < P.spp)
-    stv(W.color, gp, W.npix, ldv(W.color, gp, W.npix) + ldv(W.acc, gp, W.npix));
   double* rr = W.ray[0] + slot;
   W.rscene[0][slot] = (uint16_t)s;
   if (!ok || k >= P.spp) {
@@ -1726,7 +1714,6 @@ __global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, SceneDesc D1,
   stv(rr + 3 * W.cap, 0, W.cap, d);
   W.pix[0][slot] = gp;
   stv(W.thr, gp, W.npix, mk(1, 1, 1));
-  stv(W.acc, gp, W.npix, mk(0, 0, 0));
 }
 
 // Shading table of each scene (blockIdx.y), in triangle order so the
@@ -1769,9 +1756,13 @@ __global__ __launch_bounds__(256) void k_shade_fill(const SceneDesc* scenes, int
   }
 }
 
+// One bounce's shading. The first bounce of sample k also takes over the
+// slot's accumulator: with flush >= 0 the path sums still there (sample
+// k - slots, whose shading has finished) go into the image first; then
+// the accumulator starts from this pass's own terms.
 template <bool MULTI>
 __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, int bounce, int cur, int sample,
-                                                       unsigned long long* counters) {
+                                                       unsigned long long* counters, int flush) {
   __shared__ uint32_t s_tmp[8];
   const uint32_t n_in = W.counts[cur];
   const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
@@ -1791,6 +1782,10 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
     const HitRec32 hr = W.hrec[i];
     const int pos = hr.pos;
     if (W.rec) W.rec[rec_index(W, p, sample, bounce)] = make_int2(pos, -1);
+    if (bounce == 0) {
+      if (flush >= 0) stv(W.color, p, W.npix, ldv(W.color, p, W.npix) + ldv(W.acc, p, W.npix));
+      stv(W.acc, p, W.npix, mk(0, 0, 0));
+    }
     if (pos < 0) {
       stv(W.acc, p, W.npix, ldv(W.acc, p, W.npix) + rt::hadamard(thr, mk(0.9, 0.9, 0.9)));
     } else {
@@ -2092,15 +2087,24 @@ __global__ void k_wf_reset(uint32_t* counts, uint32_t mask, uint32_t* fetch, uin
     for (uint32_t k = threadIdx.x; k < words; k += blockDim.x) fetch[k] = 0u;
 }
 
-// End of one of the last samples (the others are added by the camera pass
-// that reuses their slot): color += path sum for scenes that took sample k.
+// The accumulators of the sample slots after the last sample.
+struct AccSlots {
+  double* acc[kMaxSampleSlots];
+};
+
+// End of the render: each slot's accumulator still holds the path sums of
+// the pixel's last sample on that slot (samples spp - slots .. spp - 1 of
+// its scene; earlier ones were added by the shading pass that reused the
+// slot), added here in sample order.
 template <bool MULTI>
-__global__ __launch_bounds__(kThreads) void k_wf_flush(WfState W, SceneDesc D1, int k) {
+__global__ __launch_bounds__(kThreads) void k_wf_flush_last(WfState W, SceneDesc D1, AccSlots A, int slots) {
   int s, px, py;
   uint32_t gp;
   if (!tile_pixel<MULTI>(W, D1, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
-  if (k >= scene_of<MULTI>(W, D1, s).P.spp) return;
-  stv(W.color, gp, W.npix, ldv(W.color, gp, W.npix) + ldv(W.acc, gp, W.npix));
+  const int spp = scene_of<MULTI>(W, D1, s).P.spp;
+  d3 c = ldv(W.color, gp, W.npix);
+  for (int k = spp > slots ? spp - slots : 0; k < spp; ++k) c = c + ldv(A.acc[k % slots], gp, W.npix);
+  stv(W.color, gp, W.npix, c);
 }
 
 template <bool MULTI>
@@ -2554,8 +2558,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   for (int it = 0; it < max_spp; ++it) {
     const int par = it % kSampleSlots;
     const WfState& W = Wp[par];
-    if (ov && it >= kSampleSlots) RT_TRY(hipStreamWaitEvent(s, ov->flushed[par], 0));
-    RT_WF(s, W, k_wf_camera, it, (uint32_t)cap, it >= kSampleSlots ? it - kSampleSlots : -1);
+    RT_WF(s, W, k_wf_camera, it, (uint32_t)cap);
     int cur = 0;
     for (int bounce = 0; bounce <= max_depth; ++bounce) {
       RayIn R{W.ray[cur], W.ray[cur] + 3 * cap, 1, cap, &W.counts[cur], 0, nullptr};
@@ -2566,7 +2569,14 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
                                bounce == 0 ? 0.0 : kBounceTmin, O, W.fetch[0], ovf, work ? W.rep + 3 : nullptr);
       if (T) RT_TRY(timer_mark(T, 0, false, s));
       if (ov && it == 0 && bounce == 0) RT_TRY(hipStreamWaitEvent(s, ov->filled, 0));
-      RT_WF(s, W, k_wf_shade, bounce, cur, it, cnt);
+      if (bounce == 0 && it >= kSampleSlots) {
+        // the slot's last sample has finished its shading on the auxiliary
+        // stream: its shadow count and any-hit fetch counters start over,
+        // and this pass adds its path sums to the image
+        if (ov) RT_TRY(hipStreamWaitEvent(s, ov->flushed[par], 0));
+        k_wf_reset<<<1, 256, 0, s>>>(W.counts, 1u << (2 + par), W.fetch[1], kFetchWords);
+      }
+      RT_WF(s, W, k_wf_shade, bounce, cur, it, cnt, bounce == 0 && it >= kSampleSlots ? it - kSampleSlots : -1);
       // the queue traced above is consumed: clear its count and the fetch counters
       k_wf_reset<<<1, 256, 0, s>>>(W.counts, 1u << cur, W.fetch[0], kFetchWords);
       // earlier bounces share the accumulator with the next bounce's shading
@@ -2594,12 +2604,12 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
                               ovf, work ? W.rep + 5 : nullptr);
       if (T) RT_TRY(timer_mark(T, 1, false, sb));
       RT_WF(sb, W, k_wf_shadow_finish, it, bounce, cnt);
-      // the last bounce's shadow count and fetch counters are cleared by the
-      // camera pass of the next sample on this slot
+      // the last bounce's shadow count and fetch counters are cleared before
+      // the next sample on this slot is shaded
       if (bounce < max_depth) k_wf_reset<<<1, 256, 0, sb>>>(W.counts, 1u << (2 + par), W.fetch[1], kFetchWords);
       cur ^= 1;
     }
-    // sample it's path sums are complete: the camera pass of sample
+    // sample it's path sums are complete: the first shading pass of sample
     // it + kSampleSlots adds them to the image
     if (ov) RT_TRY(hipEventRecord(ov->flushed[par], sa));
   }
@@ -2607,8 +2617,13 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     RT_TRY(hipEventRecord(ov->done, sa));
     RT_TRY(hipStreamWaitEvent(s, ov->done, 0));
   }
-  // the last samples' path sums (no later camera pass took them)
-  for (int it = std::max(0, max_spp - kSampleSlots); it < max_spp; ++it) RT_WF(s, Wp[it % kSampleSlots], k_wf_flush, it);
+  // every pixel's last samples (no later shading pass took them), in
+  // sample order
+  {
+    AccSlots A;
+    for (int j = 0; j < kSampleSlots; ++j) A.acc[j] = Wp[j].acc;
+    RT_WF(s, W0, k_wf_flush_last, A, kSampleSlots);
+  }
   if (multi)
     k_wf_resolve<true><<<tiles, kThreads, 0, s>>>(W0, D1);
   else

@@ -97,11 +97,11 @@ def schedule(spp, depth, nslots, slot_buffers=None):
     """The passes rt_launch_wavefront enqueues, as (stream, name, reads,
     writes) plus ('record', event) / ('wait', event) markers, in enqueue
     order. slot_buffers(it) names the buffer set sample `it` uses (default
-    it % nslots, as the code does); the wait before sample it's camera pass
-    is on the shadow finish of sample it - nslots, whose path sums that
-    camera pass adds to the image (and whose shadow count and any-hit fetch
-    counters it clears); the last nslots samples are added after the
-    auxiliary stream is done."""
+    it % nslots, as the code does); the wait before sample it's first
+    shading pass is on the shadow finish of sample it - nslots, whose
+    shadow count and any-hit fetch counters are then cleared and whose path
+    sums that shading pass adds to the image; the last nslots samples are
+    added after the auxiliary stream is done."""
     slot_of = slot_buffers or (lambda it: it % nslots)
     ops = []
 
@@ -123,23 +123,22 @@ def schedule(spp, depth, nslots, slot_buffers=None):
             op("s", f"reset shist[{j}]", [], [f"shist[{j}]"])
     flushed = {}
     for it in range(spp):
-        if it >= nslots:
-            ops.append(("wait", "s", ("flushed", it - nslots)))
-            op("s", f"camera {it}", ["rng", own("acc", it), "color"],
-               ["rng", "ray0", "pix0", "rscene0", "thr", own("acc", it), "counts01", "color", own("scount", it),
-                own("fetch1", it)])
-        else:
-            op("s", f"camera {it}", ["rng"], ["rng", "ray0", "pix0", "rscene0", "thr", own("acc", it), "counts01"])
+        op("s", f"camera {it}", ["rng"], ["rng", "ray0", "pix0", "rscene0", "thr", "counts01"])
         cur = 0
         for b in range(depth + 1):
             ray, nxt = f"ray{cur}", f"ray{cur ^ 1}"
             op("s", f"closest {it}.{b}", [ray, "counts01", "views"], ["hrec", "fetch0", "rep", "ovf"])
             if it == 0 and b == 0:
                 ops.append(("wait", "s", "filled"))
+            flush = b == 0 and it >= nslots
+            if flush:
+                ops.append(("wait", "s", ("flushed", it - nslots)))
+                op("s", f"reset slot {it}", [], [own("scount", it), own("fetch1", it)])
             op("s", f"shade {it}.{b}", ["hrec", ray, f"pix{cur}", f"rscene{cur}", "thr", "rng", "shade_table",
-                                         own("scount", it)],
+                                         own("scount", it), own("acc", it)] + (["color"] if flush else []),
                ["rng", "thr", own("acc", it), nxt, f"pix{cur ^ 1}", f"rscene{cur ^ 1}", "counts01", own("shq", it),
-                own("spix", it), own("sscene", it), own("scount", it), own("skey", it), "rep", f"rec{it}"])
+                own("spix", it), own("sscene", it), own("scount", it), own("skey", it), "rep", f"rec{it}"]
+               + (["color"] if flush else []))
             op("s", f"reset queue {it}.{b}", [], ["counts01", "fetch0"])
             sb = "s"
             if b == depth:
@@ -159,8 +158,7 @@ def schedule(spp, depth, nslots, slot_buffers=None):
         ops.append(("record", "a", ("flushed", it)))
     ops.append(("record", "a", "done"))
     ops.append(("wait", "s", "done"))
-    for it in range(max(0, spp - nslots), spp):
-        op("s", f"flush {it}", [own("acc", it), "color"], ["color"])
+    op("s", "flush last", [own("acc", it) for it in range(max(0, spp - nslots), spp)] + ["color"], ["color"])
     op("s", "resolve", ["color"], ["out"])
     return ops
 
