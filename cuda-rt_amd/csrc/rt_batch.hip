@@ -4,6 +4,7 @@
 
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <unordered_set>
 #include <vector>
@@ -30,6 +31,7 @@ struct rt_batch {
   size_t ws_bytes = 0, shade_bytes = 0;
   size_t npix = 0, ntiles = 0, nscenes = 0;
   int max_spp = 0, max_depth = 0;
+  std::vector<uint32_t> live_tiles;  // per sample k: tiles of the scenes with more than k samples (a prefix)
   rt_tuning tuning;
   hipStream_t streams[kBuildStreams] = {};
   hipEvent_t fork = nullptr;
@@ -78,11 +80,27 @@ rt_status rt_batch_create(rt_ctx* ctx, const rt_batch_item* items, int32_t count
     shade_off[i] = shade_bytes;
     shade_bytes += rt_align256(rt_shade_table_bytes((size_t)items[i].mesh.n));
     pix_off[i] = b->npix;
-    tile_off[i] = b->ntiles;
     b->npix += (size_t)c.width * c.height;
-    b->ntiles += (size_t)((c.width + 15) / 16) * ((c.height + 15) / 16);
     b->max_spp = items[i].opts.spp > b->max_spp ? items[i].opts.spp : b->max_spp;
     b->max_depth = items[i].opts.depth > b->max_depth ? items[i].opts.depth : b->max_depth;
+  }
+  // Tiles go to the scenes in order of decreasing sample count (stable), so
+  // the tiles that take sample k are a prefix: later samples launch their
+  // camera and first shading passes over that prefix only.
+  std::vector<int> by_spp(count);
+  for (int i = 0; i < count; ++i) by_spp[i] = i;
+  std::stable_sort(by_spp.begin(), by_spp.end(),
+                   [&](int x, int y) { return items[x].opts.spp > items[y].opts.spp; });
+  for (int i : by_spp) {
+    const rt_camera& c = items[i].cam;
+    tile_off[i] = b->ntiles;
+    b->ntiles += (size_t)((c.width + 15) / 16) * ((c.height + 15) / 16);
+  }
+  b->live_tiles.assign(b->max_spp, 0u);
+  for (int i = 0; i < count; ++i) {
+    const rt_camera& c = items[i].cam;
+    const uint32_t nt = (uint32_t)(((c.width + 15) / 16) * ((c.height + 15) / 16));
+    for (int k = 0; k < items[i].opts.spp; ++k) b->live_tiles[k] += nt;
   }
   if (b->npix >= (1ull << 32) || b->ntiles * 256 >= (1ull << 32) || ntris >= (1ull << 32)) {
     rt_batch_destroy(b);
@@ -179,7 +197,8 @@ rt_status rt_batch_render(rt_batch* b, uint64_t* counters_dev, void* stream) {
     RT_HIP(b->ctx, rt_ensure_layout(it.bvh, b->tuning.node_layout, (hipStream_t)stream));
   RT_HIP(b->ctx, rt_launch_wavefront(b->ws, b->npix, b->ntiles, b->nscenes, b->max_spp, b->max_depth,
                                      b->ctx->num_cus, &b->tuning, counters_dev, nullptr, (hipStream_t)stream,
-                                     nullptr, nullptr, nullptr, &b->ctx->timer, rt_ctx_overlap(b->ctx)));
+                                     nullptr, nullptr, nullptr, &b->ctx->timer, rt_ctx_overlap(b->ctx),
+                                     b->live_tiles.data()));
   return RT_OK;
 }
 

@@ -291,7 +291,8 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
                                uint32_t* overflow, hipStream_t s, int32_t* records = nullptr,
                                double* color64 = nullptr, const void* single_scene = nullptr,
-                               rt_kernel_timer* timer = nullptr, const rt_overlap* ov = nullptr);
+                               rt_kernel_timer* timer = nullptr, const rt_overlap* ov = nullptr,
+                               const uint32_t* live_tiles = nullptr);
 // Differentiable render (rt_diff.hip).
 hipError_t rt_launch_zero_f64(double* p, size_t n, hipStream_t s);
 hipError_t rt_launch_resolve(const float* big, int32_t W, int32_t H, int32_t ss, bool reference_divisor, float* out,

@@ -2489,7 +2489,8 @@ static hipError_t timer_mark(rt_kernel_timer* T, int kind, bool start, hipStream
 hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size_t nscenes, int max_spp,
                                int max_depth, int num_cus, const rt_tuning* tuning, uint64_t* counters,
                                uint32_t* overflow, hipStream_t s, int32_t* records, double* color64,
-                               const void* single_scene, rt_kernel_timer* timer, const rt_overlap* ov) {
+                               const void* single_scene, rt_kernel_timer* timer, const rt_overlap* ov,
+                               const uint32_t* live_tiles) {
   // Wp[k % kSampleSlots] is the state sample k's passes see (its slot's
   // accumulator, shadow queue and shadow-sort buffers).
   WfState Wp[kSampleSlots];
@@ -2505,13 +2506,14 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   const bool multi = single_scene == nullptr;
   SceneDesc D1{};
   if (!multi) D1 = *(const SceneDesc*)single_scene;
-#define RT_WF(STREAM, STATE, KERNEL, ...)                                        \
+#define RT_WFG(STREAM, GRID, STATE, KERNEL, ...)                                 \
   do {                                                                           \
     if (multi)                                                                   \
-      KERNEL<true><<<tiles, kThreads, 0, STREAM>>>(STATE, D1, __VA_ARGS__);      \
+      KERNEL<true><<<(GRID), kThreads, 0, STREAM>>>(STATE, D1, __VA_ARGS__);     \
     else                                                                         \
-      KERNEL<false><<<tiles, kThreads, 0, STREAM>>>(STATE, D1, __VA_ARGS__);     \
+      KERNEL<false><<<(GRID), kThreads, 0, STREAM>>>(STATE, D1, __VA_ARGS__);    \
   } while (0)
+#define RT_WF(STREAM, STATE, KERNEL, ...) RT_WFG(STREAM, tiles, STATE, KERNEL, __VA_ARGS__)
 #define RT_TRY(X)                          \
   do {                                     \
     hipError_t e_ = (X);                   \
@@ -2558,7 +2560,10 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   for (int it = 0; it < max_spp; ++it) {
     const int par = it % kSampleSlots;
     const WfState& W = Wp[par];
-    RT_WF(s, W, k_wf_camera, it, (uint32_t)cap);
+    // tiles still sampling: with live_tiles (a batch whose scenes' tiles are
+    // ordered by sample count, most first) a prefix of the tiles
+    const int live = live_tiles ? (int)live_tiles[it] : tiles;
+    RT_WFG(s, live, W, k_wf_camera, it, (uint32_t)live * kThreads);
     int cur = 0;
     for (int bounce = 0; bounce <= max_depth; ++bounce) {
       RayIn R{W.ray[cur], W.ray[cur] + 3 * cap, 1, cap, &W.counts[cur], 0, nullptr};
@@ -2576,7 +2581,8 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
         if (ov) RT_TRY(hipStreamWaitEvent(s, ov->flushed[par], 0));
         k_wf_reset<<<1, 256, 0, s>>>(W.counts, 1u << (2 + par), W.fetch[1], kFetchWords);
       }
-      RT_WF(s, W, k_wf_shade, bounce, cur, it, cnt, bounce == 0 && it >= kSampleSlots ? it - kSampleSlots : -1);
+      RT_WFG(s, bounce == 0 ? live : tiles, W, k_wf_shade, bounce, cur, it, cnt,
+             bounce == 0 && it >= kSampleSlots ? it - kSampleSlots : -1);
       // the queue traced above is consumed: clear its count and the fetch counters
       k_wf_reset<<<1, 256, 0, s>>>(W.counts, 1u << cur, W.fetch[0], kFetchWords);
       // earlier bounces share the accumulator with the next bounce's shading
@@ -2630,6 +2636,7 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     k_wf_resolve<false><<<tiles, kThreads, 0, s>>>(W0, D1);
   if (work) k_wf_counters_out<<<1, 64, 0, s>>>(W0.rep, cnt, T ? T->dev_work : nullptr);
 #undef RT_WF
+#undef RT_WFG
 #undef RT_TRY
   return hipGetLastError();
 }
