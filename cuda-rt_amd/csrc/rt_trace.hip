@@ -1681,6 +1681,16 @@ __global__ __launch_bounds__(kThreads) void k_wf_init(WfState W, SceneDesc D1) {
   if (!tile_pixel<MULTI>(W, D1, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
   const RenderParams& P = scene_of<MULTI>(W, D1, s).P;
   W.rng[gp] = rt::wang_hash(rt::seed_index(px, py, P.W) ^ P.seed);
+}
+
+// The image sum starts at zero. It is first added to by the shading pass of
+// sample kSampleSlots (or the final flush), so this pass runs on the
+// auxiliary stream, off the caller's path to the first closest-hit pass.
+template <bool MULTI>
+__global__ __launch_bounds__(kThreads) void k_wf_zero_color(WfState W, SceneDesc D1) {
+  int s, px, py;
+  uint32_t gp;
+  if (!tile_pixel<MULTI>(W, D1, blockIdx.x, threadIdx.x, s, px, py, gp)) return;
   stv(W.color, gp, W.npix, mk(0, 0, 0));
 }
 
@@ -2544,6 +2554,10 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     k_shade_fill<false><<<std::min(cdiv(D1.B.n, 256), 4096), 256, 0, sa>>>(W0.scenes, 1, D1);
   }
   if (ov) RT_TRY(hipEventRecord(ov->filled, sa));
+  if (multi)
+    k_wf_zero_color<true><<<tiles, kThreads, 0, sa>>>(W0, D1);
+  else
+    k_wf_zero_color<false><<<tiles, kThreads, 0, sa>>>(W0, D1);
   const int grid = persistent_grid(num_cus, t, (uint32_t)cap);
   uint32_t* ovf = overflow ? overflow : W0.counts + 8;
   if (multi)

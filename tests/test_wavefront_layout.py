@@ -115,7 +115,8 @@ def schedule(spp, depth, nslots, slot_buffers=None):
     ops.append(("wait", "a", "start"))
     op("a", "shade_fill", ["scenes"], ["shade_table"])
     ops.append(("record", "a", "filled"))
-    op("s", "init", [], ["counts01", "rep", "shist[0]", "rng", "color"] + [f"scount[{j}]" for j in range(nslots)])
+    op("a", "zero color", [], ["color"])
+    op("s", "init", [], ["counts01", "rep", "shist[0]", "rng"] + [f"scount[{j}]" for j in range(nslots)])
     op("s", "reset fetch0", [], ["fetch0"])
     for j in range(nslots):
         op("s", f"reset fetch1[{j}]", [], [f"fetch1[{j}]"])
