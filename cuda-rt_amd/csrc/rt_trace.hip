@@ -1723,7 +1723,6 @@ __global__ __launch_bounds__(kThreads) void k_wf_camera(WfState W, SceneDesc D1,
   stv(rr, 0, W.cap, o);
   stv(rr + 3 * W.cap, 0, W.cap, d);
   W.pix[0][slot] = gp;
-  stv(W.thr, gp, W.npix, mk(1, 1, 1));
 }
 
 // Shading table of each scene (blockIdx.y), in triangle order so the
@@ -1788,7 +1787,8 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
   if (live) {
     sc = MULTI ? W.rscene[cur][i] : 0;
     const SceneDesc& D = scene_of<MULTI>(W, D1, sc);
-    thr = ldv(W.thr, p, W.npix);
+    // a primary ray's throughput is 1 (the camera pass writes none)
+    thr = bounce == 0 ? mk(1, 1, 1) : ldv(W.thr, p, W.npix);
     const HitRec32 hr = W.hrec[i];
     const int pos = hr.pos;
     if (W.rec) W.rec[rec_index(W, p, sample, bounce)] = make_int2(pos, -1);
@@ -1841,7 +1841,7 @@ __global__ __launch_bounds__(kThreads) void k_wf_shade(WfState W, SceneDesc D1, 
     stv(e + 3 * c, 0, c, S.L);
     stv(e + 6 * c, 0, c, S.I1);
     stv(e + 9 * c, 0, c, S.ds);
-    stv(e + 12 * c, 0, c, thr);
+    if (bounce > 0) stv(e + 12 * c, 0, c, thr);  // first bounce: 1, not stored
     e[15 * c] = S.fo;
   }
   uint32_t ns = block_append(&W.counts[cur ^ 1], want_next, s_tmp);
@@ -2052,7 +2052,8 @@ __global__ __launch_bounds__(kThreads) void k_wf_shadow_finish(WfState W, SceneD
   d3 I = shade_finish(D.P, ldv(e + 6 * c, 0, c), e[15 * c], ldv(e + 9 * c, 0, c),
                       shadow_factor(D.P, D.B, D.mats, D.M, q, s.pos >= 0, s));
   uint32_t p = W.spix[i];
-  stv(W.acc, p, W.npix, ldv(W.acc, p, W.npix) + rt::hadamard(ldv(e + 12 * c, 0, c), I));
+  const d3 thr = bounce == 0 ? mk(1, 1, 1) : ldv(e + 12 * c, 0, c);
+  stv(W.acc, p, W.npix, ldv(W.acc, p, W.npix) + rt::hadamard(thr, I));
   if (W.rec) W.rec[rec_index(W, p, sample, bounce)].y = occluder_code(D, q, s);
 }
 

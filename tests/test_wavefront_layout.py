@@ -124,7 +124,7 @@ def schedule(spp, depth, nslots, slot_buffers=None):
             op("s", f"reset shist[{j}]", [], [f"shist[{j}]"])
     flushed = {}
     for it in range(spp):
-        op("s", f"camera {it}", ["rng"], ["rng", "ray0", "pix0", "rscene0", "thr", "counts01"])
+        op("s", f"camera {it}", ["rng"], ["rng", "ray0", "pix0", "rscene0", "counts01"])
         cur = 0
         for b in range(depth + 1):
             ray, nxt = f"ray{cur}", f"ray{cur ^ 1}"
