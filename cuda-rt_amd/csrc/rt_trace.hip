@@ -634,6 +634,11 @@ struct RayIn {
   const uint32_t* count_dev;  // number of rays (device), or nullptr
   uint32_t count;             // used when count_dev is nullptr
   const uint32_t* perm;       // optional fetch order: the k-th ray fetched is ray perm[k]
+  // Any hit only: a second queue in the same launch. Its *count_dev2 rays
+  // follow the first queue's in fetch order, and each of its buffers (rays,
+  // perm, scene, rec16) lies slot_bytes after the first queue's.
+  const uint32_t* count_dev2;
+  size_t slot_bytes;
 };
 
 // The wavefront passes keep each traversal result in one record, written
@@ -744,7 +749,8 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
                                                                unsigned long long* stats) {
   constexpr bool WIDE = LAYOUT != 2, QUANT = LAYOUT == 5 || LAYOUT == 8;
   __shared__ int s_stack[RING * THREADS];
-  const uint32_t nrays = R.count_dev ? *R.count_dev : R.count;
+  const uint32_t nrays0 = R.count_dev ? *R.count_dev : R.count;
+  const uint32_t nrays = nrays0 + (ANY && R.count_dev2 ? *R.count_dev2 : 0u);
   const uint32_t kFetchChunk = fetch_chunk;
   BvhView B = views[0];
   int n = B.n;
@@ -811,14 +817,21 @@ __global__ __launch_bounds__(THREADS) void k_trace_persistent(const BvhView* __r
         }
       }
       if (myid != 0xffffffffu) {
-        kid = myid;
-        rid = R.perm ? R.perm[myid] : myid;
+        // second queue of a paired any-hit launch: its buffers are the
+        // first queue's moved by slot_bytes (a multiple of 256)
+        size_t so = 0;
+        if (ANY && myid >= nrays0) {
+          myid -= nrays0;
+          so = R.slot_bytes;
+        }
+        kid = myid + (uint32_t)(so / sizeof(HitRec16));
+        rid = R.perm ? ((const uint32_t*)((const char*)R.perm + so))[myid] : myid;
         if (MULTI) {
-          B = views[scene[rid]];
+          B = views[((const uint16_t*)((const char*)scene + so))[rid]];
           n = B.n;
         }
-        const double* po = R.orig + (size_t)rid * R.stride;
-        const double* pd = R.dir + (size_t)rid * R.stride;
+        const double* po = (const double*)((const char*)R.orig + so) + (size_t)rid * R.stride;
+        const double* pd = (const double*)((const char*)R.dir + so) + (size_t)rid * R.stride;
         r = make_ray(mk(ldgd(po), ldgd(po + R.comp), ldgd(po + 2 * R.comp)),
                      mk(ldgd(pd), ldgd(pd + R.comp), ldgd(pd + 2 * R.comp)));
         if (WIDE) rf = make_rayf(r, B.node_box);
@@ -2320,6 +2333,13 @@ hipError_t rt_launch_trace(const rt_bvh* b, const rt_mesh_view* m, const rt_rays
 constexpr int kSampleSlots = RT_SAMPLE_SLOTS;
 // scount = counts + 2 + slot stays below counts[8], the batch overflow count
 static_assert(kSampleSlots >= 2 && kSampleSlots <= kMaxSampleSlots && 2 + kMaxSampleSlots <= 8, "sample slots");
+// The last bounce's any-hit passes of samples 2j and 2j + 1 (slots 0 and 1)
+// run as one launch: a launch ends with a drain in which few lanes still
+// have rays, and one launch over both samples' shadow rays pays it once.
+#ifndef RT_PAIR_SHADOWS
+#define RT_PAIR_SHADOWS 1
+#endif
+constexpr bool kPairShadows = RT_PAIR_SHADOWS != 0 && kSampleSlots == 2;
 
 enum WfShared {
   kRScenes, kRViews, kRTileScene, kRRng, kRColor, kRThr, kRRay0, kRRay1, kRPix0, kRPix1, kRRscene0, kRRscene1,
@@ -2513,6 +2533,13 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
     W.color64 = color64;
   }
   const WfState& W0 = Wp[0];
+  // A paired any-hit launch addresses slot 1's shadow buffers as slot 0's
+  // moved by one offset, and its records by fetch index + offset / 16.
+  const size_t pair_bytes = (size_t)((const char*)Wp[1].shq - (const char*)Wp[0].shq);
+  const auto moved = [&](const void* a, const void* b) { return (size_t)((const char*)b - (const char*)a); };
+  const bool pair = kPairShadows && pair_bytes % 256 == 0 && moved(Wp[0].sperm, Wp[1].sperm) == pair_bytes &&
+                    moved(Wp[0].srec, Wp[1].srec) == pair_bytes && moved(Wp[0].sscene, Wp[1].sscene) == pair_bytes &&
+                    pair_bytes / sizeof(HitRec16) + ntiles * kThreads < (1ull << 31);
   rt_kernel_timer* T = timer && timer->enabled ? timer : nullptr;
   const bool multi = single_scene == nullptr;
   SceneDesc D1{};
@@ -2535,12 +2562,14 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
   const size_t cap = ntiles * kThreads;
   const int tiles = (int)ntiles;
   const bool work = cnt != nullptr;  // count work only when the caller asked for counters
-  // The last bounce's shadow sort, any-hit pass, shadow resolve and flush of
+  // The last bounce's shadow sort, any-hit pass and shadow resolve of
   // sample k run on the auxiliary stream, overlapping the passes of the
   // next samples on s: they share no buffer but the replicated counters
-  // (atomics) and the colour sum, which only the flushes (all on the
-  // auxiliary stream, in sample order) write. Sample k + kSampleSlots
-  // reuses sample k's slot, so its camera pass waits for sample k's flush.
+  // (atomics). Sample k + kSampleSlots reuses sample k's slot, so its first
+  // shading pass (which adds sample k's path sums to the colour sum) waits
+  // for sample k's shadow resolve. With paired any-hit launches, samples
+  // 2j and 2j + 1's shadow rays are traced together after sample 2j + 1's
+  // shading, beside sample 2j + 2's camera and closest-hit passes.
   // The shading table is first read by sample 0's shading pass, so it is
   // filled on the auxiliary stream while sample 0's camera and closest-hit
   // passes run.
@@ -2607,6 +2636,12 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
         RT_TRY(hipEventRecord(ov->shaded, s));
         RT_TRY(hipStreamWaitEvent(sa, ov->shaded, 0));
       }
+      // Paired samples: the last bounce's any-hit pass of sample 2j waits
+      // for sample 2j + 1's and runs in the same launch (one drain instead
+      // of two); both samples' shadow finishes follow it.
+      const bool last = bounce == max_depth;
+      const bool defer = last && pair && par == 0 && it + 1 < max_spp;
+      const bool paired = last && pair && par == 1;
       RayIn SR{W.shq, W.shq + 3 * cap, 1, cap, W.scount, 0, kShadowSort ? W.sperm : nullptr};
       if (kShadowSort && !multi) {
         const int sort_blocks = cdiv((long)cap, kSortSpan);
@@ -2619,11 +2654,22 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
         k_msort_scan<<<1, kScanThreads, 0, sb>>>(W);
         k_msort_scatter<<<sort_blocks, kThreads, 0, sb>>>(W);
       }
-      HitOut SO{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, W.srec};
+      if (defer) {
+        cur ^= 1;
+        continue;
+      }
+      // a paired launch reads both slots' queues from slot 0's buffers
+      const WfState& WA = paired ? Wp[0] : W;
+      if (paired) {
+        SR = RayIn{WA.shq, WA.shq + 3 * cap, 1, cap, WA.scount, 0, kShadowSort ? WA.sperm : nullptr,
+                   Wp[1].scount, pair_bytes};
+      }
+      HitOut SO{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, WA.srec};
       if (T) RT_TRY(timer_mark(T, 1, true, sb));
-      launch_persistent<true, true>(t, grid, sb, W.views, nscenes > 1 ? W.sscene : nullptr, SR, 0.0, SO, W.fetch[1],
-                              ovf, work ? W.rep + 5 : nullptr);
+      launch_persistent<true, true>(t, grid, sb, WA.views, nscenes > 1 ? WA.sscene : nullptr, SR, 0.0, SO,
+                                    WA.fetch[1], ovf, work ? WA.rep + 5 : nullptr);
       if (T) RT_TRY(timer_mark(T, 1, false, sb));
+      if (paired) RT_WF(sb, Wp[0], k_wf_shadow_finish, it - 1, bounce, cnt);
       RT_WF(sb, W, k_wf_shadow_finish, it, bounce, cnt);
       // the last bounce's shadow count and fetch counters are cleared before
       // the next sample on this slot is shaded
@@ -2631,7 +2677,10 @@ hipError_t rt_launch_wavefront(void* workspace, size_t npix, size_t ntiles, size
       cur ^= 1;
     }
     // sample it's path sums are complete: the first shading pass of sample
-    // it + kSampleSlots adds them to the image
+    // it + kSampleSlots adds them to the image (a deferred sample's are
+    // complete with its pair's)
+    if (ov && pair && par == 0 && it + 1 < max_spp) continue;
+    if (ov && pair && par == 1) RT_TRY(hipEventRecord(ov->flushed[0], sa));
     if (ov) RT_TRY(hipEventRecord(ov->flushed[par], sa));
   }
   if (ov) {
