@@ -575,7 +575,7 @@ def main():
 
     roof = kernel_roofline("k_trace_persistent<closest hit, 4-wide quantized> (one launch = one sample's primary rays)",
                            kstats["closest"], kwork["closest"])
-    roof_any = kernel_roofline("k_trace_persistent<any hit, 4-wide quantized> (one launch = one sample's shadow rays)",
+    roof_any = kernel_roofline("k_trace_persistent<any hit, 4-wide quantized> (one launch = two samples' shadow rays)",
                                kstats["any"], kwork["any"])
     if roof is None:
         # megakernel: one k_render launch per step does all traversal and shading
@@ -587,8 +587,8 @@ def main():
     else:
         roof["traffic"] = committed_traffic("closest", args)
         roof_any["traffic"] = committed_traffic("any", args)
-        roof["note"] = ("avg_launch_ms is the launch's span in the timed steps, where it shares the GPU with the "
-                        "previous sample's any-hit launch; roofline_isolated times the same kernel alone")
+        roof["note"] = ("avg_launch_ms is the launch's span in the timed steps, where it can share the GPU with the "
+                        "any-hit launch of the two samples before it; roofline_isolated times the same kernel alone")
     roof_iso = None
     if iso is not None:
         roof_iso = {"closest": kernel_roofline("k_trace_persistent<closest hit> alone", iso["closest"],
